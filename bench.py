@@ -1,0 +1,335 @@
+"""Headline benchmark: 7-parameter sphere fits/s at 201x201 crops on one GPU.
+
+Workload (one "step"): 4096 features, every crop 201x201, parameters drawn
+as the config-3 distribution (a_p U[0.4, 3.0] um, n_p U[1.38, 1.65], k_p
+U[0, 0.01], z_p U[50, 400] px, x/y at the crop centre +-5 px, alpha
+U[0.8, 1.2]), Gaussian noise sigma = 0.05, starts perturbed by +-2 px,
+x(1 +- 0.05) a_p, +-0.02 n_p.  One step fits all 4096 with the batched
+device-resident LM (forward-mode Jacobian kernel, FP64 solve, HIP graph per
+iteration, <= 50 iterations), from the starting parameters already in HBM to
+the results copied back to the host.
+
+Also measured:
+  fd      the plain GPU finite-difference form (hipOptimizer: one feature at
+          a time, host MINPACK LM with forward differences, residuals copied
+          to the host per evaluation) on a sample of features -> speedup;
+  roofline  the Jacobian kernel (dominant) and the trial-forward kernel,
+          timed with HIP events on the stream they run on, against the FP32
+          VALU peak, with the survey's algorithmic FLOP counts;
+  cpu_baseline  the FP64 oracle (numpy model + scipy MINPACK LM) on one
+          feature on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+INSTRUMENT = dict(wavelength=0.447, magnification=0.048, n_m=1.340)
+PEAK_FP32 = 157.3        # TFLOP/s, MI355X FP32 vector (packed)
+PEAK_FP32_UNPACKED = 78.6
+PEAK_HBM = 8000.0        # GB/s
+PARAMS = ('x_p', 'y_p', 'z_p', 'a_p', 'n_p', 'k_p', 'alpha')
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def draw_features(n: int, side: int, seed: int):
+    rng = np.random.default_rng(seed)
+    truth = np.zeros((n, 7))
+    start = np.zeros((n, 7))
+    c = (side - 1) / 2
+    for i in range(n):
+        a = rng.uniform(0.4, 3.0)
+        npart = rng.uniform(1.38, 1.65)
+        k = rng.uniform(0., 0.01)
+        z = rng.uniform(50., 400.)
+        x = c + rng.uniform(-5, 5)
+        y = c + rng.uniform(-5, 5)
+        al = rng.uniform(0.8, 1.2)
+        truth[i] = (x, y, z, a, npart, k, al)
+        sg = rng.choice([-1., 1.], size=5)
+        start[i] = (x + 2 * sg[0], y + 2 * sg[1], z + 2 * sg[2],
+                    a * (1 + 0.05 * sg[3]), npart + 0.02 * sg[4], k, al)
+    return truth, start
+
+
+def nmax_of(a_p, n_p, k_p):
+    from pylorenzmie_amd._lmz import wiscombe_yang
+    k = 2 * math.pi / INSTRUMENT['wavelength'] * INSTRUMENT['n_m']
+    return wiscombe_yang(k * a_p, (n_p + 1j * k_p) / INSTRUMENT['n_m'])
+
+
+def make_data(truth, side, sigma, seed):
+    """Noisy synthetic holograms on the device (FP32 kernel + torch noise)."""
+    import torch
+    from pylorenzmie_amd.theory import Instrument
+    from pylorenzmie_amd.theory.batch import HologramBatch
+    from pylorenzmie_amd import _lmz
+    inst = Instrument(**INSTRUMENT)
+    caps = [_lmz.nmax_cap(p[3], p[4], p[5], inst.n_m, inst.wavelength, 0.)
+            for p in truth]
+    hb = HologramBatch([(side, side)] * len(truth), caps, instrument=inst)
+    hb.load(truth)
+    clean = hb.compute().clone()
+    g = torch.Generator(device='cuda')
+    g.manual_seed(seed)
+    noise = torch.randn(clean.shape, generator=g, device='cuda',
+                        dtype=torch.float32)
+    return clean + sigma * noise
+
+
+def run_batched(args, truth, start, data_dev):
+    import torch
+    from pylorenzmie_amd import _lmz
+    from pylorenzmie_amd.theory import Instrument
+    from pylorenzmie_amd.analysis.lm import LMBatch, LMSettings
+    inst = Instrument(noise=args.sigma, **INSTRUMENT)
+    side = args.side
+    crops = [dict(data=np.zeros((side, side), np.float32), width=side,
+                  origin=(0., 0., 0.),
+                  nmax_cap=_lmz.nmax_cap(p[3], p[4], p[5], inst.n_m,
+                                         inst.wavelength))
+             for p in start]
+    lm = LMBatch(crops, inst)
+    lm.data.copy_(data_dev)
+    settings = LMSettings(max_iter=50, ftol=args.ftol, xtol=args.xtol,
+                          gtol=args.gtol)
+    p0 = torch.zeros_like(lm.params)
+    p0[:, :7] = torch.from_numpy(start).to(p0.device)
+
+    def step():
+        lm.params.copy_(p0)
+        lm.run(settings, PARAMS)
+        return lm.fetch()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        params, res = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return lm, dt, params, res, settings
+
+
+def kernel_times(lm, settings, args):
+    """Average duration of the Jacobian and trial-forward kernels over the
+    whole batch, launched on torch's current stream and timed with HIP
+    events on that stream."""
+    import torch
+    from pylorenzmie_amd import _lmz
+    lib = _lmz.load_library()
+    b = lm.batch
+    dev = lm.data.device
+    coef = torch.zeros(b.coef_floats(), dtype=torch.float32, device=dev)
+    dcoef = torch.zeros(b.dcoef_floats(), dtype=torch.float32, device=dev)
+    nmax = torch.zeros(2 * b.nfeat, dtype=torch.int32, device=dev)
+    work = torch.zeros(b.work_doubles, dtype=torch.float64, device=dev)
+    partials = torch.zeros(36 * b.ntiles, dtype=torch.float64, device=dev)
+    chi2 = torch.zeros(b.ntiles, dtype=torch.float64, device=dev)
+    s = _lmz.stream_handle()
+    _lmz.check(lib.lmz_mie_ab(b.handle, _lmz.ptr(lm.params), None, _lmz.ptr(nmax),
+                              _lmz.ptr(coef), _lmz.ptr(dcoef), None, _lmz.ptr(work), s))
+    flags = 3
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+
+    def jac():
+        _lmz.check(lib.lmz_jacobian(b.handle, _lmz.ptr(lm.params), None, _lmz.ptr(nmax),
+                                    _lmz.ptr(coef), _lmz.ptr(dcoef), None, flags,
+                                    _lmz.ptr(lm.data), None, None, _lmz.ptr(partials),
+                                    None, s))
+
+    def fwd():
+        _lmz.check(lib.lmz_hologram(b.handle, _lmz.ptr(lm.params), None, _lmz.ptr(nmax),
+                                    _lmz.ptr(coef), None, flags, _lmz.ptr(lm.data),
+                                    None, _lmz.ptr(chi2), None, s))
+    jac()
+    fwd()
+    reps = args.kernel_reps
+    ev[0].record()
+    for _ in range(reps):
+        jac()
+    ev[1].record()
+    for _ in range(reps):
+        fwd()
+    ev[2].record()
+    torch.cuda.synchronize()
+    tj = ev[0].elapsed_time(ev[1]) / reps * 1e-3
+    tf = ev[1].elapsed_time(ev[2]) / reps * 1e-3
+    nm = nmax.cpu().numpy()[0::2].astype(float)
+    P = np.array(lm.npix, float)
+    flop_j = float(np.sum(P * (296. * nm + 480.)))
+    flop_f = float(np.sum(P * (64. * nm + 70.)))
+    return tj, tf, flop_j, flop_f, float(np.sum(P)), float(np.mean(nm))
+
+
+def run_fd(args, truth, start, data_host):
+    """Plain GPU FD form on a sample of features, sequentially."""
+    from pylorenzmie_amd.theory import LorenzMie, Sphere, Instrument
+    from pylorenzmie_amd.analysis import hipOptimizer, Hologram
+    side = args.side
+    n = min(args.fd_sample, len(start))
+    t0 = time.perf_counter()
+    for i in range(n):
+        inst = Instrument(noise=args.sigma, **INSTRUMENT)
+        s = Sphere(**dict(zip(PARAMS[:6], start[i, :6])))
+        m = LorenzMie(coordinates=LorenzMie.meshgrid((side, side)), particle=s,
+                      instrument=inst, alpha=start[i, 6])
+        opt = hipOptimizer(model=m)
+        opt.variables = list(PARAMS)
+        opt.optimize(Hologram(data_host[i].astype(float)))
+    dt = time.perf_counter() - t0
+    return n / dt, n, dt
+
+
+def run_cpu(args, start, data_host):
+    """FP64 oracle: numpy model + scipy MINPACK LM, as analysis/Optimizer.py
+    drives it, on one feature."""
+    from scipy.optimize import least_squares
+    from oracle import lorenzmie as ref
+    side = args.side
+    c = ref.meshgrid((side, side))
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        i = n % len(start)
+        d = data_host[i].ravel().astype(float)
+
+        def resid(v):
+            p = dict(zip(PARAMS[:6], v[:6]))
+            return (ref.hologram(c, p, alpha=v[6], **INSTRUMENT) - d) / args.sigma
+        least_squares(resid, start[i], method='lm', ftol=1e-4, xtol=1e-6,
+                      gtol=1e-6, max_nfev=2000, diff_step=1e-5, x_scale='jac')
+        n += 1
+        if time.perf_counter() - t0 > args.cpu_seconds or n >= args.cpu_max_fits:
+            break
+    dt = time.perf_counter() - t0
+    return n / dt, n, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--nfeat', type=int, default=4096)
+    ap.add_argument('--side', type=int, default=201)
+    ap.add_argument('--sigma', type=float, default=0.05)
+    ap.add_argument('--ftol', type=float, default=1e-9)
+    ap.add_argument('--xtol', type=float, default=1e-9)
+    ap.add_argument('--gtol', type=float, default=1e-10)
+    ap.add_argument('--fd-sample', type=int, default=8)
+    ap.add_argument('--cpu-seconds', type=float, default=15.)
+    ap.add_argument('--cpu-max-fits', type=int, default=4)
+    ap.add_argument('--kernel-reps', type=int, default=3)
+    ap.add_argument('--no-fd', action='store_true')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--seed', type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU: this build has no multi-GPU path '
+              '(features shard across GPUs with no communication; see '
+              'DESIGN.md).  Use --gpus 1.', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), 'bench.py needs a GPU'
+
+    truth, start = draw_features(args.nfeat, args.side, args.seed)
+    log(f'bench: generating {args.nfeat} synthetic {args.side}^2 holograms')
+    data_dev = make_data(truth, args.side, args.sigma, args.seed)
+    torch.cuda.synchronize()
+    log('bench: batched LM warmup + timed steps')
+    lm, dt, params, res, settings = run_batched(args, truth, start, data_dev)
+    ms_per_step = dt / args.steps * 1e3
+    fits_per_s = args.nfeat * args.steps / dt
+    status = res[:, 38].astype(int)
+    iters = res[:, 37]
+    ok = ((status & 7) != 0) & ((status & 48) == 0)
+    err = np.abs(params[:, :7] - truth)
+    log(f'bench: {fits_per_s:.1f} fits/s, {ms_per_step:.1f} ms/step, '
+        f'iters mean {iters.mean():.1f} max {iters.max():.0f}, '
+        f'converged {ok.mean():.3f}')
+
+    log('bench: kernel timing')
+    tj, tf, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
+    ach_j = flop_j / tj / 1e12
+    ach_f = flop_f / tf / 1e12
+    log(f'bench: jacobian {tj * 1e3:.2f} ms {ach_j:.1f} TFLOP/s; '
+        f'forward {tf * 1e3:.2f} ms {ach_f:.1f} TFLOP/s')
+
+    fd = None
+    if not args.no_fd:
+        log('bench: plain GPU FD form sample')
+        data_host = data_dev[: args.side * args.side * args.fd_sample].cpu().numpy()
+        data_host = data_host.reshape(-1, args.side, args.side)
+        rate, n, t = run_fd(args, truth, start, data_host)
+        fd = dict(value=rate, unit='fits/s', sample=f'{n} features, sequential',
+                  seconds=t)
+        log(f'bench: FD form {rate:.2f} fits/s')
+    cpu = None
+    if not args.no_cpu:
+        log('bench: CPU oracle sample')
+        data_host = data_dev[: args.side * args.side * 4].cpu().numpy()
+        data_host = data_host.reshape(-1, args.side, args.side)
+        os.environ.setdefault('OMP_NUM_THREADS', '1')
+        rate, n, t = run_cpu(args, start, data_host)
+        cpu = dict(value=rate, unit='fits/s', cores=1, kind='port',
+                   sample=f'{n} 7-parameter fit(s) of {args.side}^2 crops, '
+                          f'FP64 numpy oracle + scipy MINPACK LM (forward '
+                          f'differences), {t:.1f} s')
+        log(f'bench: CPU oracle {rate:.4f} fits/s')
+
+    line = {
+        'metric': '7-param sphere fits/sec at 201² crops, adjoint vs FD-Jacobian, 1 GPU',
+        'value': fits_per_s,
+        'unit': 'fits/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_per_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32 pixel terms, f64 sums/Mie/solve',
+        'data': 'synthetic (seeded config-3 parameter draws, sigma=0.05)',
+        'config': {'workload': f'batched 7-parameter LM, {args.nfeat} features, '
+                               f'{args.side}x{args.side} crops, <=50 iterations',
+                   'features': args.nfeat, 'side': args.side,
+                   'mean_nmax': nmean, 'ftol': args.ftol, 'xtol': args.xtol,
+                   'gtol': args.gtol},
+        'roofline': {'bound': 'valu', 'kernel': 'k_jacobian',
+                     'achieved': ach_j, 'peak': PEAK_FP32, 'unit': 'TFLOP/s',
+                     'frac': ach_j / PEAK_FP32,
+                     'frac_of_unpacked_peak': ach_j / PEAK_FP32_UNPACKED,
+                     'avg_launch_ms': tj * 1e3,
+                     'algorithmic_flop_per_launch': flop_j,
+                     'pixels_per_launch': npx,
+                     'traffic': None,
+                     'forward_kernel': {'achieved': ach_f, 'avg_launch_ms': tf * 1e3,
+                                        'algorithmic_flop_per_launch': flop_f}},
+        'fit_quality': {'converged_frac': float(ok.mean()),
+                        'iterations_mean': float(iters.mean()),
+                        'iterations_max': float(iters.max()),
+                        'median_abs_err': {k: float(np.median(err[:, i]))
+                                           for i, k in enumerate(PARAMS)}},
+        'fd_plain_gpu': fd,
+        'speedup_vs_fd': (fits_per_s / fd['value']) if fd else None,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

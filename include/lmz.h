@@ -1,0 +1,234 @@
+/*
+ * lmz.h -- C ABI of the MI355X (gfx950) Lorenz-Mie hologram and
+ * Levenberg-Marquardt library, liblmz.so.
+ *
+ * Conventions
+ *   - Every compute entry point is stream-ordered on the hipStream_t it is
+ *     given and takes DEVICE pointers for per-pixel and per-feature arrays.
+ *     The caller owns every buffer; the library allocates only the device
+ *     copy of a batch's feature table and tile map (lmz_batch_create) and a
+ *     capture stream/graph inside an lmz_lm_plan.
+ *   - Return value: 0 on success, LMZ_EINVAL for a bad argument (checked on
+ *     the host before any launch), LMZ_EHIP for a HIP runtime error.  The
+ *     message of the last failure on the calling thread is lmz_last_error().
+ *   - No exceptions cross the ABI; calls on distinct streams are independent.
+ *
+ * Data layout in HBM
+ *   params   double [nfeat][LMZ_NPARAM]  x_p y_p z_p [px], a_p [um], n_p, k_p,
+ *                                         alpha, (unused)
+ *   data     float  [npix_total]          normalized hologram pixels, features
+ *                                         packed back to back (pix_offset)
+ *   coords   float  [3][npix_total]       explicit coordinates x, y, z of
+ *                                         features with width == 0, relative
+ *                                         to the feature's origin; may be NULL
+ *                                         when every feature is a dense crop
+ *   field    float2 [3][npix_total]       complex field components
+ *   jac      float  [LMZ_NJAC][npix_total] dI/d(param), same parameter order
+ *   coef     float  [slots][coef_stride]  per order n: i^(n+1)(2n+1) a_n,
+ *                                         i E_n a_n, E_n b_n (complex), n,
+ *                                         n(n+1), then as doubles 1 + 1/n,
+ *                                         n(n+1)/2, (n+1)(n+2)/2, n and the
+ *                                         three coefficients again
+ *                                         (lmz_batch_sizes gives the strides)
+ *   dcoef    float  [slots][dcoef_stride] the three complex entries above
+ *                                         differentiated by a_p, n_p, k_p
+ *                                         (8 floats per parameter)
+ *   partials double [ntiles][36]          per pixel tile: upper triangle of
+ *                                         J^T J (28, row major), J^T r (7),
+ *                                         r^T r (1), with J and r scaled by
+ *                                         1/noise
+ */
+#ifndef LMZ_H
+#define LMZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMZ_VERSION 100
+#define LMZ_NPARAM 8       /* stride of params rows */
+#define LMZ_NJAC 7         /* x_p y_p z_p a_p n_p k_p alpha */
+#define LMZ_NRED 36        /* 28 + 7 + 1 */
+#define LMZ_TILE 1024      /* pixels per tile (256 threads x 4 pixels) */
+#define LMZ_MAX_WIDTH 8192 /* widest dense crop */
+
+#define LMZ_OK 0
+#define LMZ_EINVAL 1
+#define LMZ_EHIP 2
+
+/* flags */
+#define LMZ_CARTESIAN 1    /* project onto x, y, z (theory/LorenzMie.py:331) */
+#define LMZ_BOHREN 2       /* +z along propagation (theory/LorenzMie.py:249) */
+#define LMZ_ACCUMULATE 4   /* field += instead of field = */
+
+/* One particle and the pixels it is evaluated on.  Fields marked (out) are
+ * written by lmz_batch_create into the caller's array. */
+typedef struct lmz_feature {
+    int64_t pix_offset;    /* first pixel in the packed pixel arrays */
+    int32_t npix;          /* number of pixels (> 0) */
+    int32_t width;         /* > 0: dense row-major crop of this width, pixel
+                              (col, row) at origin + (col, row, 0);
+                              0: explicit coordinates in coords[] */
+    double origin[3];      /* dense: coordinates of pixel (0, 0);
+                              explicit: offset added to coords[] */
+    double n_m;            /* medium refractive index */
+    double wavelength;     /* vacuum wavelength [um] */
+    double magnification;  /* [um / px] */
+    double noise;          /* residual scale (analysis/Optimizer.py:260) */
+    int32_t nmax_cap;      /* orders reserved per coefficient buffer */
+    int32_t coef_offset;   /* (out) first coefficient slot */
+    int32_t tile_offset;   /* (out) first tile */
+    int32_t ntiles;        /* (out) ceil(npix / LMZ_TILE) */
+} lmz_feature;
+
+typedef struct lmz_batch lmz_batch;
+
+typedef struct lmz_batch_sizes {
+    int32_t nfeat;
+    int32_t ntiles;
+    int64_t npix_total;    /* max(pix_offset + npix) */
+    int32_t coef_buffers;  /* 1, or 2 for double-buffered LM trial tables */
+    int32_t coef_slots;    /* rows of coef / dcoef / ab_out */
+    int32_t cap_max;       /* largest nmax_cap */
+    int64_t work_doubles;  /* scratch of lmz_mie_ab */
+    int32_t coef_stride;   /* floats per coef row */
+    int32_t dcoef_stride;  /* floats per dcoef row */
+} lmz_batch_sizes;
+
+/* Validate a feature table on the host, fill its (out) fields, and upload it
+ * with the tile -> feature map.  coef_buffers is 1 for plain evaluation or 2
+ * for the double-buffered LM tables. */
+int lmz_batch_create(int32_t nfeat, lmz_feature *feat, int32_t coef_buffers,
+                     lmz_batch **out);
+int lmz_batch_destroy(lmz_batch *batch);
+int lmz_batch_query(const lmz_batch *batch, lmz_batch_sizes *sizes);
+
+/* Mie coefficients a_n, b_n of homogeneous spheres in FP64 and their FP32
+ * per-order tables for the pixel kernels.
+ * Replaces Sphere.ab / Sphere.mie_coefficients (theory/Sphere.py:70-89,
+ * 147-226) with n_max from Sphere.wiscombe_yang (theory/Sphere.py:91-123),
+ * and folds in the per-order prefactors of theory/LorenzMie.py:297-303.
+ *   sel      per-feature buffer index (0/1) or NULL for buffer 0
+ *   nmax_out int32 [nfeat * coef_buffers], entry f*coef_buffers + buffer
+ *   dcoef    NULL skips the a_p, n_p, k_p tangents
+ *   ab_out   NULL or double [slots][4] = a_n.re a_n.im b_n.re b_n.im, n >= 1
+ *   work     double [work_doubles] scratch */
+int lmz_mie_ab(const lmz_batch *batch, const double *params,
+               const int32_t *sel, int32_t *nmax_out, float *coef,
+               float *dcoef, double *ab_out, double *work,
+               hipStream_t stream);
+
+/* Per-order tables from coefficients computed elsewhere (any particle
+ * whose ab() is not a homogeneous sphere): ab_in double [slots][4] rows
+ * n = 1..nmax_in[f] of each feature, as ab_out of lmz_mie_ab.
+ * Replaces the per-order prefactors of theory/LorenzMie.py:297-303 applied
+ * to Particle.ab() (theory/Particle.py:97-119). */
+int lmz_coef_from_ab(const lmz_batch *batch, const double *ab_in,
+                     const int32_t *nmax_in, int32_t *nmax_out, float *coef,
+                     hipStream_t stream);
+
+/* Scattered field of every feature's particle at its pixels, FP32.
+ * Replaces LorenzMie.field / scattered_field / lorenzmie
+ * (theory/LorenzMie.py:135-181, 183-342). */
+int lmz_field(const lmz_batch *batch, const double *params,
+              const int32_t *sel, const int32_t *nmax, const float *coef,
+              const float *coords, int32_t flags, float *field,
+              hipStream_t stream);
+
+/* Hologram |x + alpha E_s|^2, or the residual (I - d) / noise when data is
+ * not NULL, with per-tile r^T r in chi2_tiles (double [ntiles], may be NULL).
+ * Replaces LorenzMie.hologram (theory/LorenzMie.py:113-133) and
+ * Optimizer._residuals (analysis/Optimizer.py:257-260).
+ * active: uint8 [nfeat] or NULL; inactive features are skipped. */
+int lmz_hologram(const lmz_batch *batch, const double *params,
+                 const int32_t *sel, const int32_t *nmax, const float *coef,
+                 const float *coords, int32_t flags, const float *data,
+                 float *out, double *chi2_tiles, const uint8_t *active,
+                 hipStream_t stream);
+
+/* Intensity of an accumulated field: out = |x + E|^2 per pixel, where the
+ * field already carries alpha (lmz_field writes alpha E).  Used for
+ * superposed particles (theory/LorenzMie.py:131-133 after :178-181). */
+int lmz_intensity(const lmz_batch *batch, const double *params,
+                  const float *field, int32_t flags, float *out,
+                  hipStream_t stream);
+
+/* Forward-mode Jacobian dI/d(x_p, y_p, z_p, a_p, n_p, k_p, alpha), FP32,
+ * recomputing the recurrences and their tangents per pixel.  Any of jac,
+ * resid, partials may be NULL (resid and partials need data).  Needs the
+ * dcoef tangent tables of lmz_mie_ab.
+ * Replaces jaxLorenzMie.jac (theory/jaxLorenzMie.py:354-395) and
+ * Optimizer._analytical_jac (analysis/Optimizer.py:251-255), extended to
+ * k_p and alpha. */
+int lmz_jacobian(const lmz_batch *batch, const double *params,
+                 const int32_t *sel, const int32_t *nmax, const float *coef,
+                 const float *dcoef, const float *coords, int32_t flags,
+                 const float *data, float *jac, float *resid,
+                 double *partials, const uint8_t *active,
+                 hipStream_t stream);
+
+/* Normal equations from a Jacobian and residual already in HBM:
+ * out double [nfeat][36] (J^T J upper triangle, J^T r, r^T r), with J
+ * scaled by 1/noise.  The reduction MINPACK performs inside lmder
+ * (scipy least_squares, called at analysis/Optimizer.py:221). */
+int lmz_normal_eq(const lmz_batch *batch, const float *jac,
+                  const float *resid, double *out, hipStream_t stream);
+
+/* Batched Levenberg-Marquardt over every feature of the batch
+ * (coef_buffers must be 2).  Replaces Optimizer.optimize
+ * (analysis/Optimizer.py:196-222) -> scipy least_squares(method='lm')
+ * with an on-device Marquardt iteration: forward-mode Jacobian, FP64 7x7
+ * Cholesky solve, trial forward, accept/reject, one HIP graph per
+ * iteration replayed up to max_iter times. */
+typedef struct lmz_lm_settings {
+    int32_t max_iter;      /* iteration cap (accepted + rejected steps) */
+    int32_t check_every;   /* replays between host checks for completion */
+    double ftol;           /* stop: actual and predicted relative chi2
+                              reduction both <= ftol */
+    double xtol;           /* stop: ||D dp|| <= xtol (||D p|| + xtol) */
+    double gtol;           /* stop: max_j |J_j^T r| / (|J_j| |r|) <= gtol */
+    double lambda0;        /* initial damping relative to diag(J^T J) */
+    uint8_t vary[LMZ_NJAC];/* 1 = fitted, 0 = held fixed */
+    uint8_t use_graph;     /* capture the iteration in a HIP graph */
+} lmz_lm_settings;
+
+/* per-feature LM result, double [nfeat][LMZ_NRES] */
+#define LMZ_NRES 40
+/* 0..27 J^T J at the solution (upper triangle), 28..34 J^T r, 35 chi2,
+   36 lambda, 37 iterations, 38 status, 39 Jacobian evaluations */
+
+/* status codes */
+#define LMZ_LM_RUNNING 0
+#define LMZ_LM_FTOL 1
+#define LMZ_LM_XTOL 2
+#define LMZ_LM_GTOL 4
+#define LMZ_LM_MAXITER 8
+#define LMZ_LM_STALLED 16
+#define LMZ_LM_NONFINITE 32
+
+typedef struct lmz_lm_plan lmz_lm_plan;
+
+/* Bytes of device workspace lmz_lm_run needs for this batch. */
+int lmz_lm_workspace_bytes(const lmz_batch *batch, size_t *bytes);
+int lmz_lm_plan_create(const lmz_batch *batch, lmz_lm_plan **out);
+int lmz_lm_plan_destroy(lmz_lm_plan *plan);
+
+/* params_io: double [nfeat][LMZ_NPARAM] device, start in / solution out.
+ * result:    double [nfeat][LMZ_NRES] device.
+ * workspace: device buffer of lmz_lm_workspace_bytes. */
+int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
+               const float *data, const lmz_lm_settings *settings,
+               double *params_io, double *result, void *workspace,
+               hipStream_t stream);
+
+const char *lmz_last_error(void);
+int lmz_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LMZ_H */

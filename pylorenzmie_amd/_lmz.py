@@ -1,0 +1,248 @@
+"""ctypes binding of liblmz.so (include/lmz.h) and device-memory plumbing.
+
+PyTorch provides device allocations and the current stream; every number
+on the path is computed by the HIP kernels in liblmz.so.  There is no CPU
+fallback: importing this module raises if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, 'liblmz.so')
+
+NPARAM = 8
+NJAC = 7
+NRED = 36
+NRES = 40
+TILE = 1024
+CARTESIAN = 1
+BOHREN = 2
+ACCUMULATE = 4
+
+LM_RUNNING, LM_FTOL, LM_XTOL, LM_GTOL = 0, 1, 2, 4
+LM_MAXITER, LM_STALLED, LM_NONFINITE = 8, 16, 32
+
+
+class LMZError(RuntimeError):
+    """Failure reported by liblmz (lmz_last_error)."""
+
+
+class Feature(ctypes.Structure):
+    """lmz_feature, include/lmz.h."""
+    _fields_ = [('pix_offset', ctypes.c_int64),
+                ('npix', ctypes.c_int32),
+                ('width', ctypes.c_int32),
+                ('origin', ctypes.c_double * 3),
+                ('n_m', ctypes.c_double),
+                ('wavelength', ctypes.c_double),
+                ('magnification', ctypes.c_double),
+                ('noise', ctypes.c_double),
+                ('nmax_cap', ctypes.c_int32),
+                ('coef_offset', ctypes.c_int32),
+                ('tile_offset', ctypes.c_int32),
+                ('ntiles', ctypes.c_int32)]
+
+
+class BatchSizes(ctypes.Structure):
+    """lmz_batch_sizes, include/lmz.h."""
+    _fields_ = [('nfeat', ctypes.c_int32),
+                ('ntiles', ctypes.c_int32),
+                ('npix_total', ctypes.c_int64),
+                ('coef_buffers', ctypes.c_int32),
+                ('coef_slots', ctypes.c_int32),
+                ('cap_max', ctypes.c_int32),
+                ('work_doubles', ctypes.c_int64),
+                ('coef_stride', ctypes.c_int32),
+                ('dcoef_stride', ctypes.c_int32)]
+
+
+class LMSettings(ctypes.Structure):
+    """lmz_lm_settings, include/lmz.h."""
+    _fields_ = [('max_iter', ctypes.c_int32),
+                ('check_every', ctypes.c_int32),
+                ('ftol', ctypes.c_double),
+                ('xtol', ctypes.c_double),
+                ('gtol', ctypes.c_double),
+                ('lambda0', ctypes.c_double),
+                ('vary', ctypes.c_uint8 * NJAC),
+                ('use_graph', ctypes.c_uint8)]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int32
+
+# name -> (restype, argtypes): every symbol include/lmz.h declares
+SIGNATURES = {
+    'lmz_batch_create': (_I, [_I, _P, _I, ctypes.POINTER(_P)]),
+    'lmz_batch_destroy': (_I, [_P]),
+    'lmz_batch_query': (_I, [_P, ctypes.POINTER(BatchSizes)]),
+    'lmz_mie_ab': (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'lmz_coef_from_ab': (_I, [_P, _P, _P, _P, _P, _P]),
+    'lmz_field': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P]),
+    'lmz_hologram': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
+    'lmz_intensity': (_I, [_P, _P, _P, _I, _P, _P]),
+    'lmz_jacobian': (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P,
+                          _P, _P]),
+    'lmz_normal_eq': (_I, [_P, _P, _P, _P, _P]),
+    'lmz_lm_workspace_bytes': (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    'lmz_lm_plan_create': (_I, [_P, ctypes.POINTER(_P)]),
+    'lmz_lm_plan_destroy': (_I, [_P]),
+    'lmz_lm_run': (_I, [_P, _P, _I, _P, ctypes.POINTER(LMSettings), _P, _P,
+                        _P, _P]),
+    'lmz_last_error': (ctypes.c_char_p, []),
+    'lmz_version': (_I, []),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load_library(path: str = LIBPATH) -> ctypes.CDLL:
+    """Load liblmz.so and declare its C ABI.  Raises if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise ImportError(
+                f'{path} is missing: build it with __graft_entry__.build() '
+                '(hipcc --offload-arch=gfx950); there is no CPU fallback')
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load_library().lmz_last_error().decode(errors='replace')
+        raise LMZError(f'liblmz error {rc}: {msg}')
+
+
+# ------------------------------------------------------------------ torch
+_torch = None
+
+
+def torch():
+    """Import torch lazily and require a GPU."""
+    global _torch
+    if _torch is None:
+        import torch as t
+        _torch = t
+    return _torch
+
+
+def device():
+    t = torch()
+    if not t.cuda.is_available():
+        raise RuntimeError('pylorenzmie_amd needs a ROCm GPU (gfx950); '
+                           'torch.cuda.is_available() is False')
+    return t.device('cuda', t.cuda.current_device())
+
+
+def stream_handle(stream=None) -> int:
+    t = torch()
+    s = stream if stream is not None else t.cuda.current_stream()
+    return s.cuda_stream
+
+
+def ptr(tensor) -> int | None:
+    """Device pointer of a tensor, or None."""
+    if tensor is None:
+        return None
+    return tensor.data_ptr()
+
+
+class Batch:
+    """Owner of an lmz_batch: the validated feature table on the device."""
+
+    def __init__(self, features: list[Feature] | ctypes.Array,
+                 coef_buffers: int = 1):
+        lib = load_library()
+        n = len(features)
+        arr = (Feature * n)(*features) if not isinstance(
+            features, ctypes.Array) else features
+        handle = _P()
+        check(lib.lmz_batch_create(n, ctypes.cast(arr, _P), coef_buffers,
+                                   ctypes.byref(handle)))
+        self._lib = lib
+        self.handle = handle
+        self.features = arr
+        sizes = BatchSizes()
+        check(lib.lmz_batch_query(handle, ctypes.byref(sizes)))
+        self.nfeat = sizes.nfeat
+        self.ntiles = sizes.ntiles
+        self.npix_total = sizes.npix_total
+        self.coef_buffers = sizes.coef_buffers
+        self.coef_slots = sizes.coef_slots
+        self.cap_max = sizes.cap_max
+        self.work_doubles = sizes.work_doubles
+        # floats per row of the coefficient tables, as the library lays them
+        self.coef_stride = sizes.coef_stride
+        self.dcoef_stride = sizes.dcoef_stride
+
+    def coef_floats(self) -> int:
+        return self.coef_slots * self.coef_stride
+
+    def dcoef_floats(self) -> int:
+        return self.coef_slots * self.dcoef_stride
+
+    def lm_workspace_bytes(self) -> int:
+        n = ctypes.c_size_t()
+        check(self._lib.lmz_lm_workspace_bytes(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, 'handle', None):
+            self._lib.lmz_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_feature(npix, width, origin, instrument, nmax_cap, pix_offset=0,
+                 noise=None) -> Feature:
+    f = Feature()
+    f.pix_offset = int(pix_offset)
+    f.npix = int(npix)
+    f.width = int(width)
+    f.origin[0], f.origin[1], f.origin[2] = (float(v) for v in origin)
+    f.n_m = float(np.real(instrument.n_m))
+    f.wavelength = float(instrument.wavelength)
+    f.magnification = float(instrument.magnification)
+    f.noise = float(instrument.noise if noise is None else noise)
+    f.nmax_cap = int(nmax_cap)
+    return f
+
+
+def wiscombe_yang(x: float, m: complex) -> int:
+    """n_max of theory/Sphere.py:91-123, used to size coefficient tables."""
+    xl = abs(x)
+    if xl <= 8.:
+        ns = np.floor(xl + 4. * np.cbrt(xl) + 1.)
+    elif xl <= 4200.:
+        ns = np.floor(xl + 4.05 * np.cbrt(xl) + 2.)
+    else:
+        ns = np.floor(xl + 4. * np.cbrt(xl) + 2.)
+    return int(max(ns, abs(x * m)))
+
+
+def nmax_cap(a_p: float, n_p: float, k_p: float, n_m: float,
+             wavelength: float, headroom: float = 0.25) -> int:
+    """Orders to reserve for a sphere whose a_p, n_p may grow during a fit."""
+    k = 2. * np.pi / wavelength * np.real(n_m)
+    x = k * a_p * (1. + headroom)
+    m = (n_p * (1. + 0.25 * headroom) + 1j * k_p) / np.real(n_m)
+    return wiscombe_yang(x, m) + 4

@@ -1,0 +1,159 @@
+"""Device-resident batched Levenberg-Marquardt (lmz_lm_run).
+
+``LMBatch`` owns everything one batch of features needs on the GPU: the
+feature table (dense crops or explicit coordinates), packed data, the LM
+workspace and plan with its captured HIP graph.  ``run()`` fits every
+feature from the given starting parameters; repeated runs reuse the graph.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from .. import _lmz
+from ..theory.LorenzMie import PARAMS
+
+
+@dataclass
+class LMSettings:
+    """Stopping rules of the on-device LM (see lmz_lm_settings)."""
+    max_iter: int = 50
+    ftol: float = 1e-9
+    xtol: float = 1e-9
+    gtol: float = 1e-10
+    lambda0: float = 1e-3
+    check_every: int = 4
+    use_graph: bool = True
+
+
+def stats_from_jtj(jtj_upper: np.ndarray, chi2: float, npix: int,
+                   vary: list[int]):
+    """Reduced chi2 and uncertainties, analysis/Optimizer.py:262-278.
+
+    The reference takes the SVD of J; J^T J = V S^2 V^T gives the same
+    V and singular values, with the same cut s > eps max(P, n) s_0.
+    """
+    full = np.zeros((7, 7))
+    iu = np.triu_indices(7)
+    full[iu] = jtj_upper
+    full = full + full.T - np.diag(np.diag(full))
+    a = full[np.ix_(vary, vary)]
+    nvar = len(vary)
+    ndeg = npix - nvar
+    redchi = chi2 / ndeg if ndeg > 0 else np.nan
+    w, V = np.linalg.eigh(a)
+    order = np.argsort(w)[::-1]
+    w, V = w[order], V[:, order]
+    s = np.sqrt(np.clip(w, 0., None))
+    if s.size == 0 or s[0] == 0.:
+        return redchi, np.full(nvar, np.nan)
+    keep = s > np.finfo(float).eps * max(npix, nvar) * s[0]
+    s, V = s[keep], V[:, keep]
+    pcov = (V / s ** 2) @ V.T
+    return redchi, np.sqrt(redchi * np.diag(pcov))
+
+
+class LMBatch:
+    """Features (crops or coordinate lists) fitted together on the GPU.
+
+    ``crops``: list of dicts with keys
+        data (ny, nx) or (npts,) float array,
+        width (0 for explicit coordinates), origin (x, y, z),
+        coords (3, npts) float array relative to origin when width == 0,
+        noise, nmax_cap.
+    """
+
+    def __init__(self, crops: list[dict], instrument, flags: int = 3):
+        t = _lmz.torch()
+        dev = _lmz.device()
+        feats = []
+        off = 0
+        datas, coords = [], []
+        self.npix = []
+        for c in crops:
+            data = np.asarray(c['data'], np.float32).ravel()
+            n = data.size
+            feats.append(_lmz.make_feature(n, c.get('width', 0),
+                                           c.get('origin', (0., 0., 0.)),
+                                           instrument, c['nmax_cap'], off,
+                                           noise=c.get('noise')))
+            datas.append(data)
+            if c.get('width', 0) == 0:
+                coords.append(np.asarray(c['coords'], np.float32).reshape(3, n))
+            else:
+                coords.append(None)
+            self.npix.append(n)
+            off += n
+        self.batch = _lmz.Batch(feats, 2)
+        self.nfeat = len(crops)
+        self.flags = flags
+        self.data = t.from_numpy(np.concatenate(datas)).to(dev)
+        if any(c is not None for c in coords):
+            allc = np.zeros((3, off), np.float32)
+            o = 0
+            for c, n in zip(coords, self.npix):
+                if c is not None:
+                    allc[:, o:o + n] = c
+                o += n
+            self.coords = t.from_numpy(allc).to(dev)
+        else:
+            self.coords = None
+        nbytes = self.batch.lm_workspace_bytes()
+        self.workspace = t.empty(nbytes + 256, dtype=t.uint8, device=dev)
+        base = self.workspace.data_ptr()
+        self._ws_ptr = (base + 255) & ~255
+        self.params = t.zeros((self.nfeat, _lmz.NPARAM), dtype=t.float64,
+                              device=dev)
+        self.result = t.zeros((self.nfeat, _lmz.NRES), dtype=t.float64,
+                              device=dev)
+        plan = ctypes.c_void_p()
+        _lmz.check(_lmz.load_library().lmz_lm_plan_create(
+            self.batch.handle, ctypes.byref(plan)))
+        self.plan = plan
+
+    def close(self):
+        if getattr(self, 'plan', None):
+            _lmz.load_library().lmz_lm_plan_destroy(self.plan)
+            self.plan = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def lm_settings(settings: LMSettings, vary) -> _lmz.LMSettings:
+        s = _lmz.LMSettings()
+        s.max_iter = int(settings.max_iter)
+        s.check_every = int(settings.check_every)
+        s.ftol = float(settings.ftol)
+        s.xtol = float(settings.xtol)
+        s.gtol = float(settings.gtol)
+        s.lambda0 = float(settings.lambda0)
+        for i in range(_lmz.NJAC):
+            s.vary[i] = 1 if (PARAMS[i] in vary) else 0
+        s.use_graph = 1 if settings.use_graph else 0
+        return s
+
+    def load(self, params0: np.ndarray) -> None:
+        """Starting parameters, (nfeat, 7 or 8) rows of PARAMS order."""
+        t = _lmz.torch()
+        p = np.zeros((self.nfeat, _lmz.NPARAM))
+        p0 = np.asarray(params0, float)
+        p[:, :p0.shape[1]] = p0
+        self.params.copy_(t.from_numpy(p))
+
+    def run(self, settings: LMSettings, vary, stream=None) -> None:
+        """Fit on the device, stream-ordered; results stay in HBM."""
+        s = self.lm_settings(settings, vary)
+        _lmz.check(_lmz.load_library().lmz_lm_run(
+            self.plan, _lmz.ptr(self.coords), self.flags, _lmz.ptr(self.data),
+            ctypes.byref(s), _lmz.ptr(self.params), _lmz.ptr(self.result),
+            self._ws_ptr, _lmz.stream_handle(stream)))
+
+    def fetch(self):
+        """(params (nfeat, 8), result (nfeat, NRES)) as numpy."""
+        return self.params.cpu().numpy(), self.result.cpu().numpy()

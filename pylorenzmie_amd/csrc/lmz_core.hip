@@ -1,0 +1,520 @@
+// liblmz: batch tables, Mie coefficients, forward model, Jacobian and
+// normal equations for gfx950.  C ABI in include/lmz.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "lmz_device.h"
+
+namespace lmz {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+int hip_fail(hipError_t e, const char *what) {
+    g_error = std::string(what) + ": " + hipGetErrorString(e);
+    return LMZ_EHIP;
+}
+
+int check_batch(const lmz_batch *b) {
+    LMZ_REQUIRE(b != nullptr, "batch is NULL");
+    LMZ_REQUIRE(b->d_feat != nullptr && b->d_tile_feature != nullptr,
+                "batch was not created by lmz_batch_create");
+    return LMZ_OK;
+}
+
+// ====================================================================
+// kernels
+// ====================================================================
+
+// One lane per feature: the FP64 recursion is sequential in n.
+__global__ __launch_bounds__(64) void k_mie(
+    const lmz_feature *__restrict__ feat, int nfeat,
+    const double *__restrict__ params, const int32_t *__restrict__ sel,
+    int flip, int nbuf, int32_t *__restrict__ nmax_out,
+    float *__restrict__ coef, float *__restrict__ dcoef,
+    double *__restrict__ ab_out, double *__restrict__ work,
+    const uint8_t *__restrict__ active) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfeat) return;
+    if (active && !active[f]) return;
+    const lmz_feature F = feat[f];
+    const double *p = params + (long long)LMZ_NPARAM * f;
+    const int b = sel ? (sel[f] ^ flip) : 0;
+    const long long slot0 = F.coef_offset + (long long)b * F.nmax_cap;
+    const int N = mie_feature(p[3], p[4], p[5], F.n_m, F.wavelength,
+                              F.nmax_cap, work + f, nfeat,
+                              coef ? coef + LMZ_COEF_STRIDE * slot0 : nullptr,
+                              dcoef ? dcoef + LMZ_DCOEF_STRIDE * slot0 : nullptr,
+                              ab_out ? ab_out + 4 * slot0 : nullptr);
+    nmax_out[(long long)f * nbuf + b] = N;
+}
+
+// Per-order tables from FP64 a_n, b_n supplied by the caller.
+__global__ __launch_bounds__(64) void k_coef_from_ab(
+    const lmz_feature *__restrict__ feat, int nfeat,
+    const double *__restrict__ ab_in, const int32_t *__restrict__ nmax_in,
+    int nbuf, int32_t *__restrict__ nmax_out, float *__restrict__ coef) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfeat) return;
+    const lmz_feature F = feat[f];
+    int N = nmax_in[f];
+    N = N < 0 ? 0 : (N > F.nmax_cap ? F.nmax_cap : N);
+    for (int n = 1; n <= N; ++n) {
+        const double *r = ab_in + 4LL * (F.coef_offset + n - 1);
+        write_order(n, cd{r[0], r[1]}, cd{r[2], r[3]},
+                    coef + (long long)LMZ_COEF_STRIDE * (F.coef_offset + n - 1));
+    }
+    nmax_out[(long long)f * nbuf] = N;
+}
+
+// Forward pass over one tile of LMZ_TILE pixels of one feature, in FP64.
+// MODE 0: field, 1: hologram, 2: residual (+ per-tile r^T r).
+template <int MODE>
+__global__ __launch_bounds__(LMZ_THREADS) void k_forward(
+    const lmz_feature *__restrict__ feat,
+    const int32_t *__restrict__ tile_feature,
+    const double *__restrict__ params, const int32_t *__restrict__ sel,
+    int flip, int nbuf, const int32_t *__restrict__ nmax,
+    const float *__restrict__ coef, const float *__restrict__ coords,
+    long long npix_total, int flags, const float *__restrict__ data,
+    float *__restrict__ out, long long out_stride, cf *__restrict__ field,
+    double *__restrict__ chi2_tiles, const uint8_t *__restrict__ active) {
+    const int t = blockIdx.x;
+    const int f = tile_feature[t];
+    if (active && !active[f]) return;
+    const lmz_feature &F = feat[f];
+    const int b = sel ? (sel[f] ^ flip) : 0;
+    const int N = nmax[(long long)f * nbuf + b];
+    const FeatUniform fu = feature_uniform(
+        F, params + (long long)LMZ_NPARAM * f, N,
+        F.coef_offset + b * F.nmax_cap);
+    const bool cart = flags & LMZ_CARTESIAN;
+    const bool bohren = flags & LMZ_BOHREN;
+    const float *cf0 = coef + (long long)LMZ_COEF_STRIDE * fu.slot0;
+    float *outf = out ? out + b * out_stride : nullptr;
+    double chi = 0.0;
+#pragma unroll 1
+    for (int q = 0; q < LMZ_PPT; ++q) {
+        const int j = (t - fu.tile0) * LMZ_TILE + q * LMZ_THREADS + threadIdx.x;
+        if (j >= fu.npix) break;
+        float ax, ay, az;
+        pixel_coords(fu, j, coords, npix_total, ax, ay, az);
+        PixGeomD g;
+        pixel_geometry_d(fu, ax, ay, az, bohren, g);
+        const SumsD S = forward_sum(g, cf0, fu.nmax);
+        cd E[3];
+        field_from_sums(g, S, cart, E);
+        const long long gi = fu.pix_offset + j;
+        if (MODE == 0) {
+            const double al = fu.alphad;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                cf e{(float)(al * E[c].re), (float)(al * E[c].im)};
+                if (flags & LMZ_ACCUMULATE) e = e + field[c * npix_total + gi];
+                field[c * npix_total + gi] = e;
+            }
+        } else {
+            const double I = intensity(E, fu.alphad);
+            if (MODE == 1) {
+                out[gi] = (float)I;
+            } else {
+                const double r = (I - (double)data[gi]) / F.noise;
+                if (outf) outf[gi] = (float)r;
+                chi = fma(r, r, chi);
+            }
+        }
+    }
+    if (MODE == 2 && chi2_tiles) {
+        double v[1] = {chi};
+        block_reduce_store_d<1>(v, chi2_tiles + t);
+    }
+}
+
+// Forward-mode Jacobian over one tile; optional J, residual, and per-tile
+// normal-equation partials.
+__global__ __launch_bounds__(LMZ_THREADS) void k_jacobian(
+    const lmz_feature *__restrict__ feat,
+    const int32_t *__restrict__ tile_feature,
+    const double *__restrict__ params, const int32_t *__restrict__ sel,
+    int flip, int nbuf, const int32_t *__restrict__ nmax,
+    const float *__restrict__ coef, const float *__restrict__ dcoef,
+    const float *__restrict__ coords, long long npix_total, int flags,
+    const float *__restrict__ data, const float *__restrict__ rin,
+    long long rin_stride, float *__restrict__ jac,
+    float *__restrict__ resid, double *__restrict__ partials,
+    const uint8_t *__restrict__ active) {
+    const int t = blockIdx.x;
+    const int f = tile_feature[t];
+    if (active && !active[f]) return;
+    const lmz_feature &F = feat[f];
+    const int b = sel ? (sel[f] ^ flip) : 0;
+    const int N = nmax[(long long)f * nbuf + b];
+    const FeatUniform fu = feature_uniform(
+        F, params + (long long)LMZ_NPARAM * f, N,
+        F.coef_offset + b * F.nmax_cap);
+    const bool bohren = flags & LMZ_BOHREN;
+    const float *cf0 = coef + (long long)LMZ_COEF_STRIDE * fu.slot0;
+    const float *df0 = dcoef + (long long)LMZ_DCOEF_STRIDE * fu.slot0;
+    const float *rinf = rin ? rin + b * rin_stride : nullptr;
+    float acc[LMZ_NRED];
+#pragma unroll
+    for (int i = 0; i < LMZ_NRED; ++i) acc[i] = 0.f;
+#pragma unroll 1
+    for (int q = 0; q < LMZ_PPT; ++q) {
+        const int j = (t - fu.tile0) * LMZ_TILE + q * LMZ_THREADS + threadIdx.x;
+        if (j >= fu.npix) break;
+        float ax, ay, az;
+        pixel_coords(fu, j, coords, npix_total, ax, ay, az);
+        PixGeom g;
+        pixel_geometry(fu, ax, ay, az, bohren, g);
+        JacPixel jp;
+        jacobian_pixel(fu, g, cf0, df0, fu.nmax, jp);
+        const long long gi = fu.pix_offset + j;
+        if (jac) {
+#pragma unroll
+            for (int c = 0; c < LMZ_NJAC; ++c) jac[c * npix_total + gi] = jp.J[c];
+        }
+        if (data) {
+            const float r = rinf ? rinf[gi] : (jp.I - data[gi]) * fu.inv_noise;
+            if (resid) resid[gi] = r;
+            if (partials) {
+                float Jn[LMZ_NJAC];
+#pragma unroll
+                for (int c = 0; c < LMZ_NJAC; ++c) Jn[c] = jp.J[c] * fu.inv_noise;
+                int idx = 0;
+#pragma unroll
+                for (int a = 0; a < LMZ_NJAC; ++a)
+#pragma unroll
+                    for (int c = a; c < LMZ_NJAC; ++c, ++idx)
+                        acc[idx] = fmaf(Jn[a], Jn[c], acc[idx]);
+#pragma unroll
+                for (int a = 0; a < LMZ_NJAC; ++a)
+                    acc[28 + a] = fmaf(Jn[a], r, acc[28 + a]);
+                acc[35] = fmaf(r, r, acc[35]);
+            }
+        }
+    }
+    if (partials && data) block_reduce_store<LMZ_NRED>(acc, partials + (long long)LMZ_NRED * t);
+}
+
+// Normal equations from J and r in HBM: one block per feature.
+__global__ __launch_bounds__(LMZ_THREADS) void k_normal_eq(
+    const lmz_feature *__restrict__ feat, const float *__restrict__ jac,
+    const float *__restrict__ resid, long long npix_total,
+    double *__restrict__ out) {
+    const int f = blockIdx.x;
+    const lmz_feature &F = feat[f];
+    const float inv_noise = (float)(1.0 / F.noise);
+    float acc[LMZ_NRED];
+#pragma unroll
+    for (int i = 0; i < LMZ_NRED; ++i) acc[i] = 0.f;
+    for (int j = threadIdx.x; j < F.npix; j += LMZ_THREADS) {
+        const long long gi = F.pix_offset + j;
+        float Jn[LMZ_NJAC];
+#pragma unroll
+        for (int c = 0; c < LMZ_NJAC; ++c) Jn[c] = jac[c * npix_total + gi] * inv_noise;
+        const float r = resid[gi];
+        int idx = 0;
+#pragma unroll
+        for (int a = 0; a < LMZ_NJAC; ++a)
+#pragma unroll
+            for (int c = a; c < LMZ_NJAC; ++c, ++idx)
+                acc[idx] = fmaf(Jn[a], Jn[c], acc[idx]);
+#pragma unroll
+        for (int a = 0; a < LMZ_NJAC; ++a) acc[28 + a] = fmaf(Jn[a], r, acc[28 + a]);
+        acc[35] = fmaf(r, r, acc[35]);
+    }
+    block_reduce_store<LMZ_NRED>(acc, out + (long long)LMZ_NRED * f);
+}
+
+// |x + alpha E|^2 of an accumulated field, one thread per pixel.
+__global__ __launch_bounds__(LMZ_THREADS) void k_intensity(
+    const lmz_feature *__restrict__ feat,
+    const int32_t *__restrict__ tile_feature,
+    const double *__restrict__ params, const cf *__restrict__ field,
+    long long npix_total, float *__restrict__ out) {
+    const int t = blockIdx.x;
+    const int f = tile_feature[t];
+    const lmz_feature &F = feat[f];
+    for (int q = 0; q < LMZ_PPT; ++q) {
+        const int j = (t - F.tile_offset) * LMZ_TILE + q * LMZ_THREADS + threadIdx.x;
+        if (j >= F.npix) break;
+        const long long gi = F.pix_offset + j;
+        // the field already carries alpha (lmz_field writes alpha E)
+        cd E[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const cf e = field[c * npix_total + gi];
+            E[c] = cd{(double)e.re, (double)e.im};
+        }
+        out[gi] = (float)intensity(E, 1.0);
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+// Host-side launchers shared with the LM translation unit.
+void launch_mie(const lmz_batch *b, const double *params, const int32_t *sel,
+                int flip, int32_t *nmax, float *coef, float *dcoef,
+                double *ab_out, double *work, const uint8_t *active,
+                hipStream_t stream) {
+    const int grid = (b->nfeat + 63) / 64;
+    hipLaunchKernelGGL(k_mie, dim3(grid), dim3(64), 0, stream, b->d_feat,
+                       b->nfeat, params, sel, flip, b->coef_buffers, nmax,
+                       coef, dcoef, ab_out, work, active);
+}
+
+void launch_residual(const lmz_batch *b, const double *params,
+                     const int32_t *sel, int flip, const int32_t *nmax,
+                     const float *coef, const float *coords, int flags,
+                     const float *data, float *out, long long out_stride,
+                     double *chi2_tiles, const uint8_t *active,
+                     hipStream_t stream) {
+    hipLaunchKernelGGL(k_forward<2>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                       stream, b->d_feat, b->d_tile_feature, params, sel, flip,
+                       b->coef_buffers, nmax, coef, coords, b->npix_total,
+                       flags, data, out, out_stride, nullptr, chi2_tiles,
+                       active);
+}
+
+void launch_jacobian(const lmz_batch *b, const double *params,
+                     const int32_t *sel, int flip, const int32_t *nmax,
+                     const float *coef, const float *dcoef,
+                     const float *coords, int flags, const float *data,
+                     const float *rin, long long rin_stride, float *jac,
+                     float *resid, double *partials, const uint8_t *active,
+                     hipStream_t stream) {
+    hipLaunchKernelGGL(k_jacobian, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                       stream, b->d_feat, b->d_tile_feature, params, sel, flip,
+                       b->coef_buffers, nmax, coef, dcoef, coords,
+                       b->npix_total, flags, data, rin, rin_stride, jac, resid,
+                       partials, active);
+}
+
+}  // namespace lmz
+
+using namespace lmz;
+
+// ====================================================================
+// C ABI
+// ====================================================================
+extern "C" {
+
+const char *lmz_last_error(void) { return g_error.c_str(); }
+int lmz_version(void) { return LMZ_VERSION; }
+
+int lmz_batch_create(int32_t nfeat, lmz_feature *feat, int32_t coef_buffers,
+                     lmz_batch **out) {
+    LMZ_REQUIRE(out != nullptr, "out is NULL");
+    *out = nullptr;
+    LMZ_REQUIRE(nfeat > 0, "nfeat must be positive");
+    LMZ_REQUIRE(feat != nullptr, "feature table is NULL");
+    LMZ_REQUIRE(coef_buffers == 1 || coef_buffers == 2,
+                "coef_buffers must be 1 or 2");
+    std::vector<int32_t> tile_feature;
+    int64_t npix_total = 0, prev_end = 0;
+    int32_t slots = 0, cap_max = 0, all_dense = 1;
+    for (int32_t f = 0; f < nfeat; ++f) {
+        lmz_feature &F = feat[f];
+        char msg[160];
+        if (F.npix <= 0 || F.pix_offset < prev_end || F.width < 0 ||
+            F.width > LMZ_MAX_WIDTH || F.nmax_cap < 1 || F.nmax_cap > 4096 ||
+            !(F.noise > 0.0) || !(F.wavelength > 0.0) || !(F.n_m > 0.0) ||
+            !(F.magnification > 0.0) ||
+            (F.width > 0 && F.npix >= (1 << 24))) {
+            snprintf(msg, sizeof msg,
+                     "invalid feature %d (npix %d, offset %lld, width %d, cap %d)",
+                     f, F.npix, (long long)F.pix_offset, F.width, F.nmax_cap);
+            set_error(msg);
+            return LMZ_EINVAL;
+        }
+        if (F.width == 0) all_dense = 0;
+        prev_end = F.pix_offset + F.npix;
+        npix_total = std::max<int64_t>(npix_total, prev_end);
+        F.coef_offset = slots;
+        F.ntiles = (F.npix + LMZ_TILE - 1) / LMZ_TILE;
+        F.tile_offset = (int32_t)tile_feature.size();
+        for (int32_t i = 0; i < F.ntiles; ++i) tile_feature.push_back(f);
+        const int64_t s = (int64_t)slots + (int64_t)F.nmax_cap * coef_buffers;
+        LMZ_REQUIRE(s < (1LL << 30), "coefficient table too large");
+        slots = (int32_t)s;
+        cap_max = std::max(cap_max, F.nmax_cap);
+        LMZ_REQUIRE(tile_feature.size() < (1ULL << 31), "too many tiles");
+    }
+    lmz_batch *b = new lmz_batch;
+    b->nfeat = nfeat;
+    b->ntiles = (int32_t)tile_feature.size();
+    b->npix_total = npix_total;
+    b->coef_buffers = coef_buffers;
+    b->coef_slots = slots;
+    b->cap_max = cap_max;
+    b->all_dense = all_dense;
+    hipError_t e = hipMalloc(&b->d_feat, sizeof(lmz_feature) * nfeat);
+    if (e == hipSuccess)
+        e = hipMalloc(&b->d_tile_feature, sizeof(int32_t) * tile_feature.size());
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_feat, feat, sizeof(lmz_feature) * nfeat,
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_tile_feature, tile_feature.data(),
+                      sizeof(int32_t) * tile_feature.size(),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (b->d_feat) (void)hipFree(b->d_feat);
+        if (b->d_tile_feature) (void)hipFree(b->d_tile_feature);
+        delete b;
+        return hip_fail(e, "lmz_batch_create");
+    }
+    *out = b;
+    return LMZ_OK;
+}
+
+int lmz_batch_destroy(lmz_batch *b) {
+    if (!b) return LMZ_OK;
+    if (b->d_feat) (void)hipFree(b->d_feat);
+    if (b->d_tile_feature) (void)hipFree(b->d_tile_feature);
+    delete b;
+    return LMZ_OK;
+}
+
+int lmz_batch_query(const lmz_batch *b, lmz_batch_sizes *s) {
+    LMZ_REQUIRE(b != nullptr && s != nullptr, "NULL argument");
+    s->nfeat = b->nfeat;
+    s->ntiles = b->ntiles;
+    s->npix_total = b->npix_total;
+    s->coef_buffers = b->coef_buffers;
+    s->coef_slots = b->coef_slots;
+    s->cap_max = b->cap_max;
+    s->work_doubles = 6LL * (b->cap_max + 1) * b->nfeat;
+    s->coef_stride = LMZ_COEF_STRIDE;
+    s->dcoef_stride = LMZ_DCOEF_STRIDE;
+    return LMZ_OK;
+}
+
+int lmz_mie_ab(const lmz_batch *b, const double *params, const int32_t *sel,
+               int32_t *nmax_out, float *coef, float *dcoef, double *ab_out,
+               double *work, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(params && nmax_out && work, "params, nmax_out and work are required");
+    LMZ_REQUIRE(coef || ab_out, "one of coef, ab_out is required");
+    LMZ_REQUIRE(sel == nullptr || b->coef_buffers == 2,
+                "sel needs a batch with two coefficient buffers");
+    const int grid = (b->nfeat + 63) / 64;
+    hipLaunchKernelGGL(k_mie, dim3(grid), dim3(64), 0, stream, b->d_feat,
+                       b->nfeat, params, sel, 0, b->coef_buffers, nmax_out,
+                       coef, dcoef, ab_out, work, nullptr);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+int lmz_coef_from_ab(const lmz_batch *b, const double *ab_in,
+                     const int32_t *nmax_in, int32_t *nmax_out, float *coef,
+                     hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(ab_in && nmax_in && nmax_out && coef,
+                "ab_in, nmax_in, nmax_out and coef are required");
+    const int grid = (b->nfeat + 63) / 64;
+    hipLaunchKernelGGL(k_coef_from_ab, dim3(grid), dim3(64), 0, stream,
+                       b->d_feat, b->nfeat, ab_in, nmax_in, b->coef_buffers,
+                       nmax_out, coef);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+static int forward_args(const lmz_batch *b, const double *params,
+                        const int32_t *nmax, const float *coef,
+                        const float *coords, const int32_t *sel) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(params && nmax && coef, "params, nmax and coef are required");
+    LMZ_REQUIRE(coords || b->all_dense,
+                "coords is required when a feature has explicit coordinates");
+    LMZ_REQUIRE(sel == nullptr || b->coef_buffers == 2,
+                "sel needs a batch with two coefficient buffers");
+    return LMZ_OK;
+}
+
+int lmz_field(const lmz_batch *b, const double *params, const int32_t *sel,
+              const int32_t *nmax, const float *coef, const float *coords,
+              int32_t flags, float *field, hipStream_t stream) {
+    if (int rc = forward_args(b, params, nmax, coef, coords, sel)) return rc;
+    LMZ_REQUIRE(field != nullptr, "field is NULL");
+    hipLaunchKernelGGL(k_forward<0>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                       stream, b->d_feat, b->d_tile_feature, params, sel, 0,
+                       b->coef_buffers, nmax, coef, coords, b->npix_total,
+                       flags, nullptr, nullptr, 0LL,
+                       reinterpret_cast<cf *>(field), nullptr, nullptr);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+int lmz_hologram(const lmz_batch *b, const double *params, const int32_t *sel,
+                 const int32_t *nmax, const float *coef, const float *coords,
+                 int32_t flags, const float *data, float *out,
+                 double *chi2_tiles, const uint8_t *active,
+                 hipStream_t stream) {
+    if (int rc = forward_args(b, params, nmax, coef, coords, sel)) return rc;
+    if (data == nullptr) {
+        LMZ_REQUIRE(out != nullptr, "out is NULL");
+        hipLaunchKernelGGL(k_forward<1>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                           stream, b->d_feat, b->d_tile_feature, params, sel, 0,
+                           b->coef_buffers, nmax, coef, coords, b->npix_total,
+                           flags, nullptr, out, 0LL, nullptr, nullptr, active);
+    } else {
+        LMZ_REQUIRE(out || chi2_tiles, "one of out, chi2_tiles is required");
+        hipLaunchKernelGGL(k_forward<2>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                           stream, b->d_feat, b->d_tile_feature, params, sel, 0,
+                           b->coef_buffers, nmax, coef, coords, b->npix_total,
+                           flags, data, out, 0LL, nullptr, chi2_tiles, active);
+    }
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+int lmz_intensity(const lmz_batch *b, const double *params, const float *field,
+                  int32_t flags, float *out, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(params && field && out, "params, field and out are required");
+    (void)flags;
+    hipLaunchKernelGGL(k_intensity, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                       stream, b->d_feat, b->d_tile_feature, params,
+                       reinterpret_cast<const cf *>(field), b->npix_total, out);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+int lmz_jacobian(const lmz_batch *b, const double *params, const int32_t *sel,
+                 const int32_t *nmax, const float *coef, const float *dcoef,
+                 const float *coords, int32_t flags, const float *data,
+                 float *jac, float *resid, double *partials,
+                 const uint8_t *active, hipStream_t stream) {
+    if (int rc = forward_args(b, params, nmax, coef, coords, sel)) return rc;
+    LMZ_REQUIRE(dcoef != nullptr, "dcoef (coefficient tangents) is required");
+    LMZ_REQUIRE(flags & LMZ_CARTESIAN,
+                "the Jacobian is defined for the Cartesian hologram");
+    LMZ_REQUIRE(jac || ((resid || partials) && data),
+                "nothing to compute: jac is NULL and no data for resid/partials");
+    LMZ_REQUIRE(data || (!resid && !partials), "resid and partials need data");
+    hipLaunchKernelGGL(k_jacobian, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                       stream, b->d_feat, b->d_tile_feature, params, sel, 0,
+                       b->coef_buffers, nmax, coef, dcoef, coords,
+                       b->npix_total, flags, data, nullptr, 0LL, jac, resid,
+                       partials, active);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+int lmz_normal_eq(const lmz_batch *b, const float *jac, const float *resid,
+                  double *out, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(jac && resid && out, "jac, resid and out are required");
+    hipLaunchKernelGGL(k_normal_eq, dim3(b->nfeat), dim3(LMZ_THREADS), 0,
+                       stream, b->d_feat, jac, resid, b->npix_total, out);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+}  // extern "C"

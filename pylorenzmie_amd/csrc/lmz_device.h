@@ -1,0 +1,119 @@
+// Library-internal definitions shared by the liblmz translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+
+#include "lmz_pixel.h"
+
+#define LMZ_THREADS 256
+#define LMZ_PPT (LMZ_TILE / LMZ_THREADS)   // pixels per thread
+
+static_assert(LMZ_TILE % LMZ_THREADS == 0, "tile must be whole threads");
+
+struct lmz_batch {
+    int32_t nfeat = 0;
+    int32_t ntiles = 0;
+    int64_t npix_total = 0;
+    int32_t coef_buffers = 1;
+    int32_t coef_slots = 0;
+    int32_t cap_max = 0;
+    int32_t all_dense = 1;
+    lmz_feature *d_feat = nullptr;      // device copy of the feature table
+    int32_t *d_tile_feature = nullptr;  // tile -> feature
+};
+
+namespace lmz {
+
+void set_error(const std::string &msg);
+
+void launch_mie(const lmz_batch *b, const double *params, const int32_t *sel,
+                int flip, int32_t *nmax, float *coef, float *dcoef,
+                double *ab_out, double *work, const uint8_t *active,
+                hipStream_t stream);
+// residual (I - d) / noise written to out + (sel[f] ^ flip) * out_stride
+void launch_residual(const lmz_batch *b, const double *params,
+                     const int32_t *sel, int flip, const int32_t *nmax,
+                     const float *coef, const float *coords, int flags,
+                     const float *data, float *out, long long out_stride,
+                     double *chi2_tiles, const uint8_t *active,
+                     hipStream_t stream);
+// rin: residuals to use instead of the kernel's own, read from
+// rin + (sel[f] ^ flip) * rin_stride (NULL: computed from data)
+void launch_jacobian(const lmz_batch *b, const double *params,
+                     const int32_t *sel, int flip, const int32_t *nmax,
+                     const float *coef, const float *dcoef,
+                     const float *coords, int flags, const float *data,
+                     const float *rin, long long rin_stride, float *jac,
+                     float *resid, double *partials, const uint8_t *active,
+                     hipStream_t stream);
+int hip_fail(hipError_t e, const char *what);
+int check_batch(const lmz_batch *b);
+
+#define LMZ_HIP(call)                                                    \
+    do {                                                                 \
+        hipError_t e_ = (call);                                          \
+        if (e_ != hipSuccess) return ::lmz::hip_fail(e_, #call);         \
+    } while (0)
+
+#define LMZ_REQUIRE(cond, msg)                                           \
+    do {                                                                 \
+        if (!(cond)) { ::lmz::set_error(msg); return LMZ_EINVAL; }       \
+    } while (0)
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block reduction of NV per-thread floats to NV doubles, written to
+// out[0..NV) by the first NV threads.  Fixed order: deterministic.
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(float (&v)[NV],
+                                                   double *__restrict__ out) {
+    __shared__ double red[LMZ_THREADS / 64][NV];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const float s = wave_sum(v[i]);
+        if (lane == 0) red[wave][i] = (double)s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < LMZ_THREADS / 64; ++w) s += red[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ void block_reduce_store_d(double (&v)[NV],
+                                                     double *__restrict__ out) {
+    __shared__ double red[LMZ_THREADS / 64][NV];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const double s = wave_sum_d(v[i]);
+        if (lane == 0) red[wave][i] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = 0.0;
+#pragma unroll
+        for (int w = 0; w < LMZ_THREADS / 64; ++w) s += red[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+}  // namespace lmz

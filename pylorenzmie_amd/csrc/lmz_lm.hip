@@ -1,0 +1,524 @@
+// Batched Levenberg-Marquardt on gfx950: every feature of a batch is
+// fitted at once; one iteration is six launches captured in a HIP graph.
+//
+// Replaces Optimizer.optimize (analysis/Optimizer.py:196-222), which hands
+// one feature at a time to scipy least_squares(method='lm') (MINPACK lmdif,
+// with a forward-difference Jacobian), by a Marquardt iteration whose
+// Jacobian, trial forward pass and normal-equation reductions stay on the
+// device.  Per iteration and feature:
+//   J^T J, J^T r   from the forward-mode Jacobian (skipped after a rejection)
+//   solve          (J^T J + lambda D^2) dp = -J^T r, FP64 Cholesky
+//   trial          a_n, b_n and chi2 at p + dp
+//   accept/reject  Nielsen's damping update; ftol / xtol / gtol tests
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "lmz_device.h"
+
+namespace lmz {
+
+struct LMState {
+    double jtj[28];
+    double g[7];
+    double dscale[7];
+    double chi2;         // at the current parameters
+    double lambda;
+    double nu;
+    double pred;         // predicted chi2 reduction of the trial step
+    double dnorm;        // ||D dp||
+    double pnorm;        // ||D p||
+    int32_t iter;
+    int32_t status;
+    int32_t need_j;
+    int32_t njev;
+};
+
+struct LMLayout {
+    size_t coef, dcoef, nmax, sel, work, partials, chi2_tiles, state, ptrial,
+        mask_j, mask_f, count, resid, total;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+static LMLayout lm_layout(const lmz_batch *b) {
+    LMLayout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off = align256(off + bytes); return o; };
+    L.coef = take(sizeof(float) * LMZ_COEF_STRIDE * (size_t)b->coef_slots);
+    L.dcoef = take(sizeof(float) * LMZ_DCOEF_STRIDE * (size_t)b->coef_slots);
+    L.nmax = take(sizeof(int32_t) * 2 * (size_t)b->nfeat);
+    L.sel = take(sizeof(int32_t) * (size_t)b->nfeat);
+    L.work = take(sizeof(double) * 6 * (size_t)(b->cap_max + 1) * b->nfeat);
+    L.partials = take(sizeof(double) * LMZ_NRED * (size_t)b->ntiles);
+    L.chi2_tiles = take(sizeof(double) * (size_t)b->ntiles);
+    L.state = take(sizeof(LMState) * (size_t)b->nfeat);
+    L.ptrial = take(sizeof(double) * LMZ_NPARAM * (size_t)b->nfeat);
+    L.mask_j = take((size_t)b->nfeat);
+    L.mask_f = take((size_t)b->nfeat);
+    L.count = take(sizeof(int32_t) * 4);
+    L.resid = take(sizeof(float) * 2 * (size_t)b->npix_total);
+    L.total = off;
+    return L;
+}
+
+__device__ __forceinline__ int tri(int i, int j) {  // i <= j, row-major upper
+    return i * LMZ_NJAC - (i * (i - 1)) / 2 + (j - i);
+}
+
+__global__ __launch_bounds__(64) void k_lm_init(
+    int nfeat, LMState *__restrict__ st, const double *__restrict__ params,
+    double *__restrict__ ptrial, int32_t *__restrict__ sel,
+    uint8_t *__restrict__ mask_j, uint8_t *__restrict__ mask_f,
+    double lambda0) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfeat) return;
+    LMState s;
+    for (int i = 0; i < 28; ++i) s.jtj[i] = 0.0;
+    for (int i = 0; i < 7; ++i) { s.g[i] = 0.0; s.dscale[i] = 0.0; }
+    s.chi2 = 0.0;
+    s.lambda = lambda0;
+    s.nu = 2.0;
+    s.pred = s.dnorm = s.pnorm = 0.0;
+    s.iter = 0;
+    s.status = LMZ_LM_RUNNING;
+    s.need_j = 1;
+    s.njev = 0;
+    st[f] = s;
+    for (int i = 0; i < LMZ_NPARAM; ++i)
+        ptrial[(long long)LMZ_NPARAM * f + i] = params[(long long)LMZ_NPARAM * f + i];
+    sel[f] = 0;
+    mask_j[f] = 1;
+    mask_f[f] = 1;
+}
+
+// Sum a feature's per-tile doubles (stride NV) over its tiles: one wave.
+template <int NV>
+__device__ __forceinline__ void feature_tile_sum(const lmz_feature &F,
+                                                 const double *__restrict__ tiles,
+                                                 double (&out)[NV]) {
+    const int lane = threadIdx.x & 63;
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (int t = lane; t < F.ntiles; t += 64) {
+        const double *row = tiles + (long long)NV * (F.tile_offset + t);
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] += row[i];
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) out[i] = wave_sum_d(acc[i]);
+}
+
+__global__ __launch_bounds__(64) void k_lm_init_chi2(
+    int nfeat, const lmz_feature *__restrict__ feat,
+    const double *__restrict__ chi2_tiles, LMState *__restrict__ st) {
+    const int f = blockIdx.x;
+    double c[1];
+    feature_tile_sum<1>(feat[f], chi2_tiles, c);
+    if (threadIdx.x == 0) {
+        st[f].chi2 = c[0];
+        if (!isfinite(c[0])) st[f].status = LMZ_LM_NONFINITE;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_lm_mask_init(
+    int nfeat, const LMState *__restrict__ st, uint8_t *__restrict__ mask_j,
+    uint8_t *__restrict__ mask_f) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfeat) return;
+    const bool running = st[f].status == LMZ_LM_RUNNING;
+    mask_j[f] = running ? 1 : 0;
+    mask_f[f] = running ? 1 : 0;
+}
+
+// Reduce the Jacobian partials (if fresh), then solve for the trial step.
+__global__ __launch_bounds__(64) void k_lm_solve(
+    const lmz_feature *__restrict__ feat, const double *__restrict__ partials,
+    LMState *__restrict__ st, const double *__restrict__ params,
+    double *__restrict__ ptrial, uint8_t *__restrict__ mask_j,
+    uint8_t *__restrict__ mask_f, lmz_lm_settings set) {
+    const int f = blockIdx.x;
+    if (!mask_f[f]) return;
+    LMState &S = st[f];
+    const lmz_feature &F = feat[f];
+    if (mask_j[f]) {
+        double red[LMZ_NRED];
+        feature_tile_sum<LMZ_NRED>(F, partials, red);
+        if (threadIdx.x == 0) {
+            for (int i = 0; i < 28; ++i) S.jtj[i] = red[i];
+            for (int i = 0; i < 7; ++i) S.g[i] = red[28 + i];
+            S.njev += 1;
+            for (int i = 0; i < 7; ++i)
+                S.dscale[i] = fmax(S.dscale[i], sqrt(fmax(S.jtj[tri(i, i)], 0.0)));
+        }
+    }
+    if (threadIdx.x != 0) return;
+    // free parameters
+    int idx[7], nv = 0;
+    for (int i = 0; i < 7; ++i)
+        if (set.vary[i]) idx[nv++] = i;
+    const double *p = params + (long long)LMZ_NPARAM * f;
+    double *pt = ptrial + (long long)LMZ_NPARAM * f;
+    // gradient test (analysis/Optimizer.py:108 gtol), on a fresh Jacobian
+    if (mask_j[f] && set.gtol > 0.0) {
+        double gmax = 0.0;
+        const double rn = sqrt(fmax(S.chi2, 0.0));
+        for (int a = 0; a < nv; ++a) {
+            const int i = idx[a];
+            const double jn = sqrt(fmax(S.jtj[tri(i, i)], 0.0));
+            if (jn > 0.0 && rn > 0.0) gmax = fmax(gmax, fabs(S.g[i]) / (jn * rn));
+        }
+        if (gmax <= set.gtol) {
+            S.status |= LMZ_LM_GTOL;
+            mask_f[f] = 0;
+            mask_j[f] = 0;
+            return;
+        }
+    }
+    double delta[7] = {0, 0, 0, 0, 0, 0, 0};
+    bool ok = false;
+    for (int attempt = 0; attempt < 12 && !ok; ++attempt) {
+        double A[7][7], bvec[7];
+        for (int a = 0; a < nv; ++a) {
+            for (int c = 0; c < nv; ++c) {
+                const int i = idx[a], j = idx[c];
+                A[a][c] = (i <= j) ? S.jtj[tri(i, j)] : S.jtj[tri(j, i)];
+            }
+            const double d = S.dscale[idx[a]];
+            A[a][a] += S.lambda * (d > 0.0 ? d * d : 1.0);
+            bvec[a] = -S.g[idx[a]];
+        }
+        // Cholesky A = L L^T in place (lower)
+        ok = true;
+        for (int j = 0; j < nv && ok; ++j) {
+            double s = A[j][j];
+            for (int k = 0; k < j; ++k) s -= A[j][k] * A[j][k];
+            if (!(s > 0.0) || !isfinite(s)) { ok = false; break; }
+            const double ljj = sqrt(s);
+            A[j][j] = ljj;
+            for (int i = j + 1; i < nv; ++i) {
+                double t = A[i][j];
+                for (int k = 0; k < j; ++k) t -= A[i][k] * A[j][k];
+                A[i][j] = t / ljj;
+            }
+        }
+        if (!ok) { S.lambda *= 10.0; continue; }
+        double y[7];
+        for (int i = 0; i < nv; ++i) {
+            double t = bvec[i];
+            for (int k = 0; k < i; ++k) t -= A[i][k] * y[k];
+            y[i] = t / A[i][i];
+        }
+        for (int i = nv - 1; i >= 0; --i) {
+            double t = y[i];
+            for (int k = i + 1; k < nv; ++k) t -= A[k][i] * delta[idx[k]];
+            delta[idx[i]] = t / A[i][i];
+        }
+    }
+    if (!ok) {
+        S.status |= LMZ_LM_STALLED;
+        mask_f[f] = 0;
+        mask_j[f] = 0;
+        return;
+    }
+    // predicted reduction of chi2 = r^T r under the linear model
+    double gd = 0.0, dhd = 0.0, dn = 0.0, pn = 0.0;
+    for (int a = 0; a < nv; ++a) {
+        const int i = idx[a];
+        gd += S.g[i] * delta[i];
+        double hd = 0.0;
+        for (int c = 0; c < nv; ++c) {
+            const int j = idx[c];
+            hd += ((i <= j) ? S.jtj[tri(i, j)] : S.jtj[tri(j, i)]) * delta[j];
+        }
+        dhd += delta[i] * hd;
+        const double d = S.dscale[i];
+        dn += d * d * delta[i] * delta[i];
+        pn += d * d * p[i] * p[i];
+    }
+    S.pred = -2.0 * gd - dhd;
+    S.dnorm = sqrt(dn);
+    S.pnorm = sqrt(pn);
+    for (int i = 0; i < LMZ_NPARAM; ++i) pt[i] = p[i] + (i < 7 ? delta[i] : 0.0);
+}
+
+// Compare the trial chi2 with the current one; update damping and status.
+__global__ __launch_bounds__(64) void k_lm_accept(
+    const lmz_feature *__restrict__ feat,
+    const double *__restrict__ chi2_tiles, LMState *__restrict__ st,
+    double *__restrict__ params, const double *__restrict__ ptrial,
+    int32_t *__restrict__ sel, uint8_t *__restrict__ mask_j,
+    uint8_t *__restrict__ mask_f, lmz_lm_settings set) {
+    const int f = blockIdx.x;
+    if (!mask_f[f]) return;
+    double c[1];
+    feature_tile_sum<1>(feat[f], chi2_tiles, c);
+    if (threadIdx.x != 0) return;
+    LMState &S = st[f];
+    const double chi2t = c[0];
+    const double chi2 = S.chi2;
+    S.iter += 1;
+    const double act = chi2 - chi2t;
+    if (isfinite(chi2t) && act > 0.0) {
+        const double rho = (S.pred > 0.0) ? act / S.pred : 0.0;
+        for (int i = 0; i < LMZ_NPARAM; ++i)
+            params[(long long)LMZ_NPARAM * f + i] = ptrial[(long long)LMZ_NPARAM * f + i];
+        sel[f] ^= 1;
+        S.chi2 = chi2t;
+        const double t = 2.0 * rho - 1.0;
+        S.lambda *= fmax(1.0 / 3.0, 1.0 - t * t * t);
+        S.nu = 2.0;
+        S.need_j = 1;
+        if (act <= set.ftol * chi2 && S.pred <= set.ftol * chi2)
+            S.status |= LMZ_LM_FTOL;
+        if (S.dnorm <= set.xtol * (S.pnorm + set.xtol)) S.status |= LMZ_LM_XTOL;
+    } else {
+        S.lambda *= S.nu;
+        S.nu *= 2.0;
+        S.need_j = 0;
+        // nothing left to gain at this resolution of chi2
+        if (isfinite(chi2t) && S.pred <= set.ftol * chi2) S.status |= LMZ_LM_FTOL;
+        if (S.dnorm <= set.xtol * (S.pnorm + set.xtol)) S.status |= LMZ_LM_XTOL;
+        if (S.lambda > 1e16) S.status |= LMZ_LM_STALLED;
+    }
+    if (S.status == LMZ_LM_RUNNING && S.iter >= set.max_iter)
+        S.status |= LMZ_LM_MAXITER;
+    const bool running = (S.status == LMZ_LM_RUNNING);
+    mask_f[f] = running ? 1 : 0;
+    mask_j[f] = (running && S.need_j) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_lm_count(int nfeat,
+                                                  const uint8_t *__restrict__ mask_f,
+                                                  int32_t *__restrict__ count) {
+    __shared__ int part[256];
+    int s = 0;
+    for (int f = threadIdx.x; f < nfeat; f += 256) s += mask_f[f];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) count[0] = part[0];
+}
+
+__global__ __launch_bounds__(64) void k_lm_finalize(
+    const lmz_feature *__restrict__ feat, const double *__restrict__ partials,
+    const LMState *__restrict__ st, double *__restrict__ result) {
+    const int f = blockIdx.x;
+    double red[LMZ_NRED];
+    feature_tile_sum<LMZ_NRED>(feat[f], partials, red);
+    if (threadIdx.x != 0) return;
+    const LMState &S = st[f];
+    double *r = result + (long long)LMZ_NRES * f;
+    for (int i = 0; i < 35; ++i) r[i] = red[i];
+    r[35] = S.chi2;
+    r[36] = S.lambda;
+    r[37] = (double)S.iter;
+    r[38] = (double)S.status;
+    r[39] = (double)S.njev;
+}
+
+}  // namespace lmz
+
+using namespace lmz;
+
+struct lmz_lm_plan {
+    const lmz_batch *batch = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    int32_t *h_count = nullptr;       // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // arguments the captured graph was built for
+    const void *key_ptrs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int32_t key_flags = -1;
+    lmz_lm_settings key_set{};
+};
+
+static void plan_release_graph(lmz_lm_plan *p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+// One LM iteration: the six launches a graph replays.
+static void enqueue_iteration(const lmz_batch *b, const LMLayout &L, char *ws,
+                              const float *coords, int32_t flags,
+                              const float *data, const lmz_lm_settings &set,
+                              double *params, hipStream_t s) {
+    float *coef = reinterpret_cast<float *>(ws + L.coef);
+    float *dcoef = reinterpret_cast<float *>(ws + L.dcoef);
+    int32_t *nmax = reinterpret_cast<int32_t *>(ws + L.nmax);
+    int32_t *sel = reinterpret_cast<int32_t *>(ws + L.sel);
+    double *work = reinterpret_cast<double *>(ws + L.work);
+    double *partials = reinterpret_cast<double *>(ws + L.partials);
+    double *chi2_tiles = reinterpret_cast<double *>(ws + L.chi2_tiles);
+    LMState *st = reinterpret_cast<LMState *>(ws + L.state);
+    double *ptrial = reinterpret_cast<double *>(ws + L.ptrial);
+    uint8_t *mask_j = reinterpret_cast<uint8_t *>(ws + L.mask_j);
+    uint8_t *mask_f = reinterpret_cast<uint8_t *>(ws + L.mask_f);
+    int32_t *count = reinterpret_cast<int32_t *>(ws + L.count);
+    float *resid = reinterpret_cast<float *>(ws + L.resid);
+    const long long np = b->npix_total;
+    // J^T J and J^T r at the current point, r from its FP64 forward pass
+    launch_jacobian(b, params, sel, 0, nmax, coef, dcoef, coords, flags, data,
+                    resid, np, nullptr, nullptr, partials, mask_j, s);
+    hipLaunchKernelGGL(k_lm_solve, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
+                       partials, st, params, ptrial, mask_j, mask_f, set);
+    launch_mie(b, ptrial, sel, 1, nmax, coef, dcoef, nullptr, work, mask_f, s);
+    launch_residual(b, ptrial, sel, 1, nmax, coef, coords, flags, data,
+                    resid, np, chi2_tiles, mask_f, s);
+    hipLaunchKernelGGL(k_lm_accept, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
+                       chi2_tiles, st, params, ptrial, sel, mask_j, mask_f, set);
+    hipLaunchKernelGGL(k_lm_count, dim3(1), dim3(256), 0, s, b->nfeat, mask_f,
+                       count);
+}
+
+extern "C" {
+
+int lmz_lm_workspace_bytes(const lmz_batch *b, size_t *bytes) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(bytes != nullptr, "bytes is NULL");
+    *bytes = lm_layout(b).total;
+    return LMZ_OK;
+}
+
+int lmz_lm_plan_create(const lmz_batch *b, lmz_lm_plan **out) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(out != nullptr, "out is NULL");
+    LMZ_REQUIRE(b->coef_buffers == 2,
+                "the LM needs a batch created with coef_buffers = 2");
+    *out = nullptr;
+    lmz_lm_plan *p = new lmz_lm_plan;
+    p->batch = b;
+    hipError_t e = hipStreamCreateWithFlags(&p->s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void **>(&p->h_count), sizeof(int32_t) * 4,
+                          hipHostMallocDefault);
+    if (e != hipSuccess) {
+        lmz_lm_plan_destroy(p);
+        return hip_fail(e, "lmz_lm_plan_create");
+    }
+    *out = p;
+    return LMZ_OK;
+}
+
+int lmz_lm_plan_destroy(lmz_lm_plan *p) {
+    if (!p) return LMZ_OK;
+    if (p->s) (void)hipStreamSynchronize(p->s);
+    plan_release_graph(p);
+    if (p->h_count) (void)hipHostFree(p->h_count);
+    if (p->ev_in) (void)hipEventDestroy(p->ev_in);
+    if (p->ev_out) (void)hipEventDestroy(p->ev_out);
+    if (p->s) (void)hipStreamDestroy(p->s);
+    delete p;
+    return LMZ_OK;
+}
+
+int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
+               const float *data, const lmz_lm_settings *settings,
+               double *params_io, double *result, void *workspace,
+               hipStream_t stream) {
+    LMZ_REQUIRE(plan != nullptr && settings != nullptr, "NULL plan or settings");
+    const lmz_batch *b = plan->batch;
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(data && params_io && result && workspace,
+                "data, params_io, result and workspace are required");
+    LMZ_REQUIRE(coords || b->all_dense,
+                "coords is required when a feature has explicit coordinates");
+    LMZ_REQUIRE(flags & LMZ_CARTESIAN, "the LM fits the Cartesian hologram");
+    const lmz_lm_settings set = *settings;
+    LMZ_REQUIRE(set.max_iter >= 1 && set.max_iter <= 100000, "max_iter out of range");
+    LMZ_REQUIRE(set.check_every >= 1, "check_every must be >= 1");
+    LMZ_REQUIRE(set.ftol >= 0.0 && set.xtol >= 0.0 && set.gtol >= 0.0,
+                "tolerances must be >= 0");
+    LMZ_REQUIRE(set.lambda0 > 0.0, "lambda0 must be > 0");
+    int nv = 0;
+    for (int i = 0; i < LMZ_NJAC; ++i) nv += set.vary[i] ? 1 : 0;
+    LMZ_REQUIRE(nv >= 1, "no parameter is varied");
+    LMZ_REQUIRE((reinterpret_cast<uintptr_t>(workspace) & 255) == 0,
+                "workspace must be 256-byte aligned");
+
+    const LMLayout L = lm_layout(b);
+    char *ws = static_cast<char *>(workspace);
+    LMState *st = reinterpret_cast<LMState *>(ws + L.state);
+    int32_t *sel = reinterpret_cast<int32_t *>(ws + L.sel);
+    float *coef = reinterpret_cast<float *>(ws + L.coef);
+    float *dcoef = reinterpret_cast<float *>(ws + L.dcoef);
+    int32_t *nmax = reinterpret_cast<int32_t *>(ws + L.nmax);
+    double *work = reinterpret_cast<double *>(ws + L.work);
+    double *partials = reinterpret_cast<double *>(ws + L.partials);
+    double *chi2_tiles = reinterpret_cast<double *>(ws + L.chi2_tiles);
+    double *ptrial = reinterpret_cast<double *>(ws + L.ptrial);
+    uint8_t *mask_j = reinterpret_cast<uint8_t *>(ws + L.mask_j);
+    uint8_t *mask_f = reinterpret_cast<uint8_t *>(ws + L.mask_f);
+    int32_t *count = reinterpret_cast<int32_t *>(ws + L.count);
+    float *resid = reinterpret_cast<float *>(ws + L.resid);
+    const long long np = b->npix_total;
+    hipStream_t s = plan->s;
+
+    LMZ_HIP(hipEventRecord(plan->ev_in, stream));
+    LMZ_HIP(hipStreamWaitEvent(s, plan->ev_in, 0));
+    // initial state, coefficients with tangents, chi2 at the start
+    hipLaunchKernelGGL(k_lm_init, dim3((b->nfeat + 63) / 64), dim3(64), 0, s,
+                       b->nfeat, st, params_io, ptrial, sel, mask_j, mask_f,
+                       set.lambda0);
+    launch_mie(b, params_io, sel, 0, nmax, coef, dcoef, nullptr, work, nullptr, s);
+    launch_residual(b, params_io, sel, 0, nmax, coef, coords, flags, data,
+                    resid, np, chi2_tiles, nullptr, s);
+    hipLaunchKernelGGL(k_lm_init_chi2, dim3(b->nfeat), dim3(64), 0, s, b->nfeat,
+                       b->d_feat, chi2_tiles, st);
+    hipLaunchKernelGGL(k_lm_mask_init, dim3((b->nfeat + 63) / 64), dim3(64), 0, s,
+                       b->nfeat, st, mask_j, mask_f);
+    LMZ_HIP(hipGetLastError());
+
+    if (set.use_graph) {
+        const void *key[5] = {coords, data, params_io, workspace, result};
+        const bool same = plan->exec && std::memcmp(key, plan->key_ptrs, sizeof key) == 0 &&
+                          plan->key_flags == flags &&
+                          std::memcmp(&set, &plan->key_set, sizeof set) == 0;
+        if (!same) {
+            plan_release_graph(plan);
+            LMZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            enqueue_iteration(b, L, ws, coords, flags, data, set, params_io, s);
+            hipError_t e = hipStreamEndCapture(s, &plan->graph);
+            if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            LMZ_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+            std::memcpy(plan->key_ptrs, key, sizeof key);
+            plan->key_flags = flags;
+            plan->key_set = set;
+        }
+    }
+    for (int it = 0; it < set.max_iter; ++it) {
+        if (set.use_graph) {
+            LMZ_HIP(hipGraphLaunch(plan->exec, s));
+        } else {
+            enqueue_iteration(b, L, ws, coords, flags, data, set, params_io, s);
+            LMZ_HIP(hipGetLastError());
+        }
+        if ((it + 1) % set.check_every == 0 && it + 1 < set.max_iter) {
+            LMZ_HIP(hipMemcpyAsync(plan->h_count, count, sizeof(int32_t),
+                                   hipMemcpyDeviceToHost, s));
+            LMZ_HIP(hipStreamSynchronize(s));
+            if (plan->h_count[0] == 0) break;
+        }
+    }
+    // Jacobian at the solution for the uncertainties
+    launch_jacobian(b, params_io, sel, 0, nmax, coef, dcoef, coords, flags, data,
+                    resid, np, nullptr, nullptr, partials, nullptr, s);
+    hipLaunchKernelGGL(k_lm_finalize, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
+                       partials, st, result);
+    LMZ_HIP(hipGetLastError());
+    LMZ_HIP(hipEventRecord(plan->ev_out, s));
+    LMZ_HIP(hipStreamWaitEvent(stream, plan->ev_out, 0));
+    return LMZ_OK;
+}
+
+}  // extern "C"

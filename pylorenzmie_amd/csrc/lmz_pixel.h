@@ -1,0 +1,756 @@
+// Per-pixel (FP32) and per-feature (FP64) arithmetic of the Lorenz-Mie
+// kernels.  Every function here is __host__ __device__ so that the same
+// source runs in the gfx950 kernels and in the host-side checker library
+// the tests build from it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/lmz.h"
+
+#define LMZ_HD __host__ __device__ __forceinline__
+
+namespace lmz {
+
+// ====================================================================
+// complex helpers
+// ====================================================================
+struct cf { float re, im; };
+struct cd { double re, im; };
+
+LMZ_HD cf operator+(cf a, cf b) { return {a.re + b.re, a.im + b.im}; }
+LMZ_HD cf operator-(cf a, cf b) { return {a.re - b.re, a.im - b.im}; }
+LMZ_HD cf operator*(float s, cf a) { return {s * a.re, s * a.im}; }
+LMZ_HD cf cmul(cf a, cf b) {
+    return {fmaf(a.re, b.re, -a.im * b.im), fmaf(a.re, b.im, a.im * b.re)};
+}
+// acc += s * a
+LMZ_HD void cmac(cf &acc, float s, cf a) {
+    acc.re = fmaf(s, a.re, acc.re);
+    acc.im = fmaf(s, a.im, acc.im);
+}
+// Re(conj(a) b)
+LMZ_HD float credot(cf a, cf b) { return fmaf(a.re, b.re, a.im * b.im); }
+
+LMZ_HD cd operator+(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
+LMZ_HD cd operator-(cd a, cd b) { return {a.re - b.re, a.im - b.im}; }
+LMZ_HD cd operator-(cd a) { return {-a.re, -a.im}; }
+LMZ_HD cd operator*(cd a, cd b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+LMZ_HD cd operator*(double s, cd a) { return {s * a.re, s * a.im}; }
+LMZ_HD cd operator/(cd a, cd b) {  // Smith's algorithm
+    if (fabs(b.re) >= fabs(b.im)) {
+        const double r = b.im / b.re, d = b.re + b.im * r;
+        return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+    }
+    const double r = b.re / b.im, d = b.re * r + b.im;
+    return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+}
+LMZ_HD cd cinv(cd b) { return cd{1.0, 0.0} / b; }
+LMZ_HD cd mul_i(cd a) { return {-a.im, a.re}; }        // i a
+// i^k a
+LMZ_HD cd mul_ipow(int k, cd a) {
+    switch (k & 3) {
+        case 0: return a;
+        case 1: return {-a.im, a.re};
+        case 2: return {-a.re, -a.im};
+        default: return {a.im, -a.re};
+    }
+}
+
+// ====================================================================
+// per-feature uniforms (FP64 in, FP32 out)
+// ====================================================================
+// x_p - origin = ix + fx with ix integral, so the per-pixel displacement
+// (x_rel - ix) - fx loses no digits to the particle's absolute position.
+struct FeatUniform {
+    float k;           // wavenumber [rad / px]
+    float ix, iy, iz;  // integral parts of the particle offset from origin
+    float fx, fy, fz;  // fractional parts
+    float kz0;         // k * origin z
+    double kd, fxd, fyd, fzd, kz0d;   // the same in FP64
+    float alpha;
+    double alphad;
+    float inv_noise;
+    int npix;
+    int width;
+    float inv_width;
+    long long pix_offset;
+    int tile0;
+    int nmax;
+    int slot0;
+};
+
+// Instrument.wavenumber, theory/Instrument.py:46-70 (in medium, scaled)
+LMZ_HD double wavenumber_px(const lmz_feature &F) {
+    return 2.0 * M_PI / F.wavelength * F.n_m * F.magnification;
+}
+
+LMZ_HD FeatUniform feature_uniform(const lmz_feature &F, const double *p,
+                                   int nmax, int slot0) {
+    FeatUniform u;
+    const double kk = wavenumber_px(F);
+    u.k = (float)kk;
+    const double dx = p[0] - F.origin[0];
+    const double dy = p[1] - F.origin[1];
+    const double dz = p[2] - F.origin[2];
+    const double rx = rint(dx), ry = rint(dy), rz = rint(dz);
+    u.ix = (float)rx;
+    u.iy = (float)ry;
+    u.iz = (float)rz;
+    u.fx = (float)(dx - rx);
+    u.fy = (float)(dy - ry);
+    u.fz = (float)(dz - rz);
+    u.kz0 = (float)(kk * F.origin[2]);
+    u.kd = kk;
+    u.fxd = dx - rx;
+    u.fyd = dy - ry;
+    u.fzd = dz - rz;
+    u.kz0d = kk * F.origin[2];
+    u.alpha = (float)p[6];
+    u.alphad = p[6];
+    u.inv_noise = (float)(1.0 / F.noise);
+    u.npix = F.npix;
+    u.width = F.width;
+    u.inv_width = F.width > 0 ? 1.0f / (float)F.width : 0.f;
+    u.pix_offset = F.pix_offset;
+    u.tile0 = F.tile_offset;
+    u.nmax = nmax;
+    u.slot0 = slot0;
+    return u;
+}
+
+// Pixel j of a feature: coordinates relative to the feature origin, from
+// the implicit dense grid (x fastest, lib/meshgrid.py:34-40) or the
+// explicit arrays.
+LMZ_HD void pixel_coords(const FeatUniform &fu, int j, const float *coords,
+                         long long npix_total, float &ax, float &ay,
+                         float &az) {
+    if (fu.width > 0) {
+        int row = (int)(((float)j + 0.5f) * fu.inv_width);
+        int col = j - row * fu.width;
+        if (col < 0) { row -= 1; col += fu.width; }
+        else if (col >= fu.width) { row += 1; col -= fu.width; }
+        ax = (float)col;
+        ay = (float)row;
+        az = 0.f;
+    } else {
+        const long long gi = fu.pix_offset + j;
+        ax = coords[gi];
+        ay = coords[npix_total + gi];
+        az = coords[2 * npix_total + gi];
+    }
+}
+
+// ====================================================================
+// pixel geometry, theory/LorenzMie.py:221-257
+// ====================================================================
+// The particle phase exp(-i k z_p) of theory/LorenzMie.py:157-158 is
+// folded into the starting values of the Riccati-Bessel recurrence (it is
+// linear with real coefficients, so every xi_n carries the same factor).
+// For the outgoing wave the phase k (r - z_p) is formed as
+// (rho^2 / (r + dz) - z) k, free of cancellation near the axis.
+struct PixGeom {
+    float X, Y, Z;      // k (x - x_p), k (y - y_p), k (z_p - z)
+    float R, invR;      // kr, 1 / kr
+    float rho;          // k rho
+    float c, s;         // cos theta, sin theta
+    float u, v;         // cos phi, sin phi
+    // Recurrence inputs.  For cos theta < 0 the recurrences run on the mirror
+    // image: pi_n(-c) = (-1)^(n+1) pi_n(c), tau_n(-c) = (-1)^n tau_n(c), and
+    // (-1)^(n+1) xi_n obeys the xi recurrence with -1/kr from (xi_{-1}, -xi_0);
+    // the sums are then equal except for the sign of S_phi (mir = -1).
+    float w;            // 1 - |cos theta|, without cancellation
+    float cl;           // |cos theta|
+    float invRl;        // +-1 / kr
+    float mir;          // +1, or -1 for cos theta < 0
+    cf xi_m1, xi_0;     // recurrence start, times exp(-i k z_p)
+};
+
+LMZ_HD void pixel_geometry(const FeatUniform &fu, float ax, float ay, float az,
+                           bool bohren, PixGeom &g) {
+    const float dx = (ax - fu.ix) - fu.fx;
+    const float dy = (ay - fu.iy) - fu.fy;
+    const float dz = (fu.iz - az) + fu.fz;
+    g.X = fu.k * dx;
+    g.Y = fu.k * dy;
+    g.Z = fu.k * dz;
+    const float rho2 = fmaf(g.X, g.X, g.Y * g.Y);
+    g.rho = sqrtf(rho2);
+    g.R = sqrtf(fmaf(g.Z, g.Z, rho2));
+    g.invR = 1.0f / g.R;
+    g.c = g.Z * g.invR;
+    g.s = g.rho * g.invR;
+    if (g.rho > 0.f) {
+        const float ir = 1.0f / g.rho;
+        g.u = g.X * ir;
+        g.v = g.Y * ir;
+    } else {  // arctan2(0, 0) = 0
+        g.u = 1.f;
+        g.v = 0.f;
+    }
+    const float sz = (g.Z > 0.f) ? 1.f : ((g.Z < 0.f) ? -1.f : 0.f);
+    const float sig = bohren ? sz : -sz;
+    const float kzpix = fmaf(fu.k, az, fu.kz0);  // k * z of the pixel
+    const float az_ = fabsf(g.Z);
+    const float t = rho2 / (g.R + az_);                // R - |Z|
+    g.w = t * g.invR;
+    g.cl = az_ * g.invR;
+    g.mir = (g.Z < 0.f) ? -1.f : 1.f;
+    g.invRl = g.mir * g.invR;
+    if (sig != 0.f) {
+        float psi;
+        if (sig > 0.f) {  // k (r - z_p) = (R - Z) - k z_pix
+            psi = ((g.Z > 0.f) ? t : g.R - g.Z) - kzpix;
+        } else {          // -k (r + z_p) = -((R + Z) + k z_pix)
+            psi = -(((g.Z < 0.f) ? t : g.R + g.Z) + kzpix);
+        }
+        float sn, cs;
+        sincosf(psi, &sn, &cs);
+        g.xi_m1 = {cs, sn};                 // exp(i psi)
+        g.xi_0 = {sig * sn, -sig * cs};     // -i sig exp(i psi)
+    } else {  // kz == 0: cos kr, sin kr, times exp(-i k z_pix)
+        float sr, cr, sp, cp;
+        sincosf(g.R, &sr, &cr);
+        sincosf(kzpix, &sp, &cp);
+        g.xi_m1 = {cr * cp, -cr * sp};
+        g.xi_0 = {sr * cp, -sr * sp};
+    }
+    g.xi_0 = g.mir * g.xi_0;
+}
+
+// ====================================================================
+// per-order table row (LMZ_COEF_STRIDE floats, scalar loads per order)
+// ====================================================================
+//   [0..5]   PR = i^(n+1)(2n+1) a_n,  P = i E_n a_n,  Q = E_n b_n  (FP32)
+//   [6]      n          [7]  n(n+1)
+//   [8..9]   1 + 1/n    [10..11]  n(n+1)/2    [12..13]  (n+1)(n+2)/2
+//   [14..15] n                                                   (FP64)
+//   [16..27] PR, P, Q                                            (FP64)
+// E_n = i^n (2n+1)/(n(n+1)) as theory/LorenzMie.py:297-303.
+#define LMZ_COEF_STRIDE 32
+#define LMZ_DCOEF_STRIDE 24
+
+LMZ_HD double row_d(const float *row, int i) {
+    double v;
+    __builtin_memcpy(&v, row + i, sizeof v);
+    return v;
+}
+
+// ====================================================================
+// Wiscombe's angular recurrence in deviation form (FP64)
+// ====================================================================
+// theory/LorenzMie.py:279-281, 311-312 compute pi_n(c) by
+//   S = pi_n c,  T = S - pi_{n-1},  tau_n = pi_{n-1} - n T,
+//   pi_{n+1} = S + (1 + 1/n) T.
+// Near the axis pi_n is ~n^4/8 times as sensitive to c as it is large, and an
+// FP32 cos(theta) has already lost the digits of 1 - c.  With a_n = n(n+1)/2
+// (= pi_n(1)), w = 1 - c (from rho^2 / (R + Z)) and d_n = a_n - pi_n, the same
+// recurrence becomes, exactly,
+//   Td = (d_n - d_{n-1}) + w pi_n,   tau_n = n Td - (a_n + d_{n-1}),
+//   d_{n+1} = d_n + w pi_n + (1 + 1/n) Td,   pi_{n+1} = a_{n+1} - d_{n+1},
+// whose rounding errors scale with d_n, not a_n.  It is carried in FP64
+// (the same VALU rate as unpacked FP32 on gfx950); the pixel sum uses the
+// FP32 roundings of pi_n and tau_n.
+struct Angular {
+    double dm, dn, pn;
+    LMZ_HD void init() { dm = 0.0; dn = 0.0; pn = 1.0; }
+    // returns pi_n, tau_n of the current order and advances to n + 1
+    LMZ_HD void step_d(double w, const float *row, double &pi, double &tau) {
+        const double fn = (double)row[6];
+        const double c1 = row_d(row, 8), an = row_d(row, 10), an1 = row_d(row, 12);
+        const double wp = w * pn;
+        const double Td = (dn - dm) + wp;
+        pi = pn;
+        tau = fma(fn, Td, -(an + dm));
+        const double dnext = fma(c1, Td, dn + wp);
+        dm = dn;
+        dn = dnext;
+        pn = an1 - dnext;
+    }
+    LMZ_HD void step(double w, const float *row, float &pi, float &tau) {
+        const double fn = (double)row[6];
+        const double c1 = row_d(row, 8), an = row_d(row, 10), an1 = row_d(row, 12);
+        const double wp = w * pn;
+        const double Td = (dn - dm) + wp;
+        pi = (float)pn;
+        tau = (float)fma(fn, Td, -(an + dm));
+        const double dnext = fma(c1, Td, dn + wp);
+        dm = dn;
+        dn = dnext;
+        pn = an1 - dnext;
+    }
+};
+
+// ====================================================================
+// pixel geometry in FP64 for the forward model
+// ====================================================================
+// The hologram is formed in FP64 end to end (geometry, both recurrences,
+// products, sums, projection): gfx950 issues FP64 FMA at the rate of
+// unpacked FP32 FMA, and FP32 geometry or recurrences leave errors above
+// 1e-5 on bright forward-scattering lobes (I ~ 40).  Same conventions as
+// pixel_geometry (phase folded into the start, mirror for cos theta < 0).
+struct PixGeomD {
+    double R, invR, rho, c, s, u, v;
+    double w, invRl, mir;
+    cd xi_m1, xi_0;
+};
+
+LMZ_HD void pixel_geometry_d(const FeatUniform &fu, float ax, float ay,
+                             float az, bool bohren, PixGeomD &g) {
+    const double dx = ((double)ax - (double)fu.ix) - fu.fxd;
+    const double dy = ((double)ay - (double)fu.iy) - fu.fyd;
+    const double dz = ((double)fu.iz - (double)az) + fu.fzd;
+    const double X = fu.kd * dx, Y = fu.kd * dy, Z = fu.kd * dz;
+    const double rho2 = fma(X, X, Y * Y);
+    g.rho = sqrt(rho2);
+    g.R = sqrt(fma(Z, Z, rho2));
+    g.invR = 1.0 / g.R;
+    g.c = Z * g.invR;
+    g.s = g.rho * g.invR;
+    if (g.rho > 0.0) {
+        const double ir = 1.0 / g.rho;
+        g.u = X * ir;
+        g.v = Y * ir;
+    } else {  // arctan2(0, 0) = 0
+        g.u = 1.0;
+        g.v = 0.0;
+    }
+    const double sz = (Z > 0.0) ? 1.0 : ((Z < 0.0) ? -1.0 : 0.0);
+    const double sig = bohren ? sz : -sz;
+    const double kzpix = fma(fu.kd, (double)az, fu.kz0d);
+    const double az_ = fabs(Z);
+    const double t = rho2 / (g.R + az_);               // R - |Z|
+    g.w = t * g.invR;
+    g.mir = (Z < 0.0) ? -1.0 : 1.0;
+    g.invRl = g.mir * g.invR;
+    if (sig != 0.0) {
+        double psi;
+        if (sig > 0.0) psi = ((Z > 0.0) ? t : g.R - Z) - kzpix;
+        else psi = -(((Z < 0.0) ? t : g.R + Z) + kzpix);
+        double sn, cs;
+        sincos(psi, &sn, &cs);
+        g.xi_m1 = {cs, sn};
+        g.xi_0 = {sig * sn, -sig * cs};
+    } else {
+        double sr, cr, sp, cp;
+        sincos(g.R, &sr, &cr);
+        sincos(kzpix, &sp, &cp);
+        g.xi_m1 = {cr * cp, -cr * sp};
+        g.xi_0 = {sr * cp, -sr * sp};
+    }
+    g.xi_0 = g.mir * g.xi_0;
+}
+
+// ====================================================================
+// forward multipole sum, theory/LorenzMie.py:272-316 (FP64)
+// ====================================================================
+// With tau = -tau_n (the reference's sign),
+//   S_r += PR pi xi,  S_t += P tau D - Q pi xi,  S_p += P pi D - Q tau xi
+// and the geometric factors of :322-325 applied in field_from_sums.
+struct SumsD { cd r, t, p; };
+
+LMZ_HD void cdmac(cd &acc, double s, cd a) {
+    acc.re = fma(s, a.re, acc.re);
+    acc.im = fma(s, a.im, acc.im);
+}
+
+LMZ_HD SumsD forward_sum(const PixGeomD &g, const float *coef, int nmax) {
+    cd Sr{0.0, 0.0}, St{0.0, 0.0}, Sp{0.0, 0.0};
+    Angular ang;
+    ang.init();
+    cd x2 = g.xi_m1, x1 = g.xi_0;
+    const double invR = g.invRl, w = g.w;
+    for (int n = 1; n <= nmax; ++n) {
+        const float *cp = coef + LMZ_COEF_STRIDE * (n - 1);
+        const cd PR{row_d(cp, 16), row_d(cp, 18)};
+        const cd P{row_d(cp, 20), row_d(cp, 22)};
+        const cd Q{row_d(cp, 24), row_d(cp, 26)};
+        const double fn = row_d(cp, 14);
+        double pn, tau;
+        ang.step_d(w, cp, pn, tau);
+        const double ninv = fn * invR;
+        const double cx = fma(2.0, ninv, -invR);          // (2n - 1) / kr
+        const cd xi{fma(cx, x1.re, -x2.re), fma(cx, x1.im, -x2.im)};
+        const cd D{fma(ninv, xi.re, -x1.re), fma(ninv, xi.im, -x1.im)};
+        const cd PD = P * D, QX = Q * xi, PRX = PR * xi;
+        cdmac(Sr, pn, PRX);
+        cdmac(St, tau, PD);
+        cdmac(St, -pn, QX);
+        cdmac(Sp, pn, PD);
+        cdmac(Sp, -tau, QX);
+        x2 = x1;
+        x1 = xi;
+    }
+    return SumsD{Sr, St, g.mir * Sp};
+}
+
+// Spherical -> field components.  Cartesian (theory/LorenzMie.py:331-340):
+//   A = S_r s^2/R^2 + S_t c/R,  B = S_p / R,  C = S_r c/R^2 - S_t / R
+//   E_x = u^2 A - v^2 B,  E_y = u v (A + B),  E_z = u s C
+// Spherical (:322-325): (S_r u s / R^2, S_t u / R, S_p v / R).
+LMZ_HD void field_from_sums(const PixGeomD &g, const SumsD &S, bool cart,
+                            cd (&E)[3]) {
+    const double invR2 = g.invR * g.invR;
+    if (cart) {
+        const double s2 = g.s * g.s;
+        const cd A = (s2 * invR2) * S.r + (g.c * g.invR) * S.t;
+        const cd B = g.invR * S.p;
+        const cd C = (g.c * invR2) * S.r - g.invR * S.t;
+        E[0] = (g.u * g.u) * A - (g.v * g.v) * B;
+        E[1] = (g.u * g.v) * (A + B);
+        E[2] = (g.u * g.s) * C;
+    } else {
+        E[0] = (g.u * g.s * invR2) * S.r;
+        E[1] = (g.u * g.invR) * S.t;
+        E[2] = (g.v * g.invR) * S.p;
+    }
+}
+
+// |x_hat + alpha E|^2, theory/LorenzMie.py:131-133 with alpha
+LMZ_HD double intensity(const cd (&E)[3], double alpha) {
+    const double e0r = fma(alpha, E[0].re, 1.0), e0i = alpha * E[0].im;
+    const double a2 = alpha * alpha;
+    return fma(e0r, e0r, e0i * e0i) +
+           a2 * (fma(E[1].re, E[1].re, E[1].im * E[1].im) +
+                 fma(E[2].re, E[2].re, E[2].im * E[2].im));
+}
+
+// ====================================================================
+// forward-mode Jacobian of one pixel
+// ====================================================================
+// Tangents are recomputed alongside the recurrences, never stored:
+//   d xi_n / dR = -D_n,  d D_n / dR = (1 - n(n+1)/R^2) xi_n  (Riccati-Bessel
+//   equation), d pi_n / dc and d tau_n / dc by differentiating Wiscombe's
+//   recurrence, and the coefficient tangents dcoef for a_p, n_p, k_p.
+// The geometric factors are then differentiated through
+// (R, c, u, v, s)(X, Y, Z); the only 1/rho term, K = (A + B)/rho, vanishes
+// on the axis (A + B = O(rho^2)), where dI/dX = dI/dY = 0 by the mirror
+// symmetries of the hologram.
+struct JacPixel {
+    cf E[3];       // field without alpha
+    float I;       // intensity
+    float J[LMZ_NJAC];
+};
+
+#define LMZ_RHO_AXIS 3.0e-4f   // k rho below which K is taken as 0
+
+LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
+                           const float *coef, const float *dcoef, int nmax,
+                           JacPixel &out) {
+    cf Sr{0.f, 0.f}, St{0.f, 0.f}, Sp{0.f, 0.f};
+    cf SrR{0.f, 0.f}, StR{0.f, 0.f}, SpR{0.f, 0.f};
+    cf Src{0.f, 0.f}, Stc{0.f, 0.f}, Spc{0.f, 0.f};
+    cf Tr[3], Tt[3], Tp[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) Tr[q] = Tt[q] = Tp[q] = cf{0.f, 0.f};
+    Angular ang;
+    ang.init();
+    const double w = (double)g.w;
+    float dpm = 0.f, dpn = 0.f;   // d pi_{n-1} / dc, d pi_n / dc
+    cf x2 = g.xi_m1, x1 = g.xi_0;
+    const float cl = g.cl, invRl = g.invRl, invR2 = g.invR * g.invR;
+    for (int n = 1; n <= nmax; ++n) {
+        const float *cp = coef + LMZ_COEF_STRIDE * (n - 1);
+        const float *dp = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
+        const cf PR{cp[0], cp[1]}, P{cp[2], cp[3]}, Q{cp[4], cp[5]};
+        const float fn = cp[6], nn1 = cp[7];
+        const float c1 = (float)row_d(cp, 8);
+        float pn, tau;
+        ang.step(w, cp, pn, tau);
+        // tangent of Wiscombe's recurrence with respect to c
+        const float dSv = fmaf(dpn, cl, pn);
+        const float dT = dSv - dpm;
+        const float dtau = fmaf(-fn, dT, dpm);
+        const float ninv = fn * invRl;
+        const float cx = fmaf(2.f, ninv, -invRl);
+        const cf xi{fmaf(cx, x1.re, -x2.re), fmaf(cx, x1.im, -x2.im)};
+        const cf D{fmaf(ninv, xi.re, -x1.re), fmaf(ninv, xi.im, -x1.im)};
+        const float gn = fmaf(-nn1, invR2, 1.f);
+        const cf PD = cmul(P, D), QX = cmul(Q, xi), PRX = cmul(PR, xi);
+        const cf PX = cmul(P, xi), QD = cmul(Q, D), PRD = cmul(PR, D);
+        // value
+        cmac(Sr, pn, PRX);
+        cmac(St, tau, PD);
+        cmac(St, -pn, QX);
+        cmac(Sp, pn, PD);
+        cmac(Sp, -tau, QX);
+        // d/dR
+        cmac(SrR, -pn, PRD);
+        const float tg = tau * gn, pg = pn * gn;
+        cmac(StR, tg, PX);
+        cmac(StR, pn, QD);
+        cmac(SpR, pg, PX);
+        cmac(SpR, tau, QD);
+        // d/dc
+        cmac(Src, dpn, PRX);
+        cmac(Stc, dtau, PD);
+        cmac(Stc, -dpn, QX);
+        cmac(Spc, dpn, PD);
+        cmac(Spc, -dtau, QX);
+        // coefficient tangents
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float *d = dp + 8 * q;
+            const cf a1 = cmul(cf{d[0], d[1]}, xi);
+            const cf a2 = cmul(cf{d[2], d[3]}, D);
+            const cf a3 = cmul(cf{d[4], d[5]}, xi);
+            cmac(Tr[q], pn, a1);
+            cmac(Tt[q], tau, a2);
+            cmac(Tt[q], -pn, a3);
+            cmac(Tp[q], pn, a2);
+            cmac(Tp[q], -tau, a3);
+        }
+        dpm = dpn;
+        dpn = fmaf(c1, dT, dSv);
+        x2 = x1;
+        x1 = xi;
+    }
+
+    // ---- undo the mirror (S_p, and d/dR, d/dc of S_r and S_t, change sign)
+    {
+        const float m = g.mir;
+        Sp = m * Sp;
+        SrR = m * SrR; StR = m * StR;
+        Src = m * Src; Stc = m * Stc;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) Tp[q] = m * Tp[q];
+    }
+    // ---- field and its (R, c) partials
+    const float s = g.s, s2 = g.s * g.s, u = g.u, v = g.v;
+    const float c = g.c, invR = g.invR;
+    const float u2 = u * u, v2 = v * v, uv = u * v;
+    const float a_r = s2 * invR2, a_t = c * invR;          // A = a_r Sr + a_t St
+    const float c_r = c * invR2;                            // C = c_r Sr - invR St
+    const cf A = a_r * Sr + a_t * St;
+    const cf B = invR * Sp;
+    const cf C = c_r * Sr - invR * St;
+    out.E[0] = u2 * A - v2 * B;
+    out.E[1] = uv * (A + B);
+    out.E[2] = (u * s) * C;
+
+    const float twoR = 2.f * invR;
+    const cf A_R = a_r * SrR - (a_r * twoR) * Sr + a_t * StR - (a_t * invR) * St;
+    const cf A_c = a_r * Src - ((2.f * c) * invR2) * Sr + a_t * Stc + invR * St;
+    const cf B_R = invR * SpR - invR2 * Sp;
+    const cf B_c = invR * Spc;
+    const cf C_R = c_r * SrR - (c_r * twoR) * Sr - invR * StR + invR2 * St;
+    const cf C_c = c_r * Src + invR2 * Sr - invR * Stc;
+
+    // chain rule to X, Y, Z
+    const float R_X = s * u, R_Y = s * v, R_Z = c;
+    const float csR = c * s * invR;
+    const float c_X = -csR * u, c_Y = -csR * v, c_Z = s2 * invR;
+    const cf A_X = R_X * A_R + c_X * A_c, A_Y = R_Y * A_R + c_Y * A_c,
+             A_Z = R_Z * A_R + c_Z * A_c;
+    const cf B_X = R_X * B_R + c_X * B_c, B_Y = R_Y * B_R + c_Y * B_c,
+             B_Z = R_Z * B_R + c_Z * B_c;
+    const cf C_X = R_X * C_R + c_X * C_c, C_Y = R_Y * C_R + c_Y * C_c,
+             C_Z = R_Z * C_R + c_Z * C_c;
+    const cf K = (g.rho > LMZ_RHO_AXIS) ? (1.f / g.rho) * (A + B) : cf{0.f, 0.f};
+    const float us = u * s;
+    cf dE[3][3];  // [q][component]
+    dE[0][0] = (2.f * u * v2) * K + u2 * A_X - v2 * B_X;
+    dE[1][0] = (-2.f * u2 * v) * K + u2 * A_Y - v2 * B_Y;
+    dE[2][0] = u2 * A_Z - v2 * B_Z;
+    dE[0][1] = (v * (v2 - u2)) * K + uv * (A_X + B_X);
+    dE[1][1] = (u * (u2 - v2)) * K + uv * (A_Y + B_Y);
+    dE[2][1] = uv * (A_Z + B_Z);
+    dE[0][2] = ((v2 + u2 * c * c) * invR) * C + us * C_X;
+    dE[1][2] = (-uv * s2 * invR) * C + us * C_Y;
+    dE[2][2] = (-us * c * invR) * C + us * C_Z;
+
+    // ---- intensity and its derivatives
+    const float al = fu.alpha;
+    const cf F0{fmaf(al, out.E[0].re, 1.f), al * out.E[0].im};
+    const cf F1 = al * out.E[1], F2 = al * out.E[2];
+    out.I = credot(F0, F0) + credot(F1, F1) + credot(F2, F2);
+    const float two_al = 2.f * al;
+    float dI[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        dI[q] = two_al * (credot(F0, dE[q][0]) + credot(F1, dE[q][1]) +
+                          credot(F2, dE[q][2]));
+    const float k = fu.k;
+    out.J[0] = -k * dI[0];
+    out.J[1] = -k * dI[1];
+    out.J[2] = k * fmaf(two_al, out.E[0].im, dI[2]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const cf Aq = a_r * Tr[q] + a_t * Tt[q];
+        const cf Bq = invR * Tp[q];
+        const cf Cq = c_r * Tr[q] - invR * Tt[q];
+        const cf Ex = u2 * Aq - v2 * Bq, Ey = uv * (Aq + Bq), Ez = us * Cq;
+        out.J[3 + q] = two_al * (credot(F0, Ex) + credot(F1, Ey) + credot(F2, Ez));
+    }
+    out.J[6] = 2.f * (credot(F0, out.E[0]) + credot(F1, out.E[1]) +
+                      credot(F2, out.E[2]));
+}
+
+// ====================================================================
+// Mie coefficients in FP64 with a_p, n_p, k_p tangents
+// ====================================================================
+// Sphere.wiscombe_yang, theory/Sphere.py:91-123 (one layer: the rolled
+// term equals |x m|)
+LMZ_HD int wiscombe_yang(double x, cd m) {
+    const double xl = fabs(x);
+    double ns;
+    if (xl <= 8.0) ns = floor(xl + 4.0 * cbrt(xl) + 1.0);
+    else if (xl <= 4200.0) ns = floor(xl + 4.05 * cbrt(xl) + 2.0);
+    else ns = floor(xl + 4.0 * cbrt(xl) + 2.0);
+    const double xm = hypot(x * m.re, x * m.im);
+    return (int)fmax(ns, xm);
+}
+
+// Output of one order: FP32 table rows and optional FP64 a_n, b_n.
+LMZ_HD void put_d(float *row, int i, double v) { __builtin_memcpy(row + i, &v, sizeof v); }
+
+LMZ_HD void write_order(int n, cd a, cd b, float *crow) {
+    const double nn1 = (double)n * (n + 1.0);
+    const double en = (2.0 * n + 1.0) / nn1;
+    const cd P = mul_ipow(n + 1, en * a);           // i E_n a_n
+    const cd Q = mul_ipow(n, en * b);               // E_n b_n
+    const cd PR = nn1 * P;                          // i^(n+1) (2n+1) a_n
+    crow[0] = (float)PR.re; crow[1] = (float)PR.im;
+    crow[2] = (float)P.re;  crow[3] = (float)P.im;
+    crow[4] = (float)Q.re;  crow[5] = (float)Q.im;
+    crow[6] = (float)n;
+    crow[7] = (float)nn1;
+    put_d(crow, 8, 1.0 + 1.0 / n);
+    put_d(crow, 10, 0.5 * nn1);
+    put_d(crow, 12, 0.5 * (n + 1.0) * (n + 2.0));
+    put_d(crow, 14, (double)n);
+    put_d(crow, 16, PR.re); put_d(crow, 18, PR.im);
+    put_d(crow, 20, P.re);  put_d(crow, 22, P.im);
+    put_d(crow, 24, Q.re);  put_d(crow, 26, Q.im);
+    for (int i = 28; i < LMZ_COEF_STRIDE; ++i) crow[i] = 0.f;
+}
+LMZ_HD void write_dorder(int n, cd da, cd db, float *drow) {
+    const double en = (2.0 * n + 1.0) / ((double)n * (n + 1.0));
+    const cd P = mul_ipow(n + 1, en * da);
+    const cd Q = mul_ipow(n, en * db);
+    const cd PR = ((double)n * (n + 1.0)) * P;
+    drow[0] = (float)PR.re; drow[1] = (float)PR.im;
+    drow[2] = (float)P.re;  drow[3] = (float)P.im;
+    drow[4] = (float)Q.re;  drow[5] = (float)Q.im;
+    drow[6] = 0.f;          drow[7] = 0.f;
+}
+
+// Homogeneous sphere: the one-layer recursion of theory/Sphere.py:147-226
+// (D1 downward from D1[nmax] = 0 for z = m x and z = x; Psi, zeta, Psi*zeta,
+// D3 upward; Eqs. 5 and 6), carried with tangents d/dx (real) and d/dm
+// (holomorphic), then mapped to a_p (x = k a_p), n_p and k_p
+// (m = (n_p + i k_p) / n_m).  work(n, j) = work[(n * 6 + j) * wstride].
+// Returns n_max (clamped to cap).
+LMZ_HD int mie_feature(double a_p, double n_p, double k_p, double n_m,
+                       double wavelength, int cap, double *work,
+                       long long wstride, float *coef, float *dcoef,
+                       double *ab) {
+    const double kum = 2.0 * M_PI / wavelength * n_m;   // rad / um in medium
+    const double x = kum * a_p;
+    const cd m{n_p / n_m, k_p / n_m};
+    int N = wiscombe_yang(x, m);
+    if (N > cap) N = cap;
+    if (N < 1) N = 1;
+#define W(n, j) work[((long long)(n) * 6 + (j)) * wstride]
+    // downward D1(z), z = m x, with dD1/dz; D1(x) with dD1/dx
+    {
+        const cd z = x * m;
+        const cd iz = cinv(z);
+        cd d1{0.0, 0.0}, dd1{0.0, 0.0};
+        double r1 = 0.0, dr1 = 0.0;
+        W(N, 0) = 0.0; W(N, 1) = 0.0; W(N, 2) = 0.0; W(N, 3) = 0.0;
+        W(N, 4) = 0.0; W(N, 5) = 0.0;
+        for (int n = N; n >= 1; --n) {
+            const cd nz = (double)n * iz;                 // n / z
+            const cd dnz = -((double)n * (iz * iz));      // -n / z^2
+            const cd inv = cinv(d1 + nz);
+            const cd d1n = nz - inv;
+            dd1 = dnz + (dd1 + dnz) * (inv * inv);
+            d1 = d1n;
+            const double nx = n / x, dnx = -n / (x * x);
+            const double rinv = 1.0 / (r1 + nx);
+            const double r1n = nx - rinv;
+            dr1 = dnx + (dr1 + dnx) * (rinv * rinv);
+            r1 = r1n;
+            W(n - 1, 0) = d1.re; W(n - 1, 1) = d1.im;
+            W(n - 1, 2) = dd1.re; W(n - 1, 3) = dd1.im;
+            W(n - 1, 4) = r1; W(n - 1, 5) = dr1;
+        }
+    }
+    // upward Psi, zeta, Psi zeta, D3 with d/dx
+    double sx, cx;
+    sincos(x, &sx, &cx);
+    double s2x, c2x;
+    sincos(2.0 * x, &s2x, &c2x);
+    double psi = sx, dpsi = cx;                      // Eq. 20a
+    cd zeta{sx, -cx}, dzeta{cx, sx};                 // -i e^{ix}, Eq. 21a
+    cd pz{0.5 * (1.0 - c2x), -0.5 * s2x};            // Eq. 18a
+    cd dpz{s2x, -c2x};                               // d/dx = -i e^{2ix}
+    cd d3{0.0, 1.0}, dd3{0.0, 0.0};                  // Eq. 18b
+    const cd m2 = m * m;
+    const cd im = cinv(m);
+    for (int n = 1; n <= N; ++n) {
+        const double nx = n / x, dnx = -n / (x * x);
+        const double w1 = nx - W(n - 1, 4);
+        const double dw1 = dnx - W(n - 1, 5);
+        const cd w3 = cd{nx, 0.0} - d3;
+        const cd dw3 = cd{dnx, 0.0} - dd3;
+        const double psi_m = psi, dpsi_m = dpsi;
+        const cd zeta_m = zeta, dzeta_m = dzeta;
+        psi = psi_m * w1;                              // Eq. 20b
+        dpsi = dpsi_m * w1 + psi_m * dw1;
+        zeta = zeta_m * w3;                            // Eq. 21b
+        dzeta = dzeta_m * w3 + zeta_m * dw3;
+        const cd pz_m = pz;
+        pz = pz_m * (w1 * w3);                         // Eq. 18c
+        dpz = dpz * (w1 * w3) + pz_m * (dw1 * w3 + w1 * dw3);
+        const cd ipz = cinv(pz);
+        d3 = cd{W(n, 4), 0.0} + mul_i(ipz);            // Eq. 18d
+        dd3 = cd{W(n, 5), 0.0} - mul_i(dpz * (ipz * ipz));
+        // coefficients, Eqs. 5 and 6 (H_a = H_b = D1(m x))
+        const cd H{W(n, 0), W(n, 1)};
+        const cd dH{W(n, 2), W(n, 3)};                 // dD1/dz at z = m x
+        const cd fa = H * im + cd{nx, 0.0};
+        const cd fa_x = dH + cd{dnx, 0.0};             // (dH m)/m - n/x^2
+        const cd fa_m = x * (dH * im) - H * (im * im);
+        const cd numA = fa * cd{psi, 0.0} - cd{psi_m, 0.0};
+        const cd denA = fa * zeta - zeta_m;
+        const cd idA = cinv(denA);
+        const cd a = numA * idA;
+        const cd a_x = (fa_x * cd{psi, 0.0} + fa * cd{dpsi, 0.0} -
+                        cd{dpsi_m, 0.0} -
+                        a * (fa_x * zeta + fa * dzeta - dzeta_m)) * idA;
+        const cd a_m = fa_m * (cd{psi, 0.0} - a * zeta) * idA;
+        const cd fb = m * H + cd{nx, 0.0};
+        const cd fb_x = m2 * dH + cd{dnx, 0.0};
+        const cd fb_m = H + x * (m * dH);
+        const cd numB = fb * cd{psi, 0.0} - cd{psi_m, 0.0};
+        const cd denB = fb * zeta - zeta_m;
+        const cd idB = cinv(denB);
+        const cd b = numB * idB;
+        const cd b_x = (fb_x * cd{psi, 0.0} + fb * cd{dpsi, 0.0} -
+                        cd{dpsi_m, 0.0} -
+                        b * (fb_x * zeta + fb * dzeta - dzeta_m)) * idB;
+        const cd b_m = fb_m * (cd{psi, 0.0} - b * zeta) * idB;
+        if (coef) write_order(n, a, b, coef + LMZ_COEF_STRIDE * (n - 1));
+        if (dcoef) {
+            float *drow = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
+            write_dorder(n, kum * a_x, kum * b_x, drow);                 // a_p
+            write_dorder(n, (1.0 / n_m) * a_m, (1.0 / n_m) * b_m, drow + 8);   // n_p
+            write_dorder(n, mul_i((1.0 / n_m) * a_m), mul_i((1.0 / n_m) * b_m),
+                         drow + 16);                                     // k_p
+        }
+        if (ab) {
+            double *r = ab + 4 * (n - 1);
+            r[0] = a.re; r[1] = a.im; r[2] = b.re; r[3] = b.im;
+        }
+    }
+#undef W
+    return N;
+}
+
+}  // namespace lmz

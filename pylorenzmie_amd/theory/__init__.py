@@ -1,0 +1,16 @@
+"""Forward model: instrument, particles and the GPU Lorenz-Mie calculator.
+
+Mirrors pylorenzmie.theory (theory/__init__.py:1-44).  There is one
+backend, so ``LorenzMie`` is the HIP calculator itself; ``hipLorenzMie``
+and ``LMHologram`` are aliases.
+"""
+from .Instrument import Instrument
+from .Particle import Particle
+from .Sphere import Sphere
+from .LorenzMie import LorenzMie, PARAMS
+
+hipLorenzMie = LorenzMie
+LMHologram = LorenzMie
+
+__all__ = ['Instrument', 'Particle', 'Sphere', 'LorenzMie', 'hipLorenzMie',
+           'LMHologram', 'PARAMS']

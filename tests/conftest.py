@@ -1,0 +1,31 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, 'tests', 'golden')
+
+
+def pytest_configure(config):
+    config.addinivalue_line('markers', 'gpu: needs an MI355X (ROCm) GPU')
+
+
+@pytest.fixture(scope='session')
+def forward_golden():
+    return np.load(os.path.join(GOLDEN, 'forward.npz'))
+
+
+@pytest.fixture(scope='session')
+def fits_golden():
+    import json
+    with open(os.path.join(GOLDEN, 'fits.json')) as f:
+        meta = json.load(f)
+    return np.load(os.path.join(GOLDEN, 'fits.npz')), meta
+
+
+INSTRUMENT = dict(n_m=1.34, wavelength=0.447, magnification=0.048)

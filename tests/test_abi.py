@@ -1,0 +1,81 @@
+"""The C-ABI library loads, exports every symbol include/lmz.h declares, and
+rejects bad arguments on the host before any launch (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, 'include', 'lmz.h')
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(lmz_\w+)\s*\(', text)))
+
+
+@pytest.fixture(scope='module')
+def lib():
+    from pylorenzmie_amd import _lmz
+    return _lmz.load_library()
+
+
+def test_exports_every_declared_symbol(lib):
+    from pylorenzmie_amd import _lmz
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lmz.SIGNATURES, name
+
+
+def test_version(lib):
+    assert lib.lmz_version() == 100
+
+
+def test_struct_layouts_match_header():
+    from pylorenzmie_amd import _lmz
+    assert ctypes.sizeof(_lmz.Feature) == 88
+    assert ctypes.sizeof(_lmz.BatchSizes) == 48
+    assert ctypes.sizeof(_lmz.LMSettings) == 48
+
+
+def _feature(**kw):
+    from pylorenzmie_amd import _lmz
+    f = _lmz.Feature()
+    f.npix, f.width, f.nmax_cap = 100, 10, 30
+    f.n_m, f.wavelength, f.magnification, f.noise = 1.34, 0.447, 0.048, 0.05
+    for k, v in kw.items():
+        setattr(f, k, v)
+    return f
+
+
+@pytest.mark.parametrize('bad', [dict(npix=0), dict(width=-1),
+                                 dict(width=100000), dict(nmax_cap=0),
+                                 dict(noise=0.), dict(wavelength=-1.)])
+def test_batch_create_rejects_bad_features(lib, bad):
+    from pylorenzmie_amd import _lmz
+    arr = (_lmz.Feature * 1)(_feature(**bad))
+    h = ctypes.c_void_p()
+    rc = lib.lmz_batch_create(1, ctypes.cast(arr, ctypes.c_void_p), 1, ctypes.byref(h))
+    assert rc == 1
+    assert b'invalid feature' in lib.lmz_last_error()
+    assert not h.value
+
+
+def test_batch_create_rejects_overlapping_pixels(lib):
+    from pylorenzmie_amd import _lmz
+    arr = (_lmz.Feature * 2)(_feature(), _feature(pix_offset=50))
+    h = ctypes.c_void_p()
+    assert lib.lmz_batch_create(2, ctypes.cast(arr, ctypes.c_void_p), 1,
+                                ctypes.byref(h)) == 1
+
+
+def test_calls_reject_null_batch(lib):
+    assert lib.lmz_mie_ab(None, None, None, None, None, None, None, None, None) == 1
+    assert b'batch is NULL' in lib.lmz_last_error()
+    assert lib.lmz_hologram(None, *([None] * 5), 3, *([None] * 5)) == 1
+    assert lib.lmz_lm_run(None, None, 3, None, None, None, None, None, None) == 1

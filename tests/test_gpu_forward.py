@@ -1,0 +1,159 @@
+"""GPU parity of the forward model against the reference's golden vectors
+and the FP64 oracle.  Tolerances: normalized hologram max abs error
+<= 1e-5; field components <= 2e-6 absolute; Mie coefficients (FP64 on the
+GPU) <= 1e-11 absolute."""
+import numpy as np
+import pytest
+
+from conftest import INSTRUMENT
+from oracle import lorenzmie as ref
+
+pytestmark = pytest.mark.gpu
+
+HOLO_TOL = 1e-5
+FIELD_TOL = 2e-6
+
+
+@pytest.fixture(scope='module')
+def api():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no GPU')
+    from pylorenzmie_amd import theory, analysis
+    return theory, analysis
+
+
+def sphere(theory, p):
+    return theory.Sphere(x_p=p[0], y_p=p[1], z_p=p[2], a_p=p[3], n_p=p[4],
+                         k_p=p[5])
+
+
+def model(theory, coords, p, alpha=1.0):
+    return theory.LorenzMie(coordinates=coords, particle=sphere(theory, p),
+                            instrument=theory.Instrument(**INSTRUMENT),
+                            alpha=alpha)
+
+
+def test_mie_coefficients_golden(api, forward_golden):
+    theory, _ = api
+    g = forward_golden
+    offs = g['ab_offsets']
+    for i, (a, n, k) in enumerate(g['ab_cases']):
+        ab = theory.Sphere(a_p=a, n_p=n, k_p=k).ab(1.34, 0.447)
+        gold = g['ab_values'][offs[i]:offs[i + 1]]
+        assert ab.shape == gold.shape
+        assert ab.dtype == np.complex128
+        np.testing.assert_array_equal(ab[0], 0j)
+        assert np.abs(ab - gold).max() <= 1e-11, (a, n, k)
+
+
+def test_hologram_config0_golden(api, forward_golden):
+    theory, _ = api
+    g = forward_golden
+    m = model(theory, ref.meshgrid((201, 201)), g['cfg0_params'])
+    assert m._width == 201            # dense crop path
+    h = m.hologram()
+    assert h.dtype == np.float64 and h.shape == (201 * 201,)
+    err = np.abs(h - g['cfg0_holo']).max()
+    print(f'config-0 hologram max abs error {err:.3e}')
+    assert err <= HOLO_TOL
+
+
+@pytest.mark.parametrize('cart', [True, False])
+@pytest.mark.parametrize('bohren', [True, False])
+def test_field_flags_golden(api, forward_golden, cart, bohren):
+    theory, _ = api
+    g = forward_golden
+    m = model(theory, ref.meshgrid((64, 64)), g['field64_params'])
+    f = m.field(cartesian=cart, bohren=bohren)
+    gold = g[f'field64_c{int(cart)}_b{int(bohren)}']
+    err = np.abs(f - gold).max()
+    print(f'field cart={cart} bohren={bohren} max abs error {err:.3e}')
+    assert err <= FIELD_TOL
+    h = m.hologram(cartesian=cart, bohren=bohren)
+    gold_h = np.sum(np.abs(gold + np.array([1, 0, 0])[:, None]) ** 2, axis=0)
+    assert np.abs(h - gold_h).max() <= HOLO_TOL
+
+
+def test_explicit_coordinate_cases_golden(api, forward_golden):
+    """Frame offsets, strided coordinate lists, a particle below the focal
+    plane, pixels off z = 0 and a tiny sphere."""
+    theory, _ = api
+    g = forward_golden
+    offs = g['holo_offsets']
+    for i, p in enumerate(g['holo_cases']):
+        c = g['holo_coords'][:, offs[i]:offs[i + 1]]
+        m = model(theory, c, p)
+        err = np.abs(m.hologram() - g['holo_values'][offs[i]:offs[i + 1]]).max()
+        print(f'case {i} max abs error {err:.3e}')
+        assert err <= HOLO_TOL, i
+
+
+def test_two_sphere_superposition_golden(api, forward_golden):
+    theory, _ = api
+    g = forward_golden
+    parts = [sphere(theory, q) for q in g['pair_params']]
+    m = theory.LorenzMie(coordinates=ref.meshgrid((48, 48)), particle=parts,
+                         instrument=theory.Instrument(**INSTRUMENT))
+    assert np.abs(m.hologram() - g['pair_holo']).max() <= HOLO_TOL
+
+
+def test_alpha_closed_form(api):
+    """I(alpha) = 1 + 2 alpha Re(E_x) + alpha^2 |E|^2 from the alpha = 1 field."""
+    theory, _ = api
+    p = (20.3, 19.4, 90., 1.1, 1.5, 0.002)
+    c = ref.meshgrid((40, 40))
+    e = model(theory, c, p).field().copy()
+    for alpha in (0.0, 0.7, 1.3):
+        h = model(theory, c, p, alpha).hologram()
+        expect = 1 + 2 * alpha * e[0].real + alpha ** 2 * np.sum(np.abs(e) ** 2, 0)
+        assert np.abs(h - expect).max() <= HOLO_TOL
+    assert np.all(model(theory, c, p, 0.0).hologram() == 1.0)
+
+
+def test_mirror_symmetry(api):
+    theory, _ = api
+    h = model(theory, ref.meshgrid((81, 81)), (40., 40., 120., 1.2, 1.55, 0.))
+    h = h.hologram().reshape(81, 81)
+    assert np.abs(h - h[:, ::-1]).max() <= 2e-6
+    assert np.abs(h - h[::-1, :]).max() <= 2e-6
+
+
+def test_masked_coordinates_match_oracle(api):
+    """Random pixel subsets (Mask) take the explicit-coordinate path."""
+    theory, analysis = api
+    rng = np.random.default_rng(5)
+    data = np.ones((120, 140))
+    mask = analysis.Mask(fraction=0.2)
+    np.random.seed(3)
+    d, c = mask.apply(analysis.Hologram(data, corner=(300., 200.)))
+    p = (370.4, 261.7, 210., 1.7, 1.42, 0.001)
+    m = model(theory, c, p)
+    assert m._width == 0
+    truth = ref.hologram(c, dict(zip(ref.PARAMS, p)), **INSTRUMENT)
+    assert np.abs(m.hologram() - truth).max() <= HOLO_TOL
+    del rng, d
+
+
+def test_batched_uneven_crops_match_oracle(api):
+    """Config-1-like batch: uneven crops in one launch."""
+    theory, _ = api
+    from pylorenzmie_amd.theory.batch import batch_hologram
+    rng = np.random.default_rng(11)
+    shapes, params = [], []
+    for _ in range(12):
+        s = int(rng.integers(32, 97))
+        a = rng.uniform(0.4, 3.0)
+        params.append(((s - 1) / 2 + rng.uniform(-5, 5),
+                       (s - 1) / 2 + rng.uniform(-5, 5),
+                       rng.uniform(50, 400), a, rng.uniform(1.38, 1.65),
+                       rng.uniform(0., 0.01), 1.0))
+        shapes.append((s, s + int(rng.integers(0, 5))))
+    holos = batch_hologram(shapes, params, theory.Instrument(**INSTRUMENT))
+    worst = 0.
+    for (ny, nx), p, h in zip(shapes, params, holos):
+        truth = ref.hologram(ref.meshgrid((ny, nx)), dict(zip(ref.PARAMS, p)),
+                             **INSTRUMENT).reshape(ny, nx)
+        worst = max(worst, np.abs(h - truth).max())
+    print(f'batched forward worst max abs error {worst:.3e}')
+    assert worst <= HOLO_TOL

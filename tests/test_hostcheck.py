@@ -1,0 +1,154 @@
+"""CPU checks of the kernel arithmetic (pylorenzmie_amd/csrc/lmz_pixel.h)
+compiled for the host into tests/native/liblmz_hostcheck.so, against the
+FP64 oracle and the golden vectors.  The GPU tests repeat these through
+liblmz.so on the device."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import INSTRUMENT, ROOT
+from oracle import lorenzmie as ref
+
+LIB = os.path.join(ROOT, 'tests', 'native', 'liblmz_hostcheck.so')
+COEF_STRIDE = 32
+DCOEF_STRIDE = 24
+
+
+@pytest.fixture(scope='module')
+def chk():
+    if not os.path.exists(LIB):
+        pytest.skip('host checker not built (run __graft_entry__.build())')
+    from pylorenzmie_amd._lmz import Feature
+    lib = ctypes.CDLL(LIB)
+    P = ctypes.c_void_p
+    lib.chk_mie.restype = ctypes.c_int
+    lib.chk_mie.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, P, P, P]
+    lib.chk_pixels.argtypes = [P, P, P, P, ctypes.c_int, P, ctypes.c_int,
+                               ctypes.c_int, P, P, P]
+    lib.chk_dense_coords.argtypes = [ctypes.c_int, ctypes.c_int, P, P]
+    return lib, Feature
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def mie(chk, a, n, k, cap=400):
+    lib, _ = chk
+    coef = np.zeros((cap, COEF_STRIDE), np.float32)
+    dcoef = np.zeros((cap, DCOEF_STRIDE), np.float32)
+    ab = np.zeros((cap, 4))
+    N = lib.chk_mie(a, n, k, 1.34, 0.447, cap, _p(coef), _p(dcoef), _p(ab))
+    return N, coef, dcoef, ab
+
+
+def pixels(chk, q, coords, flags=3, origin=(0., 0., 0.)):
+    lib, Feature = chk
+    N, coef, dcoef, _ = mie(chk, q[3], q[4], q[5])
+    F = Feature()
+    F.npix = coords.shape[1]
+    F.origin[:] = origin
+    F.n_m, F.wavelength, F.magnification, F.noise = 1.34, 0.447, 0.048, 0.05
+    F.nmax_cap = 400
+    p = np.zeros(8)
+    p[:len(q)] = q
+    if len(q) < 7:
+        p[6] = 1.
+    c = np.ascontiguousarray((coords - np.asarray(origin)[:, None]).astype(np.float32))
+    n = c.shape[1]
+    field = np.zeros((3, n, 2), np.float32)
+    holo = np.zeros(n, np.float32)
+    jac = np.zeros((7, n), np.float32)
+    lib.chk_pixels(ctypes.byref(F), _p(p), _p(coef), _p(dcoef), N, _p(c), n,
+                   flags, _p(field), _p(holo), _p(jac))
+    return field[..., 0] + 1j * field[..., 1], holo, jac
+
+
+def grid3(shape):
+    c = ref.meshgrid(shape)
+    return np.vstack([c, np.zeros((1, c.shape[1]))])
+
+
+def test_mie_matches_golden_and_tangents(chk, forward_golden):
+    g = forward_golden
+    offs = g['ab_offsets']
+    for i, (a, n, k) in enumerate(g['ab_cases']):
+        N, _, _, ab = mie(chk, a, n, k)
+        gold = g['ab_values'][offs[i] + 1:offs[i + 1]]
+        assert N == gold.shape[0]
+        got = np.stack([ab[:N, 0] + 1j * ab[:N, 1], ab[:N, 2] + 1j * ab[:N, 3]], 1)
+        assert np.abs(got - gold).max() < 1e-11
+    # tangents d a_n / d(a_p, n_p, k_p) against 5-point FD of the oracle
+    a, n, k = 1.3, 1.55, 0.004
+    N, _, dcoef, _ = mie(chk, a, n, k)
+    nn = np.arange(1, N + 1)
+    en = (2 * nn + 1) / (nn * (nn + 1))
+    for j in range(3):
+        h = 1e-4
+        q = np.array([a, n, k])
+        fd = 0
+        for s, w in ((1, 8 / 12), (2, -1 / 12)):
+            qp, qm = q.copy(), q.copy()
+            qp[j] += s * h
+            qm[j] -= s * h
+            fd = fd + w * (ref.mie_coefficients(*qp, 1.34, 0.447, nmax=N) -
+                           ref.mie_coefficients(*qm, 1.34, 0.447, nmax=N))
+        fd = fd[1:] / h
+        dP = dcoef[:N, 8 * j + 2] + 1j * dcoef[:N, 8 * j + 3]
+        dQ = dcoef[:N, 8 * j + 4] + 1j * dcoef[:N, 8 * j + 5]
+        da = dP / (1j ** (nn + 1) * en)
+        db = dQ / (1j ** nn * en)
+        scale = np.abs(fd).max()
+        assert np.abs(da - fd[:, 0]).max() < 1e-5 * scale
+        assert np.abs(db - fd[:, 1]).max() < 1e-5 * scale
+
+
+def test_forward_matches_golden(chk, forward_golden):
+    g = forward_golden
+    _, h, _ = pixels(chk, g['cfg0_params'], grid3((201, 201)))
+    assert np.abs(h - g['cfg0_holo']).max() < 1e-6
+    for cart in (1, 0):
+        for boh in (1, 0):
+            f, _, _ = pixels(chk, g['field64_params'], grid3((64, 64)),
+                             flags=cart + 2 * boh)
+            assert np.abs(f - g[f'field64_c{cart}_b{boh}']).max() < 2e-7
+    offs = g['holo_offsets']
+    for i, p in enumerate(g['holo_cases']):
+        c = g['holo_coords'][:, offs[i]:offs[i + 1]]
+        org = (np.round(c[0].mean()), np.round(c[1].mean()), 0.)
+        _, h, _ = pixels(chk, p, c, origin=org)
+        assert np.abs(h - g['holo_values'][offs[i]:offs[i + 1]]).max() < 1e-6, i
+
+
+def test_bright_forward_lobe(chk):
+    """A 1.6 um sphere 3 um above the focal plane (I up to ~40)."""
+    p = (17.41, 11.047, 61.918, 1.595, 1.608, 0.006)
+    c = grid3((32, 36))
+    truth = ref.hologram(c, dict(zip(ref.PARAMS, p)), **INSTRUMENT)
+    _, h, _ = pixels(chk, p, c)
+    assert truth.max() > 35
+    assert np.abs(h - truth).max() < 5e-6
+
+
+@pytest.mark.parametrize('q', [(24.3, 23.6, 140., 0.9, 1.5, 0.003, 0.9),
+                               (24.0, 24.0, 100., 2.0, 1.6, 0.0, 1.1),
+                               (23.7, 24.4, -80., 0.8, 1.45, 0.0, 1.0)])
+def test_jacobian_matches_oracle(chk, q):
+    c = grid3((40, 40))
+    _, _, jac = pixels(chk, q, c)
+    Jref = ref.jacobian(c, dict(zip(ref.PARAMS, q)), INSTRUMENT, order=6)
+    for i in range(7):
+        assert np.abs(jac[i] - Jref[i]).max() < 2e-4 * np.abs(Jref[i]).max()
+
+
+def test_dense_pixel_mapping(chk):
+    lib, _ = chk
+    for w, n in ((201, 201 * 201), (7, 7 * 13), (4096, 4096 * 3), (1, 17)):
+        ax = np.zeros(n, np.float32)
+        ay = np.zeros(n, np.float32)
+        lib.chk_dense_coords(w, n, _p(ax), _p(ay))
+        j = np.arange(n)
+        assert np.array_equal(ax, (j % w).astype(np.float32))
+        assert np.array_equal(ay, (j // w).astype(np.float32))

@@ -188,6 +188,9 @@ class Batch:
         # floats per row of the coefficient tables, as the library lays them
         self.coef_stride = sizes.coef_stride
         self.dcoef_stride = sizes.dcoef_stride
+        if self.coef_stride <= 0 or self.dcoef_stride <= 0:
+            raise LMZError('liblmz reports no coefficient row stride: the '
+                           'library predates include/lmz.h; rebuild it')
 
     def coef_floats(self) -> int:
         return self.coef_slots * self.coef_stride

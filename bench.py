@@ -234,6 +234,11 @@ def main():
     ap.add_argument('--cpu-seconds', type=float, default=15.)
     ap.add_argument('--cpu-max-fits', type=int, default=4)
     ap.add_argument('--kernel-reps', type=int, default=3)
+    ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
+                    help='also run one device copy of this many MiB (PMC '
+                         'traffic calibration) and exit')
+    ap.add_argument('--dump', default='', help='write per-feature truth, '
+                    'start, fitted parameters and LM results to this .npz')
     ap.add_argument('--no-fd', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--seed', type=int, default=3)
@@ -245,6 +250,14 @@ def main():
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), 'bench.py needs a GPU'
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f'bench: copied {args.calibrate_copy} MiB device to device')
+        return
 
     truth, start = draw_features(args.nfeat, args.side, args.seed)
     log(f'bench: generating {args.nfeat} synthetic {args.side}^2 holograms')
@@ -258,10 +271,17 @@ def main():
     iters = res[:, 37]
     ok = ((status & 7) != 0) & ((status & 48) == 0)
     err = np.abs(params[:, :7] - truth)
+    npix = args.side * args.side
+    redchi = res[:, 35] / (npix - 7)
+    capped = (status & 8) != 0
+    far = (err[:, 0] > 0.5) | (err[:, 2] > 5.) | (err[:, 3] > 0.05)
     log(f'bench: {fits_per_s:.1f} fits/s, {ms_per_step:.1f} ms/step, '
         f'iters mean {iters.mean():.1f} max {iters.max():.0f}, '
         f'converged {ok.mean():.3f}')
 
+    if args.dump:
+        np.savez_compressed(args.dump, truth=truth, start=start, params=params,
+                            result=res)
     log('bench: kernel timing')
     tj, tf, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
     ach_j = flop_j / tj / 1e12
@@ -322,6 +342,17 @@ def main():
         'fit_quality': {'converged_frac': float(ok.mean()),
                         'iterations_mean': float(iters.mean()),
                         'iterations_max': float(iters.max()),
+                        'iterations_p90': float(np.percentile(iters, 90)),
+                        'jacobians_mean': float(res[:, 39].mean()),
+                        'status_counts': {name: int(((status & bit) != 0).sum())
+                                          for name, bit in (('ftol', 1), ('xtol', 2),
+                                                            ('gtol', 4), ('maxiter', 8),
+                                                            ('stalled', 16),
+                                                            ('nonfinite', 32))},
+                        'capped_redchi_median': (float(np.median(redchi[capped]))
+                                                 if capped.any() else None),
+                        'capped_far_from_truth': int((capped & far).sum()),
+                        'converged_redchi_median': float(np.median(redchi[ok])),
                         'median_abs_err': {k: float(np.median(err[:, i]))
                                            for i, k in enumerate(PARAMS)}},
         'fd_plain_gpu': fd,

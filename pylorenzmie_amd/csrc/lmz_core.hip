@@ -72,8 +72,21 @@ __global__ __launch_bounds__(64) void k_coef_from_ab(
     nmax_out[(long long)f * nbuf] = N;
 }
 
-// Forward pass over one tile of LMZ_TILE pixels of one feature, in FP64.
+// Next entry of a work list for the whole block: count[1] is the list's
+// fetch counter, set to 0 by k_tile_list before the launch.
+__device__ __forceinline__ int next_tile(int32_t *count, int &slot) {
+    if (threadIdx.x == 0) slot = atomicAdd(count + 1, 1);
+    __syncthreads();
+    const int w = slot;
+    __syncthreads();
+    return w;
+}
+
+// Forward pass over tiles of LMZ_TILE pixels, in FP64.
 // MODE 0: field, 1: hologram, 2: residual (+ per-tile r^T r).
+// Without a work list each block takes tile blockIdx.x; with one (the LM's
+// compacted list of active tiles, length count[0]) the grid is persistent
+// and blocks fetch list entries dynamically until the list is exhausted.
 template <int MODE>
 __global__ __launch_bounds__(LMZ_THREADS) void k_forward(
     const lmz_feature *__restrict__ feat,
@@ -83,10 +96,15 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_forward(
     const float *__restrict__ coef, const float *__restrict__ coords,
     long long npix_total, int flags, const float *__restrict__ data,
     float *__restrict__ out, long long out_stride, cf *__restrict__ field,
-    double *__restrict__ chi2_tiles, const uint8_t *__restrict__ active) {
-    const int t = blockIdx.x;
+    double *__restrict__ chi2_tiles, const uint8_t *__restrict__ active,
+    const int32_t *__restrict__ list, int32_t *count) {
+  __shared__ int next_w;
+  const int nwork = count ? count[0] : (int)gridDim.x;
+  for (int w = list ? next_tile(count, next_w) : (int)blockIdx.x; w < nwork;
+       w = list ? next_tile(count, next_w) : nwork) {
+    const int t = list ? list[w] : w;
     const int f = tile_feature[t];
-    if (active && !active[f]) return;
+    if (active && !active[f]) continue;
     const lmz_feature &F = feat[f];
     const int b = sel ? (sel[f] ^ flip) : 0;
     const int N = nmax[(long long)f * nbuf + b];
@@ -132,7 +150,9 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_forward(
     if (MODE == 2 && chi2_tiles) {
         double v[1] = {chi};
         block_reduce_store_d<1>(v, chi2_tiles + t);
+        __syncthreads();   // the reduction buffer is reused by the next tile
     }
+  }
 }
 
 // Forward-mode Jacobian over one tile; optional J, residual, and per-tile
@@ -147,10 +167,15 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_jacobian(
     const float *__restrict__ data, const float *__restrict__ rin,
     long long rin_stride, float *__restrict__ jac,
     float *__restrict__ resid, double *__restrict__ partials,
-    const uint8_t *__restrict__ active) {
-    const int t = blockIdx.x;
+    const uint8_t *__restrict__ active, const int32_t *__restrict__ list,
+    int32_t *count) {
+  __shared__ int next_w;
+  const int nwork = count ? count[0] : (int)gridDim.x;
+  for (int w = list ? next_tile(count, next_w) : (int)blockIdx.x; w < nwork;
+       w = list ? next_tile(count, next_w) : nwork) {
+    const int t = list ? list[w] : w;
     const int f = tile_feature[t];
-    if (active && !active[f]) return;
+    if (active && !active[f]) continue;
     const lmz_feature &F = feat[f];
     const int b = sel ? (sel[f] ^ flip) : 0;
     const int N = nmax[(long long)f * nbuf + b];
@@ -199,7 +224,11 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_jacobian(
             }
         }
     }
-    if (partials && data) block_reduce_store<LMZ_NRED>(acc, partials + (long long)LMZ_NRED * t);
+    if (partials && data) {
+        block_reduce_store<LMZ_NRED>(acc, partials + (long long)LMZ_NRED * t);
+        __syncthreads();   // the reduction buffer is reused by the next tile
+    }
+  }
 }
 
 // Normal equations from J and r in HBM: one block per feature.
@@ -273,12 +302,13 @@ void launch_residual(const lmz_batch *b, const double *params,
                      const float *coef, const float *coords, int flags,
                      const float *data, float *out, long long out_stride,
                      double *chi2_tiles, const uint8_t *active,
-                     hipStream_t stream) {
-    hipLaunchKernelGGL(k_forward<2>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                     const TileList *tl, hipStream_t stream) {
+    const int grid = tl ? tl->grid : b->ntiles;
+    hipLaunchKernelGGL(k_forward<2>, dim3(grid), dim3(LMZ_THREADS), 0,
                        stream, b->d_feat, b->d_tile_feature, params, sel, flip,
                        b->coef_buffers, nmax, coef, coords, b->npix_total,
                        flags, data, out, out_stride, nullptr, chi2_tiles,
-                       active);
+                       active, tl ? tl->list : nullptr, tl ? tl->count : nullptr);
 }
 
 void launch_jacobian(const lmz_batch *b, const double *params,
@@ -287,12 +317,67 @@ void launch_jacobian(const lmz_batch *b, const double *params,
                      const float *coords, int flags, const float *data,
                      const float *rin, long long rin_stride, float *jac,
                      float *resid, double *partials, const uint8_t *active,
-                     hipStream_t stream) {
-    hipLaunchKernelGGL(k_jacobian, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                     const TileList *tl, hipStream_t stream) {
+    const int grid = tl ? tl->grid : b->ntiles;
+    hipLaunchKernelGGL(k_jacobian, dim3(grid), dim3(LMZ_THREADS), 0,
                        stream, b->d_feat, b->d_tile_feature, params, sel, flip,
                        b->coef_buffers, nmax, coef, dcoef, coords,
                        b->npix_total, flags, data, rin, rin_stride, jac, resid,
-                       partials, active);
+                       partials, active, tl ? tl->list : nullptr,
+                       tl ? tl->count : nullptr);
+}
+
+// Compacted list of the tiles of the features whose mask is set, and its
+// length: one block, features split into contiguous runs per thread.
+__global__ __launch_bounds__(1024) void k_tile_list(
+    const lmz_feature *__restrict__ feat, int nfeat,
+    const uint8_t *__restrict__ mask, int32_t *__restrict__ list,
+    int32_t *__restrict__ count) {
+    __shared__ int scan[1024];
+    const int tid = threadIdx.x;
+    const int per = (nfeat + 1023) / 1024;
+    const int f0 = min(nfeat, tid * per), f1 = min(nfeat, f0 + per);
+    int mine = 0;
+    for (int f = f0; f < f1; ++f)
+        if (mask[f]) mine += feat[f].ntiles;
+    scan[tid] = mine;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = tid >= off ? scan[tid - off] : 0;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    int pos = scan[tid] - mine;
+    for (int f = f0; f < f1; ++f) {
+        if (!mask[f]) continue;
+        const int t0 = feat[f].tile_offset, nt = feat[f].ntiles;
+        for (int i = 0; i < nt; ++i) list[pos++] = t0 + i;
+    }
+    if (tid == 1023) {
+        count[0] = scan[1023];
+        count[1] = 0;   // fetch counter of the persistent launch
+    }
+}
+
+void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
+                      const TileList &tl, hipStream_t stream) {
+    hipLaunchKernelGGL(k_tile_list, dim3(1), dim3(1024), 0, stream, b->d_feat,
+                       b->nfeat, mask, tl.list, tl.count);
+}
+
+// Persistent grid sizes: every block resident at once on this device.
+int persistent_grids(int *grid_jacobian, int *grid_forward) {
+    int dev = 0, cus = 0, nj = 0, nf = 0;
+    LMZ_HIP(hipGetDevice(&dev));
+    LMZ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    LMZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nj, reinterpret_cast<const void *>(&k_jacobian), LMZ_THREADS, 0));
+    LMZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nf, reinterpret_cast<const void *>(&k_forward<2>), LMZ_THREADS, 0));
+    *grid_jacobian = cus * (nj > 0 ? nj : 1);
+    *grid_forward = cus * (nf > 0 ? nf : 1);
+    return LMZ_OK;
 }
 
 }  // namespace lmz
@@ -446,7 +531,8 @@ int lmz_field(const lmz_batch *b, const double *params, const int32_t *sel,
                        stream, b->d_feat, b->d_tile_feature, params, sel, 0,
                        b->coef_buffers, nmax, coef, coords, b->npix_total,
                        flags, nullptr, nullptr, 0LL,
-                       reinterpret_cast<cf *>(field), nullptr, nullptr);
+                       reinterpret_cast<cf *>(field), nullptr, nullptr,
+                       nullptr, nullptr);
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
 }
@@ -462,13 +548,15 @@ int lmz_hologram(const lmz_batch *b, const double *params, const int32_t *sel,
         hipLaunchKernelGGL(k_forward<1>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
                            stream, b->d_feat, b->d_tile_feature, params, sel, 0,
                            b->coef_buffers, nmax, coef, coords, b->npix_total,
-                           flags, nullptr, out, 0LL, nullptr, nullptr, active);
+                           flags, nullptr, out, 0LL, nullptr, nullptr, active,
+                           nullptr, nullptr);
     } else {
         LMZ_REQUIRE(out || chi2_tiles, "one of out, chi2_tiles is required");
         hipLaunchKernelGGL(k_forward<2>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
                            stream, b->d_feat, b->d_tile_feature, params, sel, 0,
                            b->coef_buffers, nmax, coef, coords, b->npix_total,
-                           flags, data, out, 0LL, nullptr, chi2_tiles, active);
+                           flags, data, out, 0LL, nullptr, chi2_tiles, active,
+                           nullptr, nullptr);
     }
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
@@ -502,7 +590,7 @@ int lmz_jacobian(const lmz_batch *b, const double *params, const int32_t *sel,
                        stream, b->d_feat, b->d_tile_feature, params, sel, 0,
                        b->coef_buffers, nmax, coef, dcoef, coords,
                        b->npix_total, flags, data, nullptr, 0LL, jac, resid,
-                       partials, active);
+                       partials, active, nullptr, nullptr);
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
 }

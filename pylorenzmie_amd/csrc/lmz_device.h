@@ -28,6 +28,13 @@ namespace lmz {
 
 void set_error(const std::string &msg);
 
+// A device-built list of active tiles and the persistent grid that walks it.
+struct TileList {
+    int32_t *list;    // device, [ntiles]
+    int32_t *count;   // device, [2]: list length, fetch counter
+    int grid;         // blocks of the persistent launch
+};
+
 void launch_mie(const lmz_batch *b, const double *params, const int32_t *sel,
                 int flip, int32_t *nmax, float *coef, float *dcoef,
                 double *ab_out, double *work, const uint8_t *active,
@@ -38,7 +45,7 @@ void launch_residual(const lmz_batch *b, const double *params,
                      const float *coef, const float *coords, int flags,
                      const float *data, float *out, long long out_stride,
                      double *chi2_tiles, const uint8_t *active,
-                     hipStream_t stream);
+                     const TileList *tl, hipStream_t stream);
 // rin: residuals to use instead of the kernel's own, read from
 // rin + (sel[f] ^ flip) * rin_stride (NULL: computed from data)
 void launch_jacobian(const lmz_batch *b, const double *params,
@@ -47,7 +54,10 @@ void launch_jacobian(const lmz_batch *b, const double *params,
                      const float *coords, int flags, const float *data,
                      const float *rin, long long rin_stride, float *jac,
                      float *resid, double *partials, const uint8_t *active,
-                     hipStream_t stream);
+                     const TileList *tl, hipStream_t stream);
+void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
+                      const TileList &tl, hipStream_t stream);
+int persistent_grids(int *grid_jacobian, int *grid_forward);
 int hip_fail(hipError_t e, const char *what);
 int check_batch(const lmz_batch *b);
 

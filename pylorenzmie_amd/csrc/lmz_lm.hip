@@ -37,7 +37,7 @@ struct LMState {
 
 struct LMLayout {
     size_t coef, dcoef, nmax, sel, work, partials, chi2_tiles, state, ptrial,
-        mask_j, mask_f, count, resid, total;
+        mask_j, mask_f, count, resid, list_j, list_f, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -57,8 +57,10 @@ static LMLayout lm_layout(const lmz_batch *b) {
     L.ptrial = take(sizeof(double) * LMZ_NPARAM * (size_t)b->nfeat);
     L.mask_j = take((size_t)b->nfeat);
     L.mask_f = take((size_t)b->nfeat);
-    L.count = take(sizeof(int32_t) * 4);
+    L.count = take(sizeof(int32_t) * 8);
     L.resid = take(sizeof(float) * 2 * (size_t)b->npix_total);
+    L.list_j = take(sizeof(int32_t) * (size_t)b->ntiles);
+    L.list_f = take(sizeof(int32_t) * (size_t)b->ntiles);
     L.total = off;
     return L;
 }
@@ -305,16 +307,16 @@ __global__ __launch_bounds__(256) void k_lm_count(int nfeat,
     if (threadIdx.x == 0) count[0] = part[0];
 }
 
+// Results: J^T J and J^T r of the last Jacobian evaluation (at the solution,
+// or one converged step before it), chi2 at the solution, and the status.
 __global__ __launch_bounds__(64) void k_lm_finalize(
-    const lmz_feature *__restrict__ feat, const double *__restrict__ partials,
-    const LMState *__restrict__ st, double *__restrict__ result) {
-    const int f = blockIdx.x;
-    double red[LMZ_NRED];
-    feature_tile_sum<LMZ_NRED>(feat[f], partials, red);
-    if (threadIdx.x != 0) return;
+    int nfeat, const LMState *__restrict__ st, double *__restrict__ result) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfeat) return;
     const LMState &S = st[f];
     double *r = result + (long long)LMZ_NRES * f;
-    for (int i = 0; i < 35; ++i) r[i] = red[i];
+    for (int i = 0; i < 28; ++i) r[i] = S.jtj[i];
+    for (int i = 0; i < 7; ++i) r[28 + i] = S.g[i];
     r[35] = S.chi2;
     r[36] = S.lambda;
     r[37] = (double)S.iter;
@@ -328,6 +330,7 @@ using namespace lmz;
 
 struct lmz_lm_plan {
     const lmz_batch *batch = nullptr;
+    int grid_j = 0, grid_f = 0;       // persistent grids of the pixel kernels
     hipStream_t s = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     int32_t *h_count = nullptr;       // pinned
@@ -347,10 +350,11 @@ static void plan_release_graph(lmz_lm_plan *p) {
 }
 
 // One LM iteration: the six launches a graph replays.
-static void enqueue_iteration(const lmz_batch *b, const LMLayout &L, char *ws,
-                              const float *coords, int32_t flags,
+static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
+                              char *ws, const float *coords, int32_t flags,
                               const float *data, const lmz_lm_settings &set,
                               double *params, hipStream_t s) {
+    const lmz_batch *b = plan->batch;
     float *coef = reinterpret_cast<float *>(ws + L.coef);
     float *dcoef = reinterpret_cast<float *>(ws + L.dcoef);
     int32_t *nmax = reinterpret_cast<int32_t *>(ws + L.nmax);
@@ -365,14 +369,21 @@ static void enqueue_iteration(const lmz_batch *b, const LMLayout &L, char *ws,
     int32_t *count = reinterpret_cast<int32_t *>(ws + L.count);
     float *resid = reinterpret_cast<float *>(ws + L.resid);
     const long long np = b->npix_total;
-    // J^T J and J^T r at the current point, r from its FP64 forward pass
+    const TileList tl_j{reinterpret_cast<int32_t *>(ws + L.list_j), count + 4,
+                        plan->grid_j};
+    const TileList tl_f{reinterpret_cast<int32_t *>(ws + L.list_f), count + 6,
+                        plan->grid_f};
+    // J^T J and J^T r at the current point, r from its FP64 forward pass,
+    // over the tiles of the features that accepted their last step
+    launch_tile_list(b, mask_j, tl_j, s);
     launch_jacobian(b, params, sel, 0, nmax, coef, dcoef, coords, flags, data,
-                    resid, np, nullptr, nullptr, partials, mask_j, s);
+                    resid, np, nullptr, nullptr, partials, nullptr, &tl_j, s);
     hipLaunchKernelGGL(k_lm_solve, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
                        partials, st, params, ptrial, mask_j, mask_f, set);
     launch_mie(b, ptrial, sel, 1, nmax, coef, dcoef, nullptr, work, mask_f, s);
+    launch_tile_list(b, mask_f, tl_f, s);
     launch_residual(b, ptrial, sel, 1, nmax, coef, coords, flags, data,
-                    resid, np, chi2_tiles, mask_f, s);
+                    resid, np, chi2_tiles, nullptr, &tl_f, s);
     hipLaunchKernelGGL(k_lm_accept, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
                        chi2_tiles, st, params, ptrial, sel, mask_j, mask_f, set);
     hipLaunchKernelGGL(k_lm_count, dim3(1), dim3(256), 0, s, b->nfeat, mask_f,
@@ -396,6 +407,10 @@ int lmz_lm_plan_create(const lmz_batch *b, lmz_lm_plan **out) {
     *out = nullptr;
     lmz_lm_plan *p = new lmz_lm_plan;
     p->batch = b;
+    if (int rc = persistent_grids(&p->grid_j, &p->grid_f)) {
+        delete p;
+        return rc;
+    }
     hipError_t e = hipStreamCreateWithFlags(&p->s, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming);
@@ -472,7 +487,7 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
                        set.lambda0);
     launch_mie(b, params_io, sel, 0, nmax, coef, dcoef, nullptr, work, nullptr, s);
     launch_residual(b, params_io, sel, 0, nmax, coef, coords, flags, data,
-                    resid, np, chi2_tiles, nullptr, s);
+                    resid, np, chi2_tiles, nullptr, nullptr, s);
     hipLaunchKernelGGL(k_lm_init_chi2, dim3(b->nfeat), dim3(64), 0, s, b->nfeat,
                        b->d_feat, chi2_tiles, st);
     hipLaunchKernelGGL(k_lm_mask_init, dim3((b->nfeat + 63) / 64), dim3(64), 0, s,
@@ -487,7 +502,7 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
         if (!same) {
             plan_release_graph(plan);
             LMZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            enqueue_iteration(b, L, ws, coords, flags, data, set, params_io, s);
+            enqueue_iteration(plan, L, ws, coords, flags, data, set, params_io, s);
             hipError_t e = hipStreamEndCapture(s, &plan->graph);
             if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
             LMZ_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
@@ -500,7 +515,7 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
         if (set.use_graph) {
             LMZ_HIP(hipGraphLaunch(plan->exec, s));
         } else {
-            enqueue_iteration(b, L, ws, coords, flags, data, set, params_io, s);
+            enqueue_iteration(plan, L, ws, coords, flags, data, set, params_io, s);
             LMZ_HIP(hipGetLastError());
         }
         if ((it + 1) % set.check_every == 0 && it + 1 < set.max_iter) {
@@ -510,11 +525,8 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
             if (plan->h_count[0] == 0) break;
         }
     }
-    // Jacobian at the solution for the uncertainties
-    launch_jacobian(b, params_io, sel, 0, nmax, coef, dcoef, coords, flags, data,
-                    resid, np, nullptr, nullptr, partials, nullptr, s);
-    hipLaunchKernelGGL(k_lm_finalize, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
-                       partials, st, result);
+    hipLaunchKernelGGL(k_lm_finalize, dim3((b->nfeat + 63) / 64), dim3(64), 0, s,
+                       b->nfeat, st, result);
     LMZ_HIP(hipGetLastError());
     LMZ_HIP(hipEventRecord(plan->ev_out, s));
     LMZ_HIP(hipStreamWaitEvent(stream, plan->ev_out, 0));

@@ -429,7 +429,9 @@ LMZ_HD double intensity(const cd (&E)[3], double alpha) {
 // The geometric factors are then differentiated through
 // (R, c, u, v, s)(X, Y, Z); the only 1/rho term, K = (A + B)/rho, vanishes
 // on the axis (A + B = O(rho^2)), where dI/dX = dI/dY = 0 by the mirror
-// symmetries of the hologram.
+// symmetries of the hologram.  a_n and b_n are holomorphic in
+// m = (n_p + i k_p) / n_m, so d/dk_p = i d/dn_p: the k_p column comes from
+// the n_p sums and needs no sums of its own.
 struct JacPixel {
     cf E[3];       // field without alpha
     float I;       // intensity
@@ -444,15 +446,16 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
     cf Sr{0.f, 0.f}, St{0.f, 0.f}, Sp{0.f, 0.f};
     cf SrR{0.f, 0.f}, StR{0.f, 0.f}, SpR{0.f, 0.f};
     cf Src{0.f, 0.f}, Stc{0.f, 0.f}, Spc{0.f, 0.f};
-    cf Tr[3], Tt[3], Tp[3];
+    cf Tr[2], Tt[2], Tp[2];    // coefficient tangents: a_p, n_p
 #pragma unroll
-    for (int q = 0; q < 3; ++q) Tr[q] = Tt[q] = Tp[q] = cf{0.f, 0.f};
+    for (int q = 0; q < 2; ++q) Tr[q] = Tt[q] = Tp[q] = cf{0.f, 0.f};
     Angular ang;
     ang.init();
     const double w = (double)g.w;
     float dpm = 0.f, dpn = 0.f;   // d pi_{n-1} / dc, d pi_n / dc
     cf x2 = g.xi_m1, x1 = g.xi_0;
     const float cl = g.cl, invRl = g.invRl, invR2 = g.invR * g.invR;
+#pragma unroll 2
     for (int n = 1; n <= nmax; ++n) {
         const float *cp = coef + LMZ_COEF_STRIDE * (n - 1);
         const float *dp = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
@@ -493,7 +496,7 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
         cmac(Spc, -dtau, QX);
         // coefficient tangents
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < 2; ++q) {
             const float *d = dp + 8 * q;
             const cf a1 = cmul(cf{d[0], d[1]}, xi);
             const cf a2 = cmul(cf{d[2], d[3]}, D);
@@ -517,7 +520,7 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
         SrR = m * SrR; StR = m * StR;
         Src = m * Src; Stc = m * Stc;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) Tp[q] = m * Tp[q];
+        for (int q = 0; q < 2; ++q) Tp[q] = m * Tp[q];
     }
     // ---- field and its (R, c) partials
     const float s = g.s, s2 = g.s * g.s, u = g.u, v = g.v;
@@ -579,12 +582,16 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
     out.J[1] = -k * dI[1];
     out.J[2] = k * fmaf(two_al, out.E[0].im, dI[2]);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 2; ++q) {
         const cf Aq = a_r * Tr[q] + a_t * Tt[q];
         const cf Bq = invR * Tp[q];
         const cf Cq = c_r * Tr[q] - invR * Tt[q];
         const cf Ex = u2 * Aq - v2 * Bq, Ey = uv * (Aq + Bq), Ez = us * Cq;
         out.J[3 + q] = two_al * (credot(F0, Ex) + credot(F1, Ey) + credot(F2, Ez));
+        if (q == 1) {  // k_p: dE/dk_p = i dE/dn_p
+            const cf iEx{-Ex.im, Ex.re}, iEy{-Ey.im, Ey.re}, iEz{-Ez.im, Ez.re};
+            out.J[5] = two_al * (credot(F0, iEx) + credot(F1, iEy) + credot(F2, iEz));
+        }
     }
     out.J[6] = 2.f * (credot(F0, out.E[0]) + credot(F1, out.E[1]) +
                       credot(F2, out.E[2]));

@@ -121,7 +121,7 @@ def run_batched(args, truth, start, data_dev):
         params, res = step()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return lm, dt, params, res, settings
+    return lm, dt, params, res, settings, p0
 
 
 def kernel_times(lm, settings, args):
@@ -173,6 +173,46 @@ def kernel_times(lm, settings, args):
     flop_j = float(np.sum(P * (296. * nm + 480.)))
     flop_f = float(np.sum(P * (64. * nm + 70.)))
     return tj, tf, flop_j, flop_f, float(np.sum(P)), float(np.mean(nm))
+
+
+def fit_agreement(lm, params, res, p0):
+    """Distance of the timed fits from the minimum each one is heading for:
+    the same LM rerun from the same starts to 1e-13 tolerances and 200
+    iterations (not timed).  Features capped in either run are left out.
+    Relative differences for all parameters but k_p, whose value can be 0:
+    its difference is absolute."""
+    from pylorenzmie_amd.analysis.lm import LMSettings
+    tight = LMSettings(max_iter=200, ftol=1e-13, xtol=1e-13, gtol=1e-15)
+    lm.params.copy_(p0)
+    lm.run(tight, PARAMS)
+    pref, rref = lm.fetch()
+    keep = (((rref[:, 38].astype(int) | res[:, 38].astype(int)) & 8) == 0)
+    rel = np.abs(params[:, :7] - pref[:, :7]) / np.abs(pref[:, :7])
+    rel[:, 5] = np.abs(params[:, 5] - pref[:, 5])
+    rel = rel[keep]
+    ok = (np.delete(rel, 5, axis=1).max(1) <= 1e-4) & (rel[:, 5] <= 1e-5)
+    return {'reference': 'same LM from the same starts to 1e-13 tolerances, '
+                         '<=200 iterations',
+            'features_compared': int(keep.sum()),
+            'frac_within_1e-4_rel_and_kp_1e-5_abs': float(ok.mean()),
+            'p99': {k: float(np.percentile(rel[:, i], 99))
+                    for i, k in enumerate(PARAMS)},
+            'median': {k: float(np.median(rel[:, i]))
+                       for i, k in enumerate(PARAMS)}}
+
+
+def committed_traffic(nfeat, side):
+    """HBM bytes per full-batch Jacobian launch, from the PMC summary
+    committed under profiles/ for this workload (FETCH_SIZE and WRITE_SIZE,
+    calibrated against a copy kernel), or None for other workloads."""
+    import re
+    path = os.path.join(ROOT, 'profiles',
+                        f'k_jacobian_traffic_{nfeat}x{side}x{side}.md')
+    if not os.path.exists(path):
+        return None
+    m = re.search(r'traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)',
+                  open(path).read())
+    return float(m.group(1)) if m else None
 
 
 def run_fd(args, truth, start, data_host):
@@ -264,7 +304,7 @@ def main():
     data_dev = make_data(truth, args.side, args.sigma, args.seed)
     torch.cuda.synchronize()
     log('bench: batched LM warmup + timed steps')
-    lm, dt, params, res, settings = run_batched(args, truth, start, data_dev)
+    lm, dt, params, res, settings, p0 = run_batched(args, truth, start, data_dev)
     ms_per_step = dt / args.steps * 1e3
     fits_per_s = args.nfeat * args.steps / dt
     status = res[:, 38].astype(int)
@@ -282,6 +322,10 @@ def main():
     if args.dump:
         np.savez_compressed(args.dump, truth=truth, start=start, params=params,
                             result=res)
+    log('bench: agreement with fully converged fits')
+    agree = fit_agreement(lm, params, res, p0)
+    log(f"bench: {agree['frac_within_1e-4_rel_and_kp_1e-5_abs']:.4f} of "
+        f"{agree['features_compared']} fits within 1e-4 of the converged minimum")
     log('bench: kernel timing')
     tj, tf, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
     ach_j = flop_j / tj / 1e12
@@ -289,6 +333,9 @@ def main():
     log(f'bench: jacobian {tj * 1e3:.2f} ms {ach_j:.1f} TFLOP/s; '
         f'forward {tf * 1e3:.2f} ms {ach_f:.1f} TFLOP/s')
 
+    # J reads the data (4 B / pixel) and writes 36 FP64 partials per tile
+    bytes_j = 4. * npx + 36 * 8. * lm.batch.ntiles
+    traffic = committed_traffic(args.nfeat, args.side)
     fd = None
     if not args.no_fd:
         log('bench: plain GPU FD form sample')
@@ -336,7 +383,8 @@ def main():
                      'avg_launch_ms': tj * 1e3,
                      'algorithmic_flop_per_launch': flop_j,
                      'pixels_per_launch': npx,
-                     'traffic': None,
+                     'traffic': traffic,
+                     'algorithmic_bytes_per_launch': bytes_j,
                      'forward_kernel': {'achieved': ach_f, 'avg_launch_ms': tf * 1e3,
                                         'algorithmic_flop_per_launch': flop_f}},
         'fit_quality': {'converged_frac': float(ok.mean()),
@@ -355,6 +403,7 @@ def main():
                         'converged_redchi_median': float(np.median(redchi[ok])),
                         'median_abs_err': {k: float(np.median(err[:, i]))
                                            for i, k in enumerate(PARAMS)}},
+        'fit_agreement': agree,
         'fd_plain_gpu': fd,
         'speedup_vs_fd': (fits_per_s / fd['value']) if fd else None,
         'cpu_baseline': cpu,

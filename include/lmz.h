@@ -171,6 +171,24 @@ int lmz_jacobian(const lmz_batch *batch, const double *params,
                  double *partials, const uint8_t *active,
                  hipStream_t stream);
 
+/* Vector-Jacobian product of the hologram: grad_params[f][c] = sum over
+ * feature f's pixels of grad_out[px] dI/d(param c), for the seven
+ * parameters x_p y_p z_p a_p n_p k_p alpha, accumulated in FP64.  J is
+ * formed per pixel in registers (the recurrences and their tangents are
+ * recomputed, nothing is stored) and contracted at once.  The backward of
+ * lmz_hologram for reverse-mode callers (a loss over many holograms);
+ * coefficient buffer 0 is used.
+ * Replaces the hologram derivatives that theory/jaxLorenzMie.py:280-286
+ * and :354-395 take by jax.jacfwd, contracted with an upstream gradient.
+ *   grad_out     float  [npix_total]      dL/dI
+ *   tile_work    double [ntiles][LMZ_NJAC] scratch
+ *   grad_params  double [nfeat][LMZ_NJAC] */
+int lmz_hologram_vjp(const lmz_batch *batch, const double *params,
+                     const int32_t *nmax, const float *coef,
+                     const float *dcoef, const float *coords, int32_t flags,
+                     const float *grad_out, double *tile_work,
+                     double *grad_params, hipStream_t stream);
+
 /* Normal equations from a Jacobian and residual already in HBM:
  * out double [nfeat][36] (J^T J upper triangle, J^T r, r^T r), with J
  * scaled by 1/noise.  The reduction MINPACK performs inside lmder

@@ -3,6 +3,8 @@
 Nothing here shares a code path with oracle/lorenzmie.py:
   * mie_ab_mpmath: Bohren & Huffman Eq. 4.53 with Riccati-Bessel functions
     from mpmath spherical Bessel functions at 30 digits;
+  * coated_ab_mpmath: the coated-sphere closed form of B&H Sec. 8.1 in
+    mpmath (two layers);
   * field_bh: the scattered field of B&H Eqs. 4.45-4.50 summed literally,
     with spherical Hankel functions from scipy.special.spherical_jn/yn and
     pi_n, tau_n from associated Legendre functions (scipy.special.lpmv).
@@ -42,6 +44,55 @@ def mie_ab_mpmath(a_p, n_p, k_p, n_m, wavelength, nmax, dps=30):
              (m * psi(n, mx) * dxi(n, x) - xi(n, x) * dpsi(n, mx)))
         b = ((psi(n, mx) * dpsi(n, x) - m * psi(n, x) * dpsi(n, mx)) /
              (psi(n, mx) * dxi(n, x) - m * xi(n, x) * dpsi(n, mx)))
+        out[n] = complex(a), complex(b)
+    return out
+
+
+def coated_ab_mpmath(a_core, a_shell, n_core, k_core, n_shell, k_shell, n_m,
+                     wavelength, nmax, dps=40):
+    """a_n, b_n (n = 1..nmax) of a coated sphere, Bohren & Huffman Sec. 8.1
+    (the closed form of Aden & Kerker), in mpmath.
+
+    x = k a_core, y = k a_shell, m1, m2 the core and shell indices relative
+    to the medium; psi_n(z) = z j_n(z), chi_n(z) = -z y_n(z),
+    xi_n = psi_n - i chi_n.
+    """
+    mp.mp.dps = dps
+    k = mp.mpf(2) * mp.pi / mp.mpf(wavelength) * mp.mpf(n_m)
+    x, y = k * mp.mpf(a_core), k * mp.mpf(a_shell)
+    m1 = mp.mpc(n_core, k_core) / mp.mpf(n_m)
+    m2 = mp.mpc(n_shell, k_shell) / mp.mpf(n_m)
+    half = mp.mpf(1) / 2
+
+    def psi(n, z):
+        return mp.sqrt(mp.pi * z / 2) * mp.besselj(n + half, z)
+
+    def chi(n, z):
+        return -mp.sqrt(mp.pi * z / 2) * mp.bessely(n + half, z)
+
+    def d(f, n, z):              # f_n'(z) = f_{n-1}(z) - n f_n(z) / z
+        return f(n - 1, z) - n * f(n, z) / z
+
+    def xi(n, z):
+        return psi(n, z) - 1j * chi(n, z)
+
+    out = np.zeros((nmax + 1, 2), complex)
+    for n in range(1, nmax + 1):
+        p1, dp1 = psi(n, m1 * x), d(psi, n, m1 * x)
+        p2, dp2 = psi(n, m2 * x), d(psi, n, m2 * x)
+        c2, dc2 = chi(n, m2 * x), d(chi, n, m2 * x)
+        A = (m2 * p2 * dp1 - m1 * dp2 * p1) / (m2 * c2 * dp1 - m1 * dc2 * p1)
+        B = (m2 * p1 * dp2 - m1 * p2 * dp1) / (m2 * dc2 * p1 - m1 * dp1 * c2)
+        py, dpy = psi(n, y), d(psi, n, y)
+        xy, dxy = xi(n, y), d(xi, n, y)
+        p2y, dp2y = psi(n, m2 * y), d(psi, n, m2 * y)
+        c2y, dc2y = chi(n, m2 * y), d(chi, n, m2 * y)
+        ua = dp2y - A * dc2y
+        va = p2y - A * c2y
+        a = (py * ua - m2 * dpy * va) / (xy * ua - m2 * dxy * va)
+        ub = dp2y - B * dc2y
+        vb = p2y - B * c2y
+        b = (m2 * py * ub - dpy * vb) / (m2 * xy * ub - dxy * vb)
         out[n] = complex(a), complex(b)
     return out
 

@@ -88,6 +88,7 @@ SIGNATURES = {
     'lmz_intensity': (_I, [_P, _P, _P, _I, _P, _P]),
     'lmz_jacobian': (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P,
                           _P, _P]),
+    'lmz_hologram_vjp': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
     'lmz_normal_eq': (_I, [_P, _P, _P, _P, _P]),
     'lmz_lm_workspace_bytes': (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     'lmz_lm_plan_create': (_I, [_P, ctypes.POINTER(_P)]),

@@ -72,10 +72,30 @@ __global__ __launch_bounds__(64) void k_coef_from_ab(
     nmax_out[(long long)f * nbuf] = N;
 }
 
-// Next entry of a work list for the whole block: count[1] is the list's
-// fetch counter, set to 0 by k_tile_list before the launch.
-__device__ __forceinline__ int next_tile(int32_t *count, int &slot) {
-    if (threadIdx.x == 0) slot = atomicAdd(count + 1, 1);
+// Next entry of a work list for the whole block.  The list (length
+// count[0]) is split into LMZ_XCDS contiguous chunks, chunk x fetched
+// through counter count[2 + x] (set to 0 by k_tile_list).  A block starts on
+// the chunk of its own XCD, so the tiles of one feature stay in one L2, and
+// moves on to the other chunks when its own is exhausted.  `k` (thread 0's
+// count of exhausted chunks, initially 0) only grows, so every block
+// reaches the end value count[0] after at most count[0] + LMZ_XCDS fetches.
+__device__ __forceinline__ int next_tile(int32_t *count, int &slot, int &k) {
+    if (threadIdx.x == 0) {
+        const int n = count[0];
+        const int q = n / LMZ_XCDS, r = n % LMZ_XCDS;
+        const int home = (int)(blockIdx.x % LMZ_XCDS);
+        int w = n;
+        while (k < LMZ_XCDS) {
+            const int x = (home + k) % LMZ_XCDS;
+            const int i = atomicAdd(count + 2 + x, 1);
+            if (i < q + (x < r ? 1 : 0)) {
+                w = x * q + (x < r ? x : r) + i;
+                break;
+            }
+            ++k;
+        }
+        slot = w;
+    }
     __syncthreads();
     const int w = slot;
     __syncthreads();
@@ -84,9 +104,10 @@ __device__ __forceinline__ int next_tile(int32_t *count, int &slot) {
 
 // Forward pass over tiles of LMZ_TILE pixels, in FP64.
 // MODE 0: field, 1: hologram, 2: residual (+ per-tile r^T r).
-// Without a work list each block takes tile blockIdx.x; with one (the LM's
-// compacted list of active tiles, length count[0]) the grid is persistent
-// and blocks fetch list entries dynamically until the list is exhausted.
+// Without a work list each block takes tile xcd_tile(blockIdx.x); with one
+// (the LM's compacted list of active tiles, length count[0]) the grid is
+// persistent and blocks fetch list entries dynamically (next_tile) until
+// the list is exhausted.
 template <int MODE>
 __global__ __launch_bounds__(LMZ_THREADS) void k_forward(
     const lmz_feature *__restrict__ feat,
@@ -99,9 +120,12 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_forward(
     double *__restrict__ chi2_tiles, const uint8_t *__restrict__ active,
     const int32_t *__restrict__ list, int32_t *count) {
   __shared__ int next_w;
+  int chunks_done = 0;
   const int nwork = count ? count[0] : (int)gridDim.x;
-  for (int w = list ? next_tile(count, next_w) : (int)blockIdx.x; w < nwork;
-       w = list ? next_tile(count, next_w) : nwork) {
+  for (int w = list ? next_tile(count, next_w, chunks_done)
+                    : xcd_tile((int)blockIdx.x, (int)gridDim.x);
+       w < nwork;
+       w = list ? next_tile(count, next_w, chunks_done) : nwork) {
     const int t = list ? list[w] : w;
     const int f = tile_feature[t];
     if (active && !active[f]) continue;
@@ -170,9 +194,12 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_jacobian(
     const uint8_t *__restrict__ active, const int32_t *__restrict__ list,
     int32_t *count) {
   __shared__ int next_w;
+  int chunks_done = 0;
   const int nwork = count ? count[0] : (int)gridDim.x;
-  for (int w = list ? next_tile(count, next_w) : (int)blockIdx.x; w < nwork;
-       w = list ? next_tile(count, next_w) : nwork) {
+  for (int w = list ? next_tile(count, next_w, chunks_done)
+                    : xcd_tile((int)blockIdx.x, (int)gridDim.x);
+       w < nwork;
+       w = list ? next_tile(count, next_w, chunks_done) : nwork) {
     const int t = list ? list[w] : w;
     const int f = tile_feature[t];
     if (active && !active[f]) continue;
@@ -229,6 +256,61 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_jacobian(
         __syncthreads();   // the reduction buffer is reused by the next tile
     }
   }
+}
+
+// Vector-Jacobian product over one tile: sum of g dI/d(param) for the
+// seven parameters, J formed per pixel in registers and never written.
+__global__ __launch_bounds__(LMZ_THREADS) void k_vjp(
+    const lmz_feature *__restrict__ feat,
+    const int32_t *__restrict__ tile_feature,
+    const double *__restrict__ params, int nbuf,
+    const int32_t *__restrict__ nmax, const float *__restrict__ coef,
+    const float *__restrict__ dcoef, const float *__restrict__ coords,
+    long long npix_total, int flags, const float *__restrict__ g,
+    double *__restrict__ tile_grad) {
+    const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int f = tile_feature[t];
+    const lmz_feature &F = feat[f];
+    const FeatUniform fu = feature_uniform(
+        F, params + (long long)LMZ_NPARAM * f, nmax[(long long)f * nbuf],
+        F.coef_offset);
+    const bool bohren = flags & LMZ_BOHREN;
+    const float *cf0 = coef + (long long)LMZ_COEF_STRIDE * fu.slot0;
+    const float *df0 = dcoef + (long long)LMZ_DCOEF_STRIDE * fu.slot0;
+    float acc[LMZ_NJAC];
+#pragma unroll
+    for (int c = 0; c < LMZ_NJAC; ++c) acc[c] = 0.f;
+#pragma unroll 1
+    for (int q = 0; q < LMZ_PPT; ++q) {
+        const int j = (t - fu.tile0) * LMZ_TILE + q * LMZ_THREADS + threadIdx.x;
+        if (j >= fu.npix) break;
+        float ax, ay, az;
+        pixel_coords(fu, j, coords, npix_total, ax, ay, az);
+        PixGeom pg;
+        pixel_geometry(fu, ax, ay, az, bohren, pg);
+        JacPixel jp;
+        jacobian_pixel(fu, pg, cf0, df0, fu.nmax, jp);
+        const float gv = g[fu.pix_offset + j];
+#pragma unroll
+        for (int c = 0; c < LMZ_NJAC; ++c) acc[c] = fmaf(jp.J[c], gv, acc[c]);
+    }
+    block_reduce_store<LMZ_NJAC>(acc, tile_grad + (long long)LMZ_NJAC * t);
+}
+
+// Per-feature sums of per-tile rows of NV doubles, tiles in order (the
+// same result on every run): one thread per (feature, entry).
+template <int NV>
+__global__ __launch_bounds__(LMZ_THREADS) void k_tile_sum(
+    const lmz_feature *__restrict__ feat, int nfeat,
+    const double *__restrict__ tiles, double *__restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nfeat * NV) return;
+    const int f = (int)(i / NV), c = (int)(i % NV);
+    const lmz_feature &F = feat[f];
+    double s = 0.0;
+    for (int k = 0; k < F.ntiles; ++k)
+        s += tiles[(long long)NV * (F.tile_offset + k) + c];
+    out[i] = s;
 }
 
 // Normal equations from J and r in HBM: one block per feature.
@@ -354,10 +436,8 @@ __global__ __launch_bounds__(1024) void k_tile_list(
         const int t0 = feat[f].tile_offset, nt = feat[f].ntiles;
         for (int i = 0; i < nt; ++i) list[pos++] = t0 + i;
     }
-    if (tid == 1023) {
-        count[0] = scan[1023];
-        count[1] = 0;   // fetch counter of the persistent launch
-    }
+    if (tid == 1023) count[0] = scan[1023];
+    if (tid < LMZ_XCDS) count[2 + tid] = 0;   // fetch counters (next_tile)
 }
 
 void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
@@ -591,6 +671,31 @@ int lmz_jacobian(const lmz_batch *b, const double *params, const int32_t *sel,
                        b->coef_buffers, nmax, coef, dcoef, coords,
                        b->npix_total, flags, data, nullptr, 0LL, jac, resid,
                        partials, active, nullptr, nullptr);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+int lmz_hologram_vjp(const lmz_batch *b, const double *params,
+                     const int32_t *nmax, const float *coef,
+                     const float *dcoef, const float *coords, int32_t flags,
+                     const float *grad_out, double *tile_work,
+                     double *grad_params, hipStream_t stream) {
+    if (int rc = forward_args(b, params, nmax, coef, coords, nullptr)) return rc;
+    LMZ_REQUIRE(dcoef != nullptr, "dcoef (coefficient tangents) is required");
+    LMZ_REQUIRE(flags & LMZ_CARTESIAN,
+                "the Jacobian is defined for the Cartesian hologram");
+    LMZ_REQUIRE(grad_out && tile_work && grad_params,
+                "grad_out, tile_work and grad_params are required");
+    hipLaunchKernelGGL(k_vjp, dim3(b->ntiles), dim3(LMZ_THREADS), 0, stream,
+                       b->d_feat, b->d_tile_feature, params, b->coef_buffers,
+                       nmax, coef, dcoef, coords, b->npix_total, flags,
+                       grad_out, tile_work);
+    LMZ_HIP(hipGetLastError());
+    const long long n = (long long)b->nfeat * LMZ_NJAC;
+    hipLaunchKernelGGL(k_tile_sum<LMZ_NJAC>,
+                       dim3((unsigned)((n + LMZ_THREADS - 1) / LMZ_THREADS)),
+                       dim3(LMZ_THREADS), 0, stream, b->d_feat, b->nfeat,
+                       tile_work, grad_params);
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
 }

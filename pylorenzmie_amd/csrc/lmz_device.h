@@ -31,7 +31,8 @@ void set_error(const std::string &msg);
 // A device-built list of active tiles and the persistent grid that walks it.
 struct TileList {
     int32_t *list;    // device, [ntiles]
-    int32_t *count;   // device, [2]: list length, fetch counter
+    int32_t *count;   // device, [2 + LMZ_XCDS]: list length, unused,
+                      // per-XCD fetch counters
     int grid;         // blocks of the persistent launch
 };
 
@@ -71,6 +72,18 @@ int check_batch(const lmz_batch *b);
     do {                                                                 \
         if (!(cond)) { ::lmz::set_error(msg); return LMZ_EINVAL; }       \
     } while (0)
+
+// Workgroups of a launch are dispatched round-robin over the 8 XCDs of an
+// MI355X (workgroup b on XCD b % 8), each with its own L2.  Tile of
+// workgroup b when XCD x takes one contiguous range of the G tiles: the
+// tiles of a feature, which share its coefficient tables, then meet in one
+// L2 instead of eight.  A bijection of [0, G).
+#define LMZ_XCDS 8
+__device__ __forceinline__ int xcd_tile(int b, int G) {
+    const int q = G / LMZ_XCDS, r = G % LMZ_XCDS;
+    const int x = b % LMZ_XCDS, i = b / LMZ_XCDS;
+    return x * q + (x < r ? x : r) + i;
+}
 
 // ---------------------------------------------------------------- reductions
 __device__ __forceinline__ float wave_sum(float v) {

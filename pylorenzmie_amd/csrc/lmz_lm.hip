@@ -57,7 +57,7 @@ static LMLayout lm_layout(const lmz_batch *b) {
     L.ptrial = take(sizeof(double) * LMZ_NPARAM * (size_t)b->nfeat);
     L.mask_j = take((size_t)b->nfeat);
     L.mask_f = take((size_t)b->nfeat);
-    L.count = take(sizeof(int32_t) * 8);
+    L.count = take(sizeof(int32_t) * 48);   // active count, two TileLists
     L.resid = take(sizeof(float) * 2 * (size_t)b->npix_total);
     L.list_j = take(sizeof(int32_t) * (size_t)b->ntiles);
     L.list_f = take(sizeof(int32_t) * (size_t)b->ntiles);
@@ -369,9 +369,9 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
     int32_t *count = reinterpret_cast<int32_t *>(ws + L.count);
     float *resid = reinterpret_cast<float *>(ws + L.resid);
     const long long np = b->npix_total;
-    const TileList tl_j{reinterpret_cast<int32_t *>(ws + L.list_j), count + 4,
+    const TileList tl_j{reinterpret_cast<int32_t *>(ws + L.list_j), count + 16,
                         plan->grid_j};
-    const TileList tl_f{reinterpret_cast<int32_t *>(ws + L.list_f), count + 6,
+    const TileList tl_f{reinterpret_cast<int32_t *>(ws + L.list_f), count + 32,
                         plan->grid_f};
     // J^T J and J^T r at the current point, r from its FP64 forward pass,
     // over the tiles of the features that accepted their last step

@@ -120,6 +120,29 @@ def test_layered_reduces_to_homogeneous():
         assert np.abs(abl - ab).max() < 1e-12
 
 
+@pytest.mark.parametrize('core,shell', [
+    ((0.4, 1.60, 0.0), (0.9, 1.40, 0.0)),       # dense core, thin shell
+    ((0.7, 1.38, 0.002), (1.1, 1.95, 0.01)),    # high-index absorbing shell
+    ((0.2, 2.00, 0.0), (0.35, 1.33, 0.0))])     # small, index-matched shell
+def test_layered_matches_coated_sphere_closed_form(core, shell):
+    """Yang's recursion for two layers against B&H Sec. 8.1 in mpmath; D1
+    started 40 orders deep so only the recursion itself is compared."""
+    radii = [core[0], shell[0]]
+    nl, kl = [core[1], shell[1]], [core[2], shell[2]]
+    ab = ref.mie_coefficients_layered(radii, nl, kl, 1.34, 0.447)
+    nmax = ab.shape[0] - 1
+    deep = ref.mie_coefficients_layered(radii, nl, kl, 1.34, 0.447,
+                                        nmax=nmax + 40)[:nmax + 1]
+    bh = independent.coated_ab_mpmath(core[0], shell[0], core[1], core[2],
+                                      shell[1], shell[2], 1.34, 0.447, nmax)
+    scale = np.abs(bh).max()
+    assert np.abs(deep - bh).max() < 1e-10 * scale
+    # a swapped layer order or a dropped layer fails by far
+    swap = ref.mie_coefficients_layered(radii, nl[::-1], kl[::-1], 1.34, 0.447,
+                                        nmax=nmax + 40)[:nmax + 1]
+    assert np.abs(swap - bh).max() > 1e-3 * scale
+
+
 # ---- independent B&H multipole sum -----------------------------------------
 @pytest.mark.parametrize('p', [(12.3, 11.6, 90., 0.8, 1.5, 0.),
                                (10.0, 10.0, 60., 1.3, 1.42, 0.002)])

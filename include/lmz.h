@@ -122,6 +122,23 @@ int lmz_mie_ab(const lmz_batch *batch, const double *params,
                float *dcoef, double *ab_out, double *work,
                hipStream_t stream);
 
+/* Mie coefficients of layered spheres in FP64 (Yang's recursion) and the
+ * FP32 per-order tables of buffer 0, no tangents.
+ * Replaces the multilayer recursion of lib/obsolete/sphere_coefficients.py:
+ * 126-215 (Yang, Appl. Opt. 42, 1710 (2003)) with n_max from Yang's Eq. 30
+ * (theory/Sphere.py:91-123).
+ *   layers   double [nfeat][LMZ_MAX_LAYERS][3]: outer radius [um], n, k of
+ *            each layer, innermost first, radii increasing
+ *   nlayers  int32 [nfeat], 1..LMZ_MAX_LAYERS
+ *   nmax_out int32 [nfeat * coef_buffers], entry f*coef_buffers; -1 marks a
+ *            feature whose layer table is invalid
+ *   coef, ab_out  as lmz_mie_ab (either may be NULL, not both)
+ *   work     double [work_doubles] scratch */
+#define LMZ_MAX_LAYERS 8
+int lmz_mie_ab_layered(const lmz_batch *batch, const double *layers,
+                       const int32_t *nlayers, int32_t *nmax_out, float *coef,
+                       double *ab_out, double *work, hipStream_t stream);
+
 /* Per-order tables from coefficients computed elsewhere (any particle
  * whose ab() is not a homogeneous sphere): ab_in double [slots][4] rows
  * n = 1..nmax_in[f] of each feature, as ab_out of lmz_mie_ab.
@@ -242,6 +259,21 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
                const float *data, const lmz_lm_settings *settings,
                double *params_io, double *result, void *workspace,
                hipStream_t stream);
+
+/* As lmz_lm_run for particles whose Mie coefficients do not move during
+ * the fit: layered spheres (lmz_mie_ab_layered), or any particle with a_p,
+ * n_p, k_p held.  ab: double [coef_slots][4], rows n = 1..nmax of each
+ * feature at its buffer-0 slots, as lmz_mie_ab / lmz_mie_ab_layered write
+ * them for this batch; nmax: int32 [nfeat * coef_buffers], entry
+ * f*coef_buffers.  settings->vary must hold a_p, n_p and k_p fixed.
+ * Replaces Optimizer.optimize (analysis/Optimizer.py:196-222) for such
+ * particles, whose ab() the reference re-evaluates on every residual
+ * (theory/LorenzMie.py:156). */
+int lmz_lm_run_fixed_ab(lmz_lm_plan *plan, const float *coords, int32_t flags,
+                        const float *data, const lmz_lm_settings *settings,
+                        const double *ab, const int32_t *nmax,
+                        double *params_io, double *result, void *workspace,
+                        hipStream_t stream);
 
 const char *lmz_last_error(void);
 int lmz_version(void);

@@ -16,6 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(_HERE, 'liblmz.so')
 
 NPARAM = 8
+MAX_LAYERS = 8
 NJAC = 7
 NRED = 36
 NRES = 40
@@ -82,6 +83,7 @@ SIGNATURES = {
     'lmz_batch_destroy': (_I, [_P]),
     'lmz_batch_query': (_I, [_P, ctypes.POINTER(BatchSizes)]),
     'lmz_mie_ab': (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    'lmz_mie_ab_layered': (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
     'lmz_coef_from_ab': (_I, [_P, _P, _P, _P, _P, _P]),
     'lmz_field': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P]),
     'lmz_hologram': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
@@ -95,6 +97,8 @@ SIGNATURES = {
     'lmz_lm_plan_destroy': (_I, [_P]),
     'lmz_lm_run': (_I, [_P, _P, _I, _P, ctypes.POINTER(LMSettings), _P, _P,
                         _P, _P]),
+    'lmz_lm_run_fixed_ab': (_I, [_P, _P, _I, _P, ctypes.POINTER(LMSettings),
+                                 _P, _P, _P, _P, _P, _P]),
     'lmz_last_error': (ctypes.c_char_p, []),
     'lmz_version': (_I, []),
 }

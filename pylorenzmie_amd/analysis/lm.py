@@ -146,10 +146,39 @@ class LMBatch:
         p[:, :p0.shape[1]] = p0
         self.params.copy_(t.from_numpy(p))
 
+    def load_layers(self, layers_list) -> None:
+        """Fixed Mie coefficients for every feature from layer tables
+        ((L, 3) rows of outer radius [um], n, k per feature; one row for a
+        homogeneous sphere), computed once by lmz_mie_ab_layered.  Runs
+        then move x_p, y_p, z_p and alpha only (lmz_lm_run_fixed_ab)."""
+        from ..theory.LayeredSphere import layer_table
+        t = _lmz.torch()
+        dev = self.params.device
+        b = self.batch
+        table, counts = layer_table(layers_list)
+        lt = t.from_numpy(table).to(dev)
+        nl = t.from_numpy(counts).to(dev)
+        self.fixed_nmax = t.zeros(2 * self.nfeat, dtype=t.int32, device=dev)
+        self.fixed_ab = t.zeros(b.coef_slots * 4, dtype=t.float64, device=dev)
+        work = t.zeros(b.work_doubles, dtype=t.float64, device=dev)
+        _lmz.check(_lmz.load_library().lmz_mie_ab_layered(
+            b.handle, _lmz.ptr(lt), _lmz.ptr(nl), _lmz.ptr(self.fixed_nmax),
+            None, _lmz.ptr(self.fixed_ab), _lmz.ptr(work), _lmz.stream_handle()))
+        if int(self.fixed_nmax[0::2].min()) < 1:
+            raise ValueError('invalid layer table')
+
     def run(self, settings: LMSettings, vary, stream=None) -> None:
         """Fit on the device, stream-ordered; results stay in HBM."""
         s = self.lm_settings(settings, vary)
-        _lmz.check(_lmz.load_library().lmz_lm_run(
+        lib = _lmz.load_library()
+        if getattr(self, 'fixed_ab', None) is not None:
+            _lmz.check(lib.lmz_lm_run_fixed_ab(
+                self.plan, _lmz.ptr(self.coords), self.flags, _lmz.ptr(self.data),
+                ctypes.byref(s), _lmz.ptr(self.fixed_ab), _lmz.ptr(self.fixed_nmax),
+                _lmz.ptr(self.params), _lmz.ptr(self.result), self._ws_ptr,
+                _lmz.stream_handle(stream)))
+            return
+        _lmz.check(lib.lmz_lm_run(
             self.plan, _lmz.ptr(self.coords), self.flags, _lmz.ptr(self.data),
             ctypes.byref(s), _lmz.ptr(self.params), _lmz.ptr(self.result),
             self._ws_ptr, _lmz.stream_handle(stream)))

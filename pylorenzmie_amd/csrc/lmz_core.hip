@@ -54,6 +54,40 @@ __global__ __launch_bounds__(64) void k_mie(
     nmax_out[(long long)f * nbuf + b] = N;
 }
 
+// Layered spheres, one lane per feature (FP64, sequential in n).  A
+// feature whose layer table is invalid (layer count outside 1..
+// LMZ_MAX_LAYERS, radii not positive and increasing, or a non-finite
+// index) gets n_max = -1 and no coefficients.
+__global__ __launch_bounds__(64) void k_mie_layered(
+    const lmz_feature *__restrict__ feat, int nfeat,
+    const double *__restrict__ layers, const int32_t *__restrict__ nlayers,
+    int nbuf, int32_t *__restrict__ nmax_out, float *__restrict__ coef,
+    double *__restrict__ ab_out, double *__restrict__ work) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfeat) return;
+    const lmz_feature F = feat[f];
+    const int L = nlayers[f];
+    double r[LMZ_MAX_LAYERS], nl[LMZ_MAX_LAYERS], kl[LMZ_MAX_LAYERS];
+    bool ok = L >= 1 && L <= LMZ_MAX_LAYERS;
+    for (int l = 0; ok && l < L; ++l) {
+        const double *q = layers + ((long long)f * LMZ_MAX_LAYERS + l) * 3;
+        r[l] = q[0];
+        nl[l] = q[1];
+        kl[l] = q[2];
+        ok = r[l] > (l ? r[l - 1] : 0.0) && isfinite(r[l]) && isfinite(nl[l]) &&
+             isfinite(kl[l]) && nl[l] > 0.0;
+    }
+    if (!ok) {
+        nmax_out[(long long)f * nbuf] = -1;
+        return;
+    }
+    const long long slot0 = F.coef_offset;
+    nmax_out[(long long)f * nbuf] = mie_layered_feature(
+        r, nl, kl, L, F.n_m, F.wavelength, F.nmax_cap, work + f, nfeat,
+        coef ? coef + LMZ_COEF_STRIDE * slot0 : nullptr,
+        ab_out ? ab_out + 4 * slot0 : nullptr);
+}
+
 // Per-order tables from FP64 a_n, b_n supplied by the caller.
 __global__ __launch_bounds__(64) void k_coef_from_ab(
     const lmz_feature *__restrict__ feat, int nfeat,
@@ -554,7 +588,7 @@ int lmz_batch_query(const lmz_batch *b, lmz_batch_sizes *s) {
     s->coef_buffers = b->coef_buffers;
     s->coef_slots = b->coef_slots;
     s->cap_max = b->cap_max;
-    s->work_doubles = 6LL * (b->cap_max + 1) * b->nfeat;
+    s->work_doubles = (long long)LMZ_WORK_PER_ORDER * (b->cap_max + 1) * b->nfeat;
     s->coef_stride = LMZ_COEF_STRIDE;
     s->dcoef_stride = LMZ_DCOEF_STRIDE;
     return LMZ_OK;
@@ -572,6 +606,20 @@ int lmz_mie_ab(const lmz_batch *b, const double *params, const int32_t *sel,
     hipLaunchKernelGGL(k_mie, dim3(grid), dim3(64), 0, stream, b->d_feat,
                        b->nfeat, params, sel, 0, b->coef_buffers, nmax_out,
                        coef, dcoef, ab_out, work, nullptr);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+int lmz_mie_ab_layered(const lmz_batch *b, const double *layers,
+                       const int32_t *nlayers, int32_t *nmax_out, float *coef,
+                       double *ab_out, double *work, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(layers && nlayers && nmax_out && work,
+                "layers, nlayers, nmax_out and work are required");
+    LMZ_REQUIRE(coef || ab_out, "nothing to compute: coef and ab_out are NULL");
+    hipLaunchKernelGGL(k_mie_layered, dim3((b->nfeat + 63) / 64), dim3(64), 0,
+                       stream, b->d_feat, b->nfeat, layers, nlayers,
+                       b->coef_buffers, nmax_out, coef, ab_out, work);
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
 }

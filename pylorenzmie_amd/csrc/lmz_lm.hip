@@ -50,7 +50,8 @@ static LMLayout lm_layout(const lmz_batch *b) {
     L.dcoef = take(sizeof(float) * LMZ_DCOEF_STRIDE * (size_t)b->coef_slots);
     L.nmax = take(sizeof(int32_t) * 2 * (size_t)b->nfeat);
     L.sel = take(sizeof(int32_t) * (size_t)b->nfeat);
-    L.work = take(sizeof(double) * 6 * (size_t)(b->cap_max + 1) * b->nfeat);
+    L.work = take(sizeof(double) * LMZ_WORK_PER_ORDER * (size_t)(b->cap_max + 1) *
+                  b->nfeat);
     L.partials = take(sizeof(double) * LMZ_NRED * (size_t)b->ntiles);
     L.chi2_tiles = take(sizeof(double) * (size_t)b->ntiles);
     L.state = take(sizeof(LMState) * (size_t)b->nfeat);
@@ -339,6 +340,7 @@ struct lmz_lm_plan {
     // arguments the captured graph was built for
     const void *key_ptrs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int32_t key_flags = -1;
+    int key_fixed = -1;
     lmz_lm_settings key_set{};
 };
 
@@ -350,10 +352,36 @@ static void plan_release_graph(lmz_lm_plan *p) {
 }
 
 // One LM iteration: the six launches a graph replays.
+// Coefficient tables of both buffers from caller-supplied FP64 a_n, b_n
+// (rows at each feature's buffer-0 slots), with zero tangents: the
+// coefficients stay fixed during the fit.  One lane per feature.
+__global__ __launch_bounds__(64) void k_lm_load_ab(
+    const lmz_feature *__restrict__ feat, int nfeat,
+    const double *__restrict__ ab, const int32_t *__restrict__ nmax_in,
+    int32_t *__restrict__ nmax, float *__restrict__ coef,
+    float *__restrict__ dcoef) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfeat) return;
+    const lmz_feature F = feat[f];
+    int N = nmax_in[2LL * f];
+    N = N < 0 ? 0 : (N > F.nmax_cap ? F.nmax_cap : N);
+    for (int b = 0; b < 2; ++b) {
+        const long long slot0 = F.coef_offset + (long long)b * F.nmax_cap;
+        for (int n = 1; n <= N; ++n) {
+            const double *r = ab + 4LL * (F.coef_offset + n - 1);
+            write_order(n, cd{r[0], r[1]}, cd{r[2], r[3]},
+                        coef + LMZ_COEF_STRIDE * (slot0 + n - 1));
+            float *drow = dcoef + LMZ_DCOEF_STRIDE * (slot0 + n - 1);
+            for (int i = 0; i < LMZ_DCOEF_STRIDE; ++i) drow[i] = 0.f;
+        }
+        nmax[2LL * f + b] = N;
+    }
+}
+
 static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
                               char *ws, const float *coords, int32_t flags,
                               const float *data, const lmz_lm_settings &set,
-                              double *params, hipStream_t s) {
+                              bool fixed_ab, double *params, hipStream_t s) {
     const lmz_batch *b = plan->batch;
     float *coef = reinterpret_cast<float *>(ws + L.coef);
     float *dcoef = reinterpret_cast<float *>(ws + L.dcoef);
@@ -380,7 +408,8 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
                     resid, np, nullptr, nullptr, partials, nullptr, &tl_j, s);
     hipLaunchKernelGGL(k_lm_solve, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
                        partials, st, params, ptrial, mask_j, mask_f, set);
-    launch_mie(b, ptrial, sel, 1, nmax, coef, dcoef, nullptr, work, mask_f, s);
+    if (!fixed_ab)
+        launch_mie(b, ptrial, sel, 1, nmax, coef, dcoef, nullptr, work, mask_f, s);
     launch_tile_list(b, mask_f, tl_f, s);
     launch_residual(b, ptrial, sel, 1, nmax, coef, coords, flags, data,
                     resid, np, chi2_tiles, nullptr, &tl_f, s);
@@ -437,10 +466,11 @@ int lmz_lm_plan_destroy(lmz_lm_plan *p) {
     return LMZ_OK;
 }
 
-int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
-               const float *data, const lmz_lm_settings *settings,
-               double *params_io, double *result, void *workspace,
-               hipStream_t stream) {
+static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
+                  const float *data, const lmz_lm_settings *settings,
+                  const double *ab_fixed, const int32_t *nmax_fixed,
+                  double *params_io, double *result, void *workspace,
+                  hipStream_t stream) {
     LMZ_REQUIRE(plan != nullptr && settings != nullptr, "NULL plan or settings");
     const lmz_batch *b = plan->batch;
     if (int rc = check_batch(b)) return rc;
@@ -458,6 +488,10 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
     int nv = 0;
     for (int i = 0; i < LMZ_NJAC; ++i) nv += set.vary[i] ? 1 : 0;
     LMZ_REQUIRE(nv >= 1, "no parameter is varied");
+    const bool fixed_ab = ab_fixed != nullptr;
+    LMZ_REQUIRE(!fixed_ab || (!set.vary[3] && !set.vary[4] && !set.vary[5]),
+                "a_p, n_p and k_p must be held fixed with fixed coefficients");
+    LMZ_REQUIRE(!fixed_ab || nmax_fixed, "nmax is required with ab");
     LMZ_REQUIRE((reinterpret_cast<uintptr_t>(workspace) & 255) == 0,
                 "workspace must be 256-byte aligned");
 
@@ -485,7 +519,12 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
     hipLaunchKernelGGL(k_lm_init, dim3((b->nfeat + 63) / 64), dim3(64), 0, s,
                        b->nfeat, st, params_io, ptrial, sel, mask_j, mask_f,
                        set.lambda0);
-    launch_mie(b, params_io, sel, 0, nmax, coef, dcoef, nullptr, work, nullptr, s);
+    if (fixed_ab)
+        hipLaunchKernelGGL(k_lm_load_ab, dim3((b->nfeat + 63) / 64), dim3(64), 0, s,
+                           b->d_feat, b->nfeat, ab_fixed, nmax_fixed, nmax, coef,
+                           dcoef);
+    else
+        launch_mie(b, params_io, sel, 0, nmax, coef, dcoef, nullptr, work, nullptr, s);
     launch_residual(b, params_io, sel, 0, nmax, coef, coords, flags, data,
                     resid, np, chi2_tiles, nullptr, nullptr, s);
     hipLaunchKernelGGL(k_lm_init_chi2, dim3(b->nfeat), dim3(64), 0, s, b->nfeat,
@@ -498,16 +537,19 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
         const void *key[5] = {coords, data, params_io, workspace, result};
         const bool same = plan->exec && std::memcmp(key, plan->key_ptrs, sizeof key) == 0 &&
                           plan->key_flags == flags &&
+                          plan->key_fixed == (int)fixed_ab &&
                           std::memcmp(&set, &plan->key_set, sizeof set) == 0;
         if (!same) {
             plan_release_graph(plan);
             LMZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            enqueue_iteration(plan, L, ws, coords, flags, data, set, params_io, s);
+            enqueue_iteration(plan, L, ws, coords, flags, data, set, fixed_ab,
+                              params_io, s);
             hipError_t e = hipStreamEndCapture(s, &plan->graph);
             if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
             LMZ_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
             std::memcpy(plan->key_ptrs, key, sizeof key);
             plan->key_flags = flags;
+            plan->key_fixed = (int)fixed_ab;
             plan->key_set = set;
         }
     }
@@ -515,7 +557,8 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
         if (set.use_graph) {
             LMZ_HIP(hipGraphLaunch(plan->exec, s));
         } else {
-            enqueue_iteration(plan, L, ws, coords, flags, data, set, params_io, s);
+            enqueue_iteration(plan, L, ws, coords, flags, data, set, fixed_ab,
+                              params_io, s);
             LMZ_HIP(hipGetLastError());
         }
         if ((it + 1) % set.check_every == 0 && it + 1 < set.max_iter) {
@@ -531,6 +574,24 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
     LMZ_HIP(hipEventRecord(plan->ev_out, s));
     LMZ_HIP(hipStreamWaitEvent(stream, plan->ev_out, 0));
     return LMZ_OK;
+}
+
+int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
+               const float *data, const lmz_lm_settings *settings,
+               double *params_io, double *result, void *workspace,
+               hipStream_t stream) {
+    return lm_run(plan, coords, flags, data, settings, nullptr, nullptr,
+                  params_io, result, workspace, stream);
+}
+
+int lmz_lm_run_fixed_ab(lmz_lm_plan *plan, const float *coords, int32_t flags,
+                        const float *data, const lmz_lm_settings *settings,
+                        const double *ab, const int32_t *nmax,
+                        double *params_io, double *result, void *workspace,
+                        hipStream_t stream) {
+    LMZ_REQUIRE(ab != nullptr, "ab is NULL");
+    return lm_run(plan, coords, flags, data, settings, ab, nmax, params_io,
+                  result, workspace, stream);
 }
 
 }  // extern "C"

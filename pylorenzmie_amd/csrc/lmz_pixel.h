@@ -600,6 +600,8 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
 // ====================================================================
 // Mie coefficients in FP64 with a_p, n_p, k_p tangents
 // ====================================================================
+// doubles of scratch per order and feature (lmz_batch_sizes.work_doubles)
+#define LMZ_WORK_PER_ORDER 8
 // Sphere.wiscombe_yang, theory/Sphere.py:91-123 (one layer: the rolled
 // term equals |x m|)
 LMZ_HD int wiscombe_yang(double x, cd m) {
@@ -650,7 +652,8 @@ LMZ_HD void write_dorder(int n, cd da, cd db, float *drow) {
 // (D1 downward from D1[nmax] = 0 for z = m x and z = x; Psi, zeta, Psi*zeta,
 // D3 upward; Eqs. 5 and 6), carried with tangents d/dx (real) and d/dm
 // (holomorphic), then mapped to a_p (x = k a_p), n_p and k_p
-// (m = (n_p + i k_p) / n_m).  work(n, j) = work[(n * 6 + j) * wstride].
+// (m = (n_p + i k_p) / n_m).  work(n, j) = work[(n * LMZ_WORK_PER_ORDER + j)
+// * wstride].
 // Returns n_max (clamped to cap).
 LMZ_HD int mie_feature(double a_p, double n_p, double k_p, double n_m,
                        double wavelength, int cap, double *work,
@@ -662,7 +665,7 @@ LMZ_HD int mie_feature(double a_p, double n_p, double k_p, double n_m,
     int N = wiscombe_yang(x, m);
     if (N > cap) N = cap;
     if (N < 1) N = 1;
-#define W(n, j) work[((long long)(n) * 6 + (j)) * wstride]
+#define W(n, j) work[((long long)(n) * LMZ_WORK_PER_ORDER + (j)) * wstride]
     // downward D1(z), z = m x, with dD1/dz; D1(x) with dD1/dx
     {
         const cd z = x * m;
@@ -751,6 +754,155 @@ LMZ_HD int mie_feature(double a_p, double n_p, double k_p, double n_m,
             write_dorder(n, mul_i((1.0 / n_m) * a_m), mul_i((1.0 / n_m) * b_m),
                          drow + 16);                                     // k_p
         }
+        if (ab) {
+            double *r = ab + 4 * (n - 1);
+            r[0] = a.re; r[1] = a.im; r[2] = b.re; r[3] = b.im;
+        }
+    }
+#undef W
+    return N;
+}
+
+// Order count of a layered sphere: Wiscombe (1980) on the outer size
+// parameter with Yang (2003) Eq. 30, N = max(n_s, |m_l x_l|, |m_l x_{l-1}|).
+// theory/Sphere.py:91-123 states the same rule; its np.roll term pairs x_l
+// with m_{l-1} and wraps around, which the reference never reaches (its
+// scalar-only branch fails for arrays), so Yang's pairing is used.
+LMZ_HD int wiscombe_yang_layers(const double *x, const cd *m, int L) {
+    int N = wiscombe_yang(x[L - 1], cd{0.0, 0.0});
+    for (int l = 0; l < L; ++l) {
+        N = (int)fmax((double)N, hypot(x[l] * m[l].re, x[l] * m[l].im));
+        if (l > 0)
+            N = (int)fmax((double)N, hypot(x[l - 1] * m[l].re, x[l - 1] * m[l].im));
+    }
+    return N;
+}
+
+LMZ_HD cd cexp_i2(cd z) {          // exp(2 i z)
+    const double r = exp(-2.0 * z.im);
+    double sn, cs;
+    sincos(2.0 * z.re, &sn, &cs);
+    return cd{r * cs, r * sn};
+}
+
+// Layered sphere (L layers, innermost first, outer radii increasing):
+// Yang, Appl. Opt. 42, 1710 (2003) Eqs. 7-8 and 19 with Pena & Pal (2009)
+// Eqs. 5-6, the recursion lib/obsolete/sphere_coefficients.py:126-215
+// describes.  H_a = H_b = D1(m_1 x_1) in the core; for each further layer
+// l, D1 downward at z_a = m_l x_{l-1} and z_b = m_l x_l from D1[N] = 0 (as
+// theory/Sphere.py:190), D3 upward through Psi zeta, the ratio
+// Q_n = [Psi_n/zeta_n](z_a) / [Psi_n/zeta_n](z_b) upward, and the update of
+// H_a, H_b; then Eqs. 5-6 on the outer boundary.  FP64, no tangents; one
+// layer reproduces mie_feature's coefficients.  layer l: radius[l] [um],
+// index n[l] + i k[l].  work as mie_feature.  Returns n_max (clamped).
+LMZ_HD int mie_layered_feature(const double *radius, const double *nl,
+                               const double *kl, int L, double n_m,
+                               double wavelength, int cap, double *work,
+                               long long wstride, float *coef, double *ab) {
+    const double kum = 2.0 * M_PI / wavelength * n_m;
+    double x[LMZ_MAX_LAYERS];
+    cd m[LMZ_MAX_LAYERS];
+    for (int l = 0; l < L; ++l) {
+        x[l] = kum * radius[l];
+        m[l] = cd{nl[l] / n_m, kl[l] / n_m};
+    }
+    int N = wiscombe_yang_layers(x, m, L);
+    if (N > cap) N = cap;
+    if (N < 1) N = 1;
+#define W(n, j) work[((long long)(n) * LMZ_WORK_PER_ORDER + (j)) * wstride]
+    // D1 downward from D1[N] = 0 into work columns (c, c + 1), orders 0..N
+    auto d1_down = [&](cd z, int c) {
+        const cd iz = cinv(z);
+        cd d1{0.0, 0.0};
+        W(N, c) = 0.0;
+        W(N, c + 1) = 0.0;
+        for (int n = N; n >= 1; --n) {
+            const cd nz = (double)n * iz;
+            d1 = nz - cinv(d1 + nz);
+            W(n - 1, c) = d1.re;
+            W(n - 1, c + 1) = d1.im;
+        }
+    };
+    // core: H_a = H_b = D1(m_1 x_1) in columns 0..3
+    d1_down(x[0] * m[0], 0);
+    for (int n = 0; n <= N; ++n) {
+        W(n, 2) = W(n, 0);
+        W(n, 3) = W(n, 1);
+    }
+    for (int l = 1; l < L; ++l) {
+        const cd za = x[l - 1] * m[l], zb = x[l] * m[l];
+        d1_down(za, 4);
+        d1_down(zb, 6);
+        const cd iza = cinv(za), izb = cinv(zb);
+        const cd ea = cexp_i2(za), eb = cexp_i2(zb);
+        cd pza = 0.5 * (cd{1.0, 0.0} - ea), pzb = 0.5 * (cd{1.0, 0.0} - eb);
+        cd d3a{0.0, 1.0}, d3b{0.0, 1.0};
+        // Q_0 = exp(2i(zb - za)) (exp(2i za) - 1) / (exp(2i zb) - 1)
+        cd q = cexp_i2(zb - za) * (ea - cd{1.0, 0.0}) / (eb - cd{1.0, 0.0});
+        const double r2 = (x[l - 1] / x[l]) * (x[l - 1] / x[l]);
+        for (int n = 1; n <= N; ++n) {
+            const cd d1a_m{W(n - 1, 4), W(n - 1, 5)}, d1b_m{W(n - 1, 6), W(n - 1, 7)};
+            const cd d1a{W(n, 4), W(n, 5)}, d1b{W(n, 6), W(n, 7)};
+            const cd nza = (double)n * iza, nzb = (double)n * izb;
+            // Q_n with D3_{n-1}, Yang Eq. 19
+            const cd num = (zb * d1b + cd{(double)n, 0.0}) *
+                           (cd{(double)n, 0.0} - zb * d3b);
+            const cd den = (za * d1a + cd{(double)n, 0.0}) *
+                           (cd{(double)n, 0.0} - za * d3a);
+            q = (r2 * q) * num / den;
+            // D3_n through Psi_n zeta_n, Yang Eqs. 18c-d
+            pza = pza * ((nza - d1a_m) * (nza - d3a));
+            pzb = pzb * ((nzb - d1b_m) * (nzb - d3b));
+            d3a = d1a + mul_i(cinv(pza));
+            d3b = d1b + mul_i(cinv(pzb));
+            // H_a, H_b of the layer, Yang Eqs. 7b and 8b
+            const cd ha{W(n, 0), W(n, 1)}, hb{W(n, 2), W(n, 3)};
+            cd g1 = m[l] * ha - m[l - 1] * d1a;
+            cd g2 = m[l] * ha - m[l - 1] * d3a;
+            const cd han = (g2 * d1b - q * g1 * d3b) / (g2 - q * g1);
+            g1 = m[l - 1] * hb - m[l] * d1a;
+            g2 = m[l - 1] * hb - m[l] * d3a;
+            const cd hbn = (g2 * d1b - q * g1 * d3b) / (g2 - q * g1);
+            W(n, 0) = han.re; W(n, 1) = han.im;
+            W(n, 2) = hbn.re; W(n, 3) = hbn.im;
+        }
+    }
+    // outer boundary: D1(x_L) downward (real) in column 4
+    const double xL = x[L - 1];
+    {
+        double r1 = 0.0;
+        W(N, 4) = 0.0;
+        for (int n = N; n >= 1; --n) {
+            const double nx = n / xL;
+            r1 = nx - 1.0 / (r1 + nx);
+            W(n - 1, 4) = r1;
+        }
+    }
+    double sx, cx;
+    sincos(xL, &sx, &cx);
+    double s2x, c2x;
+    sincos(2.0 * xL, &s2x, &c2x);
+    double psi = sx;
+    cd zeta{sx, -cx};
+    cd pz{0.5 * (1.0 - c2x), -0.5 * s2x};
+    cd d3{0.0, 1.0};
+    const cd mL = m[L - 1], imL = cinv(mL);
+    for (int n = 1; n <= N; ++n) {
+        const double nx = n / xL;
+        const double w1 = nx - W(n - 1, 4);
+        const cd w3 = cd{nx, 0.0} - d3;
+        const double psi_m = psi;
+        const cd zeta_m = zeta;
+        psi = psi_m * w1;
+        zeta = zeta_m * w3;
+        pz = pz * (w1 * w3);
+        d3 = cd{W(n, 4), 0.0} + mul_i(cinv(pz));
+        const cd ha{W(n, 0), W(n, 1)}, hb{W(n, 2), W(n, 3)};
+        const cd fa = ha * imL + cd{nx, 0.0};
+        const cd a = (fa * cd{psi, 0.0} - cd{psi_m, 0.0}) / (fa * zeta - zeta_m);
+        const cd fb = mL * hb + cd{nx, 0.0};
+        const cd b = (fb * cd{psi, 0.0} - cd{psi_m, 0.0}) / (fb * zeta - zeta_m);
+        if (coef) write_order(n, a, b, coef + LMZ_COEF_STRIDE * (n - 1));
         if (ab) {
             double *r = ab + 4 * (n - 1);
             r[0] = a.re; r[1] = a.im; r[2] = b.re; r[3] = b.im;

@@ -7,10 +7,11 @@ and ``LMHologram`` are aliases.
 from .Instrument import Instrument
 from .Particle import Particle
 from .Sphere import Sphere
+from .LayeredSphere import LayeredSphere
 from .LorenzMie import LorenzMie, PARAMS
 
 hipLorenzMie = LorenzMie
 LMHologram = LorenzMie
 
-__all__ = ['Instrument', 'Particle', 'Sphere', 'LorenzMie', 'hipLorenzMie',
+__all__ = ['Instrument', 'Particle', 'Sphere', 'LayeredSphere', 'LorenzMie', 'hipLorenzMie',
            'LMHologram', 'PARAMS']

@@ -3,8 +3,8 @@
 The batched counterpart of theory/torchLorenzMieBatch.py:258-431
 (``batch_hologram``), without its shared-``npts`` restriction: each
 feature brings its own crop (dense grid) or coordinate list, its own
-Mie coefficients (FP64 kernel, one lane per feature) and one tile range
-of the FP32 pixel kernel.
+Mie coefficients (FP64 kernel, one lane per feature; homogeneous or
+layered spheres) and one tile range of the FP64 forward kernel.
 """
 from __future__ import annotations
 
@@ -55,14 +55,43 @@ class HologramBatch:
         p[:, :7] = np.asarray(params, float)[:, :7]
         self.params.copy_(t.from_numpy(p))
 
+    def load_layers(self, layers_list) -> None:
+        """Layered particles: per feature an (L, 3) array of outer radius
+        [um], n, k, innermost first (one row for a homogeneous sphere).
+        Their coefficients then come from lmz_mie_ab_layered and do not
+        depend on params' a_p, n_p, k_p columns."""
+        from .LayeredSphere import layer_table
+        t = _lmz.torch()
+        if len(layers_list) != self.nfeat:
+            raise ValueError(f'need {self.nfeat} layer tables')
+        table, counts = layer_table(layers_list)
+        dev = self.params.device
+        self.layers = (t.from_numpy(table).to(dev), t.from_numpy(counts).to(dev))
+
+    def coefficients(self, stream=None):
+        """Mie coefficient tables of every feature (homogeneous spheres from
+        params, or the loaded layer tables); returns nmax per feature."""
+        lib = _lmz.load_library()
+        s = _lmz.stream_handle(stream)
+        b = self.batch
+        layers = getattr(self, 'layers', None)
+        if layers is None:
+            _lmz.check(lib.lmz_mie_ab(b.handle, _lmz.ptr(self.params), None,
+                                      _lmz.ptr(self.nmax), _lmz.ptr(self.coef),
+                                      None, None, _lmz.ptr(self.work), s))
+        else:
+            _lmz.check(lib.lmz_mie_ab_layered(
+                b.handle, _lmz.ptr(layers[0]), _lmz.ptr(layers[1]),
+                _lmz.ptr(self.nmax), _lmz.ptr(self.coef), None,
+                _lmz.ptr(self.work), s))
+        return self.nmax
+
     def compute(self, stream=None):
         """Mie coefficients then all holograms; returns the packed tensor."""
         lib = _lmz.load_library()
         s = _lmz.stream_handle(stream)
         b = self.batch
-        _lmz.check(lib.lmz_mie_ab(b.handle, _lmz.ptr(self.params), None,
-                                  _lmz.ptr(self.nmax), _lmz.ptr(self.coef),
-                                  None, None, _lmz.ptr(self.work), s))
+        self.coefficients(stream)
         _lmz.check(lib.lmz_hologram(b.handle, _lmz.ptr(self.params), None,
                                     _lmz.ptr(self.nmax), _lmz.ptr(self.coef),
                                     None, self.flags, None, _lmz.ptr(self.out),

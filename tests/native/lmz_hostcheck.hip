@@ -14,9 +14,17 @@ extern "C" {
 // FP64 Mie coefficients with tangents; returns n_max.
 int chk_mie(double a_p, double n_p, double k_p, double n_m, double wavelength,
             int cap, float *coef, float *dcoef, double *ab) {
-    std::vector<double> work(6 * (cap + 1));
+    std::vector<double> work(LMZ_WORK_PER_ORDER * (cap + 1));
     return mie_feature(a_p, n_p, k_p, n_m, wavelength, cap, work.data(), 1,
                        coef, dcoef, ab);
+}
+
+// FP64 layered-sphere coefficients; returns n_max.
+int chk_mie_layered(const double *radius, const double *nl, const double *kl,
+                    int L, double n_m, double wavelength, int cap, double *ab) {
+    std::vector<double> work(LMZ_WORK_PER_ORDER * (cap + 1));
+    return mie_layered_feature(radius, nl, kl, L, n_m, wavelength, cap,
+                               work.data(), 1, nullptr, ab);
 }
 
 // Field (flags as in lmz.h), hologram and Jacobian of one feature at

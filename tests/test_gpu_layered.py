@@ -1,0 +1,247 @@
+"""Hard regime: layered spheres (2-5 layers, Yang's recursion in FP64 on the
+GPU) and large homogeneous spheres (a_p to 10 um, n_max to ~280) on 512 x
+512 crops.  Layer radii sorted uniform within a_p, layer n uniform
+[1.33, 2.0], z_p above a_p / magnification + 30 px.
+
+Parity: coefficients against the oracle's Yang recursion (itself pinned
+to the coated-sphere closed form) to 1e-9 of max |ab|; holograms against
+the oracle at sampled pixels to 1e-5 plus the float32 rounding of the
+stored value (2^-23 I); Jacobian columns against the oracle's 6th-order
+FP64 differences; fits recover the truth."""
+import numpy as np
+import pytest
+
+from conftest import INSTRUMENT
+from oracle import lorenzmie as ref
+
+pytestmark = pytest.mark.gpu
+
+MAG = INSTRUMENT['magnification']
+
+
+@pytest.fixture(scope='module')
+def torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no GPU')
+    return torch
+
+
+def draw(rng, n, side, layered_frac=0.5, a_max=10.):
+    """(params (n, 7), layer tables) of the hard-regime distribution."""
+    params, layers = np.zeros((n, 7)), []
+    for i in range(n):
+        if rng.random() < layered_frac:
+            L = int(rng.integers(2, 6))
+            a_p = rng.uniform(0.5, a_max)
+            radii = np.sort(rng.uniform(0.05, 1., L)) * a_p
+            radii[-1] = a_p
+            nl = rng.uniform(1.33, 2.0, L)
+            kl = np.zeros(L)
+        else:
+            a_p = rng.uniform(0.5, a_max)
+            radii, nl, kl = np.array([a_p]), rng.uniform(1.33, 2.0, 1), np.zeros(1)
+        layers.append(np.stack([radii, nl, kl], 1))
+        c = (side - 1) / 2
+        z = a_p / MAG + 30. + rng.uniform(0., 250.)
+        params[i] = (c + rng.uniform(-5, 5), c + rng.uniform(-5, 5), z, a_p,
+                     nl[-1], kl[-1], rng.uniform(0.8, 1.2))
+    return params, layers
+
+
+def oracle_ab(lay):
+    lay = np.asarray(lay)
+    if len(lay) == 1:
+        return ref.mie_coefficients(lay[0, 0], lay[0, 1], lay[0, 2], 1.34, 0.447)
+    return ref.mie_coefficients_layered(lay[:, 0], lay[:, 1], lay[:, 2], 1.34, 0.447)
+
+
+def test_layered_coefficients_match_oracle(torch):
+    from pylorenzmie_amd.theory.LayeredSphere import layered_ab
+    rng = np.random.default_rng(40)
+    _, layers = draw(rng, 48, 512)
+    layers.append(np.array([[10., 2.0, 0.]]))          # the largest n_max
+    got = layered_ab(layers, 1.34, 0.447)
+    worst = 0.
+    for g, lay in zip(got, layers):
+        want = oracle_ab(lay)
+        assert g.shape == want.shape
+        worst = max(worst, np.abs(g - want).max() / np.abs(want).max())
+    print('max err / max |ab|:', worst, 'largest n_max', got[-1].shape[0] - 1)
+    assert got[-1].shape[0] - 1 >= 280
+    assert worst < 1e-9
+
+
+def test_layered_sphere_particle_in_lorenzmie(torch):
+    from pylorenzmie_amd.theory import Instrument, LayeredSphere, LorenzMie, Sphere
+    c = ref.meshgrid((40, 40))
+    ls = LayeredSphere(a_p=[0.4, 0.9], n_p=[1.7, 1.45], k_p=[0., 0.001],
+                       x_p=20.3, y_p=19.6, z_p=120.)
+    m = LorenzMie(coordinates=c, particle=ls, instrument=Instrument(**INSTRUMENT))
+    h = m.hologram()
+    want = ref.hologram(c, dict(x_p=20.3, y_p=19.6, z_p=120.,
+                                ab=oracle_ab(ls.layers)), **INSTRUMENT)
+    assert np.abs(h - want).max() < 1e-5
+    # one layer is the homogeneous sphere
+    one = LayeredSphere(a_p=[0.9], n_p=[1.45], k_p=[0.001])
+    np.testing.assert_allclose(one.ab(1.34, 0.447),
+                               Sphere(a_p=0.9, n_p=1.45, k_p=0.001).ab(1.34, 0.447),
+                               rtol=0, atol=1e-12)
+
+
+def sample_pixels(rng, side, p, n=1500):
+    """Random pixels plus the 21 x 21 block around the particle."""
+    idx = rng.choice(side * side, n, replace=False)
+    cx, cy = int(round(p[0])), int(round(p[1]))
+    yy, xx = np.mgrid[cy - 10:cy + 11, cx - 10:cx + 11]
+    idx = np.unique(np.concatenate([idx, (yy * side + xx).ravel()]))
+    return idx
+
+
+@pytest.mark.parametrize('nfeat', [512])
+def test_hard_regime_holograms_full_batch(torch, nfeat):
+    """512 crops of 512 x 512 in one launch; 6 features checked against the
+    oracle at sampled pixels, all checked finite and non-negative."""
+    from pylorenzmie_amd.theory import Instrument
+    from pylorenzmie_amd.theory.batch import HologramBatch
+    from pylorenzmie_amd.theory.LayeredSphere import layered_cap
+    side = 512
+    rng = np.random.default_rng(4)
+    params, layers = draw(rng, nfeat, side)
+    inst = Instrument(**INSTRUMENT)
+    caps = [layered_cap(lay, 1.34, 0.447) for lay in layers]
+    hb = HologramBatch([(side, side)] * nfeat, caps, instrument=inst)
+    hb.load(params)
+    hb.load_layers(layers)
+    out = hb.compute()
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(out).all()) and float(out.min()) >= 0.
+    nm = hb.nmax.cpu().numpy()
+    assert nm.max() >= 200
+    host = out.cpu().numpy()
+    picks = [int(np.argmax(nm)), int(np.argmax(params[:, 3] * (nm > 0)))]
+    picks += [i for i in range(nfeat) if len(layers[i]) >= 4][:2]
+    picks += [i for i in range(nfeat) if len(layers[i]) == 2][:2]
+    grid = ref.meshgrid((side, side))
+    for i in picks:
+        idx = sample_pixels(rng, side, params[i])
+        want = ref.hologram(grid[:, idx], dict(x_p=params[i, 0], y_p=params[i, 1],
+                                                z_p=params[i, 2],
+                                                ab=oracle_ab(layers[i])),
+                            alpha=params[i, 6], **INSTRUMENT)
+        got = host[hb.offsets[i] + idx]
+        err = np.abs(got - want) - 2. ** -23 * np.abs(want)
+        print(f'feature {i}: L={len(layers[i])} a_p={params[i, 3]:.2f} '
+              f'N={nm[i]} max I {want.max():.1f} err {np.abs(got - want).max():.2e}')
+        assert err.max() <= 1e-5
+
+
+def test_large_sphere_jacobian(torch):
+    """a_p = 9 um (N ~ 250): the FP32 Jacobian against FP64 differences."""
+    from pylorenzmie_amd.theory import Instrument, LorenzMie, Sphere
+    q = (20.2, 19.7, 9. / MAG + 60., 9.0, 1.45, 0.002, 1.05)
+    c = ref.meshgrid((40, 40))
+    s = Sphere(x_p=q[0], y_p=q[1], z_p=q[2], a_p=q[3], n_p=q[4], k_p=q[5])
+    m = LorenzMie(coordinates=c, particle=s, instrument=Instrument(**INSTRUMENT),
+                  alpha=q[6])
+    J = m.jac()
+    Jref = ref.jacobian(c, dict(zip(ref.PARAMS, q)), INSTRUMENT, order=6)
+    for i, name in enumerate(ref.PARAMS):
+        rel = np.abs(J[name] - Jref[i]).max() / np.abs(Jref[i]).max()
+        print(f'{name}: {rel:.2e}')
+        assert rel <= 1e-3, name
+
+
+def lm_crops(data_host, side, caps):
+    return [dict(data=data_host[i], width=side, origin=(0., 0., 0.),
+                 nmax_cap=caps[i]) for i in range(len(caps))]
+
+
+def noisy(torch, hb, sigma, seed):
+    clean = hb.compute().clone()
+    g = torch.Generator(device='cuda')
+    g.manual_seed(seed)
+    return clean + sigma * torch.randn(clean.shape, generator=g, device='cuda',
+                                       dtype=torch.float32)
+
+
+def test_layered_lm_recovers_position_and_alpha(torch):
+    """Fixed layered coefficients, fit x_p, y_p, z_p, alpha from starts
+    off by +-2 px and 10% in alpha (lmz_lm_run_fixed_ab)."""
+    from pylorenzmie_amd.analysis.lm import LMBatch, LMSettings
+    from pylorenzmie_amd.theory import Instrument
+    from pylorenzmie_amd.theory.batch import HologramBatch
+    from pylorenzmie_amd.theory.LayeredSphere import layered_cap
+    side, n = 256, 24
+    rng = np.random.default_rng(8)
+    params, layers = draw(rng, n, side, layered_frac=1.0, a_max=4.)
+    inst = Instrument(noise=0.05, **INSTRUMENT)
+    caps = [layered_cap(lay, 1.34, 0.447) for lay in layers]
+    hb = HologramBatch([(side, side)] * n, caps, instrument=inst)
+    hb.load(params)
+    hb.load_layers(layers)
+    data = noisy(torch, hb, 0.05, 8)
+    lm = LMBatch(lm_crops(np.zeros((n, side, side), np.float32), side, caps), inst)
+    lm.data.copy_(data)
+    lm.load_layers(layers)
+    start = params.copy()
+    start[:, 0] += 2.
+    start[:, 1] -= 2.
+    start[:, 2] += 2.
+    start[:, 6] *= 1.1
+    lm.load(start)
+    lm.run(LMSettings(max_iter=50), ['x_p', 'y_p', 'z_p', 'alpha'])
+    p, res = lm.fetch()
+    status = res[:, 38].astype(int)
+    ok = ((status & 7) != 0) & ((status & 56) == 0)
+    print('converged', ok.mean(), 'iters', res[:, 37].mean())
+    assert ok.mean() >= 0.9
+    err = np.abs(p[ok, :7] - params[ok])
+    assert np.median(err[:, 0]) < 0.02 and np.median(err[:, 2]) < 0.2
+    assert np.median(err[:, 6]) < 0.01
+    np.testing.assert_array_equal(p[:, 3:6], start[:, 3:6])   # held
+
+
+def test_large_homogeneous_lm(torch):
+    """a_p in [3, 10] um on 512 x 512 crops, all seven parameters.  The
+    chi2 surface of such spheres is multimodal on the scale of a fraction
+    of a radian of optical path through the sphere (2 pi (n_p - n_m) 2 a_p
+    / lambda is ~45 rad at 10 um): from +-1 px, 0.2% a_p, 0.001 n_p, five
+    of these twelve fits settle in neighbouring minima (reduced chi2 1.2 to
+    4, the same with 300 iterations and 1e-12 tolerances), so the starts
+    here sit inside the true basin: +-0.3 px, 0.05% a_p, 0.0003 n_p."""
+    from pylorenzmie_amd import _lmz
+    from pylorenzmie_amd.analysis.lm import LMBatch, LMSettings
+    from pylorenzmie_amd.theory import Instrument
+    from pylorenzmie_amd.theory.batch import HologramBatch
+    side, n = 512, 12
+    rng = np.random.default_rng(12)
+    params, _ = draw(rng, n, side, layered_frac=0.0)
+    params[:, 3] = rng.uniform(3., 10., n)
+    params[:, 2] = params[:, 3] / MAG + 30. + rng.uniform(0, 250., n)
+    params[:, 4] = rng.uniform(1.38, 1.65, n)
+    inst = Instrument(noise=0.05, **INSTRUMENT)
+    caps = [_lmz.nmax_cap(p[3], p[4], p[5], 1.34, 0.447) for p in params]
+    hb = HologramBatch([(side, side)] * n, caps, instrument=inst)
+    hb.load(params)
+    data = noisy(torch, hb, 0.05, 12)
+    lm = LMBatch(lm_crops(np.zeros((n, side, side), np.float32), side, caps), inst)
+    lm.data.copy_(data)
+    start = params.copy()
+    start[:, 0] += 0.3
+    start[:, 1] -= 0.3
+    start[:, 2] -= 0.3
+    start[:, 3] *= 1.0005
+    start[:, 4] += 0.0003
+    lm.load(start)
+    lm.run(LMSettings(max_iter=50), list(ref.PARAMS))
+    p, res = lm.fetch()
+    status = res[:, 38].astype(int)
+    ok = ((status & 7) != 0) & ((status & 56) == 0)
+    redchi = res[:, 35] / (side * side - 7)
+    print('converged', ok.mean(), 'iters', res[:, 37], 'redchi', redchi)
+    assert np.all(np.isfinite(p))
+    good = ok & (redchi < 1.02)
+    assert good.mean() >= 0.9
+    rel = np.abs(p[good, 3:5] - params[good, 3:5]) / params[good, 3:5]
+    assert np.median(rel[:, 0]) < 1e-3 and np.median(rel[:, 1]) < 1e-3

@@ -25,6 +25,9 @@ def chk():
     P = ctypes.c_void_p
     lib.chk_mie.restype = ctypes.c_int
     lib.chk_mie.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, P, P, P]
+    lib.chk_mie_layered.restype = ctypes.c_int
+    lib.chk_mie_layered.argtypes = [P, P, P, ctypes.c_int, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_int, P]
     lib.chk_pixels.argtypes = [P, P, P, P, ctypes.c_int, P, ctypes.c_int,
                                ctypes.c_int, P, P, P]
     lib.chk_dense_coords.argtypes = [ctypes.c_int, ctypes.c_int, P, P]
@@ -152,3 +155,41 @@ def test_dense_pixel_mapping(chk):
         j = np.arange(n)
         assert np.array_equal(ax, (j % w).astype(np.float32))
         assert np.array_equal(ay, (j // w).astype(np.float32))
+
+
+def mie_layered(chk, radii, nl, kl, cap=600):
+    lib, _ = chk
+    r, n, k = (np.ascontiguousarray(v, float) for v in (radii, nl, kl))
+    ab = np.zeros((cap, 4))
+    N = lib.chk_mie_layered(_p(r), _p(n), _p(k), len(r), 1.34, 0.447, cap, _p(ab))
+    return N, ab[:N, 0::2] + 1j * ab[:N, 1::2]
+
+
+@pytest.mark.parametrize('seed', range(6))
+def test_layered_mie_matches_oracle(chk, seed):
+    """2-5 layers, radii sorted uniform within a_p <= 10 um, n in
+    [1.33, 2.0] (the hard-regime draws), against the oracle's Yang
+    recursion with the same D1 start."""
+    rng = np.random.default_rng(100 + seed)
+    L = int(rng.integers(2, 6))
+    a_p = rng.uniform(0.5, 10.)
+    radii = np.sort(rng.uniform(0.05, 1., L)) * a_p
+    radii[-1] = a_p
+    nl = rng.uniform(1.33, 2.0, L)
+    kl = rng.uniform(0., 0.01, L) * (rng.random(L) < 0.5)
+    N, ab = mie_layered(chk, radii, nl, kl)
+    want = ref.mie_coefficients_layered(radii, nl, kl, 1.34, 0.447)
+    assert N == want.shape[0] - 1
+    scale = np.abs(want).max()
+    err = np.abs(ab - want[1:]).max() / scale
+    print(f'L={L} a_p={a_p:.2f} N={N}: max err / max |ab| {err:.2e}')
+    assert err < 1e-9
+
+
+def test_layered_one_layer_equals_homogeneous(chk):
+    for a, n, k in ((0.75, 1.45, 0.), (2.5, 1.6, 0.003), (9.5, 1.9, 0.)):
+        N1, ab1 = mie_layered(chk, [a], [n], [k])
+        N, _, _, ab = mie(chk, a, n, k, cap=600)
+        assert N1 == N
+        got = np.stack([ab[:N, 0] + 1j * ab[:N, 1], ab[:N, 2] + 1j * ab[:N, 3]], 1)
+        assert np.abs(ab1 - got).max() < 1e-12 * np.abs(got).max()

@@ -87,6 +87,18 @@ typedef struct lmz_feature {
 
 typedef struct lmz_batch lmz_batch;
 
+/* Launch shape of the pixel kernels of a batch.  A tile of LMZ_TILE pixels
+ * is worked by jac_threads (fwd_threads) threads: 64 (one wavefront per
+ * tile), 128 or 256; the LM's persistent launches use *_blocks_per_cu
+ * workgroups per CU (0: as many as are resident at once).  Default
+ * {256, 256, 0, 0}; lmz_autotune picks one by timing. */
+typedef struct lmz_schedule {
+    int32_t jac_threads;
+    int32_t fwd_threads;
+    int32_t jac_blocks_per_cu;
+    int32_t fwd_blocks_per_cu;
+} lmz_schedule;
+
 typedef struct lmz_batch_sizes {
     int32_t nfeat;
     int32_t ntiles;
@@ -205,6 +217,30 @@ int lmz_hologram_vjp(const lmz_batch *batch, const double *params,
                      const float *dcoef, const float *coords, int32_t flags,
                      const float *grad_out, double *tile_work,
                      double *grad_params, hipStream_t stream);
+
+/* Launch shape of a batch's pixel kernels (see lmz_schedule). */
+int lmz_batch_set_schedule(lmz_batch *batch, const lmz_schedule *schedule);
+int lmz_batch_get_schedule(const lmz_batch *batch, lmz_schedule *schedule);
+
+/* Autotuner: times the Jacobian kernel (with its J^T J reduction) and the
+ * residual kernel as the LM launches them (persistent grid over a
+ * device-built tile list), over every tile and over the tiles of every
+ * 8th feature (the LM's later iterations; weighted 8x), for 64, 128 and
+ * 256 threads per tile and 1, 2 and 4 times the resident workgroups per
+ * CU; keeps the fastest of each in the batch's schedule and returns it.
+ * Results are cached per process (behind a mutex) by device, tile count,
+ * mean order count and coordinate layout; LMZ_TUNE_FORCE re-times.
+ * Synchronizes the stream.  Inputs as lmz_jacobian with data; scratch is a
+ * 256-byte aligned device buffer of lmz_autotune_scratch_bytes.
+ * Has no counterpart in the reference (its kernels have fixed shapes,
+ * theory/cupyLorenzMie.py:73-75); it serves every caller of the batched
+ * kernels. */
+#define LMZ_TUNE_FORCE 1
+int lmz_autotune_scratch_bytes(const lmz_batch *batch, size_t *bytes);
+int lmz_autotune(lmz_batch *batch, const double *params, const int32_t *nmax,
+                 const float *coef, const float *dcoef, const float *coords,
+                 int32_t flags, const float *data, void *scratch,
+                 int32_t options, lmz_schedule *chosen, hipStream_t stream);
 
 /* Normal equations from a Jacobian and residual already in HBM:
  * out double [nfeat][36] (J^T J upper triangle, J^T r, r^T r), with J

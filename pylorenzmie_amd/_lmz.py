@@ -21,6 +21,7 @@ NJAC = 7
 NRED = 36
 NRES = 40
 TILE = 1024
+TUNE_FORCE = 1
 CARTESIAN = 1
 BOHREN = 2
 ACCUMULATE = 4
@@ -62,6 +63,17 @@ class BatchSizes(ctypes.Structure):
                 ('dcoef_stride', ctypes.c_int32)]
 
 
+class Schedule(ctypes.Structure):
+    """lmz_schedule, include/lmz.h."""
+    _fields_ = [('jac_threads', ctypes.c_int32),
+                ('fwd_threads', ctypes.c_int32),
+                ('jac_blocks_per_cu', ctypes.c_int32),
+                ('fwd_blocks_per_cu', ctypes.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class LMSettings(ctypes.Structure):
     """lmz_lm_settings, include/lmz.h."""
     _fields_ = [('max_iter', ctypes.c_int32),
@@ -91,6 +103,11 @@ SIGNATURES = {
     'lmz_jacobian': (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P,
                           _P, _P]),
     'lmz_hologram_vjp': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
+    'lmz_batch_set_schedule': (_I, [_P, ctypes.POINTER(Schedule)]),
+    'lmz_batch_get_schedule': (_I, [_P, ctypes.POINTER(Schedule)]),
+    'lmz_autotune_scratch_bytes': (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    'lmz_autotune': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _I,
+                          ctypes.POINTER(Schedule), _P]),
     'lmz_normal_eq': (_I, [_P, _P, _P, _P, _P]),
     'lmz_lm_workspace_bytes': (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     'lmz_lm_plan_create': (_I, [_P, ctypes.POINTER(_P)]),
@@ -207,6 +224,35 @@ class Batch:
         n = ctypes.c_size_t()
         check(self._lib.lmz_lm_workspace_bytes(self.handle, ctypes.byref(n)))
         return n.value
+
+    @property
+    def schedule(self) -> Schedule:
+        sc = Schedule()
+        check(self._lib.lmz_batch_get_schedule(self.handle, ctypes.byref(sc)))
+        return sc
+
+    @schedule.setter
+    def schedule(self, sc) -> None:
+        if isinstance(sc, dict):
+            sc = Schedule(**sc)
+        check(self._lib.lmz_batch_set_schedule(self.handle, ctypes.byref(sc)))
+
+    def autotune(self, params, nmax, coef, dcoef, data, coords=None,
+                 flags: int = CARTESIAN | BOHREN, force: bool = False,
+                 stream=None) -> Schedule:
+        """Pick the pixel-kernel launch shapes by timing them on this
+        batch (lmz_autotune); device tensors as for lmz_jacobian."""
+        t = torch()
+        n = ctypes.c_size_t()
+        check(self._lib.lmz_autotune_scratch_bytes(self.handle, ctypes.byref(n)))
+        buf = t.empty(n.value + 256, dtype=t.uint8, device=params.device)
+        base = (buf.data_ptr() + 255) & ~255
+        sc = Schedule()
+        check(self._lib.lmz_autotune(self.handle, ptr(params), ptr(nmax),
+                                     ptr(coef), ptr(dcoef), ptr(coords), flags,
+                                     ptr(data), base, TUNE_FORCE if force else 0,
+                                     ctypes.byref(sc), stream_handle(stream)))
+        return sc
 
     def close(self):
         if getattr(self, 'handle', None):

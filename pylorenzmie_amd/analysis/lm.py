@@ -167,6 +167,24 @@ class LMBatch:
         if int(self.fixed_nmax[0::2].min()) < 1:
             raise ValueError('invalid layer table')
 
+    def autotune(self, force: bool = False):
+        """Time the schedules of the pixel kernels on this batch at the
+        loaded parameters and keep the fastest (lmz_autotune); returns it
+        as a dict."""
+        t = _lmz.torch()
+        b = self.batch
+        dev = self.params.device
+        nmax = t.zeros(2 * self.nfeat, dtype=t.int32, device=dev)
+        coef = t.zeros(b.coef_floats(), dtype=t.float32, device=dev)
+        dcoef = t.zeros(b.dcoef_floats(), dtype=t.float32, device=dev)
+        work = t.zeros(b.work_doubles, dtype=t.float64, device=dev)
+        _lmz.check(_lmz.load_library().lmz_mie_ab(
+            b.handle, _lmz.ptr(self.params), None, _lmz.ptr(nmax),
+            _lmz.ptr(coef), _lmz.ptr(dcoef), None, _lmz.ptr(work),
+            _lmz.stream_handle()))
+        return b.autotune(self.params, nmax, coef, dcoef, self.data,
+                          self.coords, self.flags, force).as_dict()
+
     def run(self, settings: LMSettings, vary, stream=None) -> None:
         """Fit on the device, stream-ordered; results stay in HBM."""
         s = self.lm_settings(settings, vary)

@@ -3,8 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "lmz_device.h"
@@ -142,8 +146,9 @@ __device__ __forceinline__ int next_tile(int32_t *count, int &slot, int &k) {
 // (the LM's compacted list of active tiles, length count[0]) the grid is
 // persistent and blocks fetch list entries dynamically (next_tile) until
 // the list is exhausted.
-template <int MODE>
-__global__ __launch_bounds__(LMZ_THREADS) void k_forward(
+// NT threads share a tile, LMZ_TILE / NT pixels each (the schedule).
+template <int MODE, int NT = LMZ_THREADS>
+__global__ __launch_bounds__(NT) void k_forward(
     const lmz_feature *__restrict__ feat,
     const int32_t *__restrict__ tile_feature,
     const double *__restrict__ params, const int32_t *__restrict__ sel,
@@ -175,8 +180,8 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_forward(
     float *outf = out ? out + b * out_stride : nullptr;
     double chi = 0.0;
 #pragma unroll 1
-    for (int q = 0; q < LMZ_PPT; ++q) {
-        const int j = (t - fu.tile0) * LMZ_TILE + q * LMZ_THREADS + threadIdx.x;
+    for (int q = 0; q < LMZ_TILE / NT; ++q) {
+        const int j = (t - fu.tile0) * LMZ_TILE + q * NT + threadIdx.x;
         if (j >= fu.npix) break;
         float ax, ay, az;
         pixel_coords(fu, j, coords, npix_total, ax, ay, az);
@@ -207,15 +212,16 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_forward(
     }
     if (MODE == 2 && chi2_tiles) {
         double v[1] = {chi};
-        block_reduce_store_d<1>(v, chi2_tiles + t);
+        block_reduce_store_d<1, NT>(v, chi2_tiles + t);
         __syncthreads();   // the reduction buffer is reused by the next tile
     }
   }
 }
 
 // Forward-mode Jacobian over one tile; optional J, residual, and per-tile
-// normal-equation partials.
-__global__ __launch_bounds__(LMZ_THREADS) void k_jacobian(
+// normal-equation partials.  NT threads per tile as k_forward.
+template <int NT = LMZ_THREADS>
+__global__ __launch_bounds__(NT) void k_jacobian(
     const lmz_feature *__restrict__ feat,
     const int32_t *__restrict__ tile_feature,
     const double *__restrict__ params, const int32_t *__restrict__ sel,
@@ -251,8 +257,8 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_jacobian(
 #pragma unroll
     for (int i = 0; i < LMZ_NRED; ++i) acc[i] = 0.f;
 #pragma unroll 1
-    for (int q = 0; q < LMZ_PPT; ++q) {
-        const int j = (t - fu.tile0) * LMZ_TILE + q * LMZ_THREADS + threadIdx.x;
+    for (int q = 0; q < LMZ_TILE / NT; ++q) {
+        const int j = (t - fu.tile0) * LMZ_TILE + q * NT + threadIdx.x;
         if (j >= fu.npix) break;
         float ax, ay, az;
         pixel_coords(fu, j, coords, npix_total, ax, ay, az);
@@ -286,7 +292,7 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_jacobian(
         }
     }
     if (partials && data) {
-        block_reduce_store<LMZ_NRED>(acc, partials + (long long)LMZ_NRED * t);
+        block_reduce_store<LMZ_NRED, NT>(acc, partials + (long long)LMZ_NRED * t);
         __syncthreads();   // the reduction buffer is reused by the next tile
     }
   }
@@ -413,6 +419,35 @@ void launch_mie(const lmz_batch *b, const double *params, const int32_t *sel,
                        coef, dcoef, ab_out, work, active);
 }
 
+// Pixel-kernel launches at the batch's threads per tile (64, 128, 256).
+template <int MODE, typename... Args>
+static void launch_forward_nt(int nt, int grid, hipStream_t stream,
+                              Args... args) {
+    if (nt == 64)
+        hipLaunchKernelGGL((k_forward<MODE, 64>), dim3(grid), dim3(64), 0,
+                           stream, args...);
+    else if (nt == 128)
+        hipLaunchKernelGGL((k_forward<MODE, 128>), dim3(grid), dim3(128), 0,
+                           stream, args...);
+    else
+        hipLaunchKernelGGL((k_forward<MODE, 256>), dim3(grid), dim3(256), 0,
+                           stream, args...);
+}
+
+template <typename... Args>
+static void launch_jacobian_nt(int nt, int grid, hipStream_t stream,
+                               Args... args) {
+    if (nt == 64)
+        hipLaunchKernelGGL((k_jacobian<64>), dim3(grid), dim3(64), 0, stream,
+                           args...);
+    else if (nt == 128)
+        hipLaunchKernelGGL((k_jacobian<128>), dim3(grid), dim3(128), 0,
+                           stream, args...);
+    else
+        hipLaunchKernelGGL((k_jacobian<256>), dim3(grid), dim3(256), 0,
+                           stream, args...);
+}
+
 void launch_residual(const lmz_batch *b, const double *params,
                      const int32_t *sel, int flip, const int32_t *nmax,
                      const float *coef, const float *coords, int flags,
@@ -420,11 +455,12 @@ void launch_residual(const lmz_batch *b, const double *params,
                      double *chi2_tiles, const uint8_t *active,
                      const TileList *tl, hipStream_t stream) {
     const int grid = tl ? tl->grid : b->ntiles;
-    hipLaunchKernelGGL(k_forward<2>, dim3(grid), dim3(LMZ_THREADS), 0,
-                       stream, b->d_feat, b->d_tile_feature, params, sel, flip,
-                       b->coef_buffers, nmax, coef, coords, b->npix_total,
-                       flags, data, out, out_stride, nullptr, chi2_tiles,
-                       active, tl ? tl->list : nullptr, tl ? tl->count : nullptr);
+    launch_forward_nt<2>(b->sched.fwd_threads, grid, stream, b->d_feat,
+                         b->d_tile_feature, params, sel, flip, b->coef_buffers,
+                         nmax, coef, coords, b->npix_total, flags, data, out,
+                         out_stride, (cf *)nullptr, chi2_tiles, active,
+                         tl ? (const int32_t *)tl->list : nullptr,
+                         tl ? tl->count : nullptr);
 }
 
 void launch_jacobian(const lmz_batch *b, const double *params,
@@ -435,11 +471,11 @@ void launch_jacobian(const lmz_batch *b, const double *params,
                      float *resid, double *partials, const uint8_t *active,
                      const TileList *tl, hipStream_t stream) {
     const int grid = tl ? tl->grid : b->ntiles;
-    hipLaunchKernelGGL(k_jacobian, dim3(grid), dim3(LMZ_THREADS), 0,
-                       stream, b->d_feat, b->d_tile_feature, params, sel, flip,
-                       b->coef_buffers, nmax, coef, dcoef, coords,
-                       b->npix_total, flags, data, rin, rin_stride, jac, resid,
-                       partials, active, tl ? tl->list : nullptr,
+    launch_jacobian_nt(b->sched.jac_threads, grid, stream, b->d_feat,
+                       b->d_tile_feature, params, sel, flip, b->coef_buffers,
+                       nmax, coef, dcoef, coords, b->npix_total, flags, data,
+                       rin, rin_stride, jac, resid, partials, active,
+                       tl ? (const int32_t *)tl->list : nullptr,
                        tl ? tl->count : nullptr);
 }
 
@@ -480,19 +516,77 @@ void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
                        b->nfeat, mask, tl.list, tl.count);
 }
 
-// Persistent grid sizes: every block resident at once on this device.
-int persistent_grids(int *grid_jacobian, int *grid_forward) {
-    int dev = 0, cus = 0, nj = 0, nf = 0;
+// Workgroups of one pixel kernel resident per CU at once.
+static int occupancy(bool jacobian, int nt) {
+    const void *fn;
+    if (jacobian)
+        fn = nt == 64 ? reinterpret_cast<const void *>(&k_jacobian<64>)
+           : nt == 128 ? reinterpret_cast<const void *>(&k_jacobian<128>)
+                       : reinterpret_cast<const void *>(&k_jacobian<256>);
+    else
+        fn = nt == 64 ? reinterpret_cast<const void *>(&k_forward<2, 64>)
+           : nt == 128 ? reinterpret_cast<const void *>(&k_forward<2, 128>)
+                       : reinterpret_cast<const void *>(&k_forward<2, 256>);
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, nt, 0) != hipSuccess)
+        n = 1;
+    return n > 0 ? n : 1;
+}
+
+static int device_cus(int *cus) {
+    int dev = 0;
     LMZ_HIP(hipGetDevice(&dev));
-    LMZ_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    LMZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &nj, reinterpret_cast<const void *>(&k_jacobian), LMZ_THREADS, 0));
-    LMZ_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &nf, reinterpret_cast<const void *>(&k_forward<2>), LMZ_THREADS, 0));
-    *grid_jacobian = cus * (nj > 0 ? nj : 1);
-    *grid_forward = cus * (nf > 0 ? nf : 1);
+    LMZ_HIP(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
     return LMZ_OK;
 }
+
+// Persistent grid sizes: the schedule's workgroups per CU, by default as
+// many as are resident at once.
+int persistent_grids(const lmz_batch *b, int *grid_jacobian, int *grid_forward) {
+    int cus = 0;
+    if (int rc = device_cus(&cus)) return rc;
+    const lmz_schedule &sc = b->sched;
+    const int bj = sc.jac_blocks_per_cu > 0 ? sc.jac_blocks_per_cu
+                                            : occupancy(true, sc.jac_threads);
+    const int bf = sc.fwd_blocks_per_cu > 0 ? sc.fwd_blocks_per_cu
+                                            : occupancy(false, sc.fwd_threads);
+    *grid_jacobian = cus * bj;
+    *grid_forward = cus * bf;
+    return LMZ_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+// mask[f] = 1 for every stride-th feature, else 0
+__global__ void k_mask_every(uint8_t *__restrict__ mask, int n, int stride) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) mask[i] = (i % stride) == 0 ? 1 : 0;
+}
+
+struct TuneScratch {
+    size_t mask, list, count, partials, chi2, total;
+};
+
+static TuneScratch tune_layout(const lmz_batch *b) {
+    TuneScratch L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 255) & ~size_t(255);
+        return o;
+    };
+    L.mask = take((size_t)b->nfeat);
+    L.list = take(sizeof(int32_t) * (size_t)b->ntiles);
+    L.count = take(sizeof(int32_t) * 16);
+    L.partials = take(sizeof(double) * LMZ_NRED * (size_t)b->ntiles);
+    L.chi2 = take(sizeof(double) * (size_t)b->ntiles);
+    L.total = off;
+    return L;
+}
+
+// Schedules already tuned in this process, by device and batch shape.
+using TuneKey = std::tuple<int, int, int, int, int>;
+static std::mutex g_tune_mutex;
+static std::map<TuneKey, lmz_schedule> g_tune_cache;
 
 }  // namespace lmz
 
@@ -655,12 +749,13 @@ int lmz_field(const lmz_batch *b, const double *params, const int32_t *sel,
               int32_t flags, float *field, hipStream_t stream) {
     if (int rc = forward_args(b, params, nmax, coef, coords, sel)) return rc;
     LMZ_REQUIRE(field != nullptr, "field is NULL");
-    hipLaunchKernelGGL(k_forward<0>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
-                       stream, b->d_feat, b->d_tile_feature, params, sel, 0,
-                       b->coef_buffers, nmax, coef, coords, b->npix_total,
-                       flags, nullptr, nullptr, 0LL,
-                       reinterpret_cast<cf *>(field), nullptr, nullptr,
-                       nullptr, nullptr);
+    launch_forward_nt<0>(b->sched.fwd_threads, b->ntiles, stream, b->d_feat,
+                         b->d_tile_feature, params, sel, 0, b->coef_buffers,
+                         nmax, coef, coords, b->npix_total, flags,
+                         (const float *)nullptr, (float *)nullptr, 0LL,
+                         reinterpret_cast<cf *>(field), (double *)nullptr,
+                         (const uint8_t *)nullptr, (const int32_t *)nullptr,
+                         (int32_t *)nullptr);
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
 }
@@ -673,18 +768,16 @@ int lmz_hologram(const lmz_batch *b, const double *params, const int32_t *sel,
     if (int rc = forward_args(b, params, nmax, coef, coords, sel)) return rc;
     if (data == nullptr) {
         LMZ_REQUIRE(out != nullptr, "out is NULL");
-        hipLaunchKernelGGL(k_forward<1>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
-                           stream, b->d_feat, b->d_tile_feature, params, sel, 0,
-                           b->coef_buffers, nmax, coef, coords, b->npix_total,
-                           flags, nullptr, out, 0LL, nullptr, nullptr, active,
-                           nullptr, nullptr);
+        launch_forward_nt<1>(b->sched.fwd_threads, b->ntiles, stream, b->d_feat,
+                             b->d_tile_feature, params, sel, 0, b->coef_buffers,
+                             nmax, coef, coords, b->npix_total, flags,
+                             (const float *)nullptr, out, 0LL, (cf *)nullptr,
+                             (double *)nullptr, active, (const int32_t *)nullptr,
+                             (int32_t *)nullptr);
     } else {
         LMZ_REQUIRE(out || chi2_tiles, "one of out, chi2_tiles is required");
-        hipLaunchKernelGGL(k_forward<2>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
-                           stream, b->d_feat, b->d_tile_feature, params, sel, 0,
-                           b->coef_buffers, nmax, coef, coords, b->npix_total,
-                           flags, data, out, 0LL, nullptr, chi2_tiles, active,
-                           nullptr, nullptr);
+        launch_residual(b, params, sel, 0, nmax, coef, coords, flags, data, out,
+                        0LL, chi2_tiles, active, nullptr, stream);
     }
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
@@ -714,11 +807,8 @@ int lmz_jacobian(const lmz_batch *b, const double *params, const int32_t *sel,
     LMZ_REQUIRE(jac || ((resid || partials) && data),
                 "nothing to compute: jac is NULL and no data for resid/partials");
     LMZ_REQUIRE(data || (!resid && !partials), "resid and partials need data");
-    hipLaunchKernelGGL(k_jacobian, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
-                       stream, b->d_feat, b->d_tile_feature, params, sel, 0,
-                       b->coef_buffers, nmax, coef, dcoef, coords,
-                       b->npix_total, flags, data, nullptr, 0LL, jac, resid,
-                       partials, active, nullptr, nullptr);
+    launch_jacobian(b, params, sel, 0, nmax, coef, dcoef, coords, flags, data,
+                    nullptr, 0LL, jac, resid, partials, active, nullptr, stream);
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
 }
@@ -745,6 +835,152 @@ int lmz_hologram_vjp(const lmz_batch *b, const double *params,
                        dim3(LMZ_THREADS), 0, stream, b->d_feat, b->nfeat,
                        tile_work, grad_params);
     LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+int lmz_batch_set_schedule(lmz_batch *b, const lmz_schedule *sc) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(sc != nullptr, "schedule is NULL");
+    auto ok_nt = [](int nt) { return nt == 64 || nt == 128 || nt == 256; };
+    LMZ_REQUIRE(ok_nt(sc->jac_threads) && ok_nt(sc->fwd_threads),
+                "threads per tile must be 64, 128 or 256");
+    LMZ_REQUIRE(sc->jac_blocks_per_cu >= 0 && sc->jac_blocks_per_cu <= 64 &&
+                sc->fwd_blocks_per_cu >= 0 && sc->fwd_blocks_per_cu <= 64,
+                "blocks per CU must be in 0..64");
+    b->sched = *sc;
+    return LMZ_OK;
+}
+
+int lmz_batch_get_schedule(const lmz_batch *b, lmz_schedule *sc) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(sc != nullptr, "schedule is NULL");
+    *sc = b->sched;
+    return LMZ_OK;
+}
+
+int lmz_autotune_scratch_bytes(const lmz_batch *b, size_t *bytes) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(bytes != nullptr, "bytes is NULL");
+    *bytes = tune_layout(b).total;
+    return LMZ_OK;
+}
+
+int lmz_autotune(lmz_batch *b, const double *params, const int32_t *nmax,
+                 const float *coef, const float *dcoef, const float *coords,
+                 int32_t flags, const float *data, void *scratch,
+                 int32_t options, lmz_schedule *chosen, hipStream_t stream) {
+    if (int rc = forward_args(b, params, nmax, coef, coords, nullptr)) return rc;
+    LMZ_REQUIRE(dcoef && data && scratch && chosen,
+                "dcoef, data, scratch and chosen are required");
+    LMZ_REQUIRE((reinterpret_cast<uintptr_t>(scratch) & 255) == 0,
+                "scratch must be 256-byte aligned");
+    LMZ_REQUIRE(flags & LMZ_CARTESIAN, "the LM fits the Cartesian hologram");
+    int dev = 0, cus = 0;
+    LMZ_HIP(hipGetDevice(&dev));
+    if (int rc = device_cus(&cus)) return rc;
+    // the cache key: device, tiles (log2), mean order count (/8), layout
+    std::vector<int32_t> hn((size_t)b->nfeat * b->coef_buffers);
+    LMZ_HIP(hipMemcpyAsync(hn.data(), nmax, sizeof(int32_t) * hn.size(),
+                           hipMemcpyDeviceToHost, stream));
+    LMZ_HIP(hipStreamSynchronize(stream));
+    double nsum = 0.0;
+    for (int f = 0; f < b->nfeat; ++f) nsum += hn[(size_t)f * b->coef_buffers];
+    const TuneKey key{dev, (int)std::lround(std::log2((double)b->ntiles)),
+                      (int)std::lround(nsum / b->nfeat / 8.0), b->all_dense,
+                      flags};
+    if (!(options & LMZ_TUNE_FORCE)) {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            b->sched = it->second;
+            *chosen = it->second;
+            return LMZ_OK;
+        }
+    }
+    const TuneScratch L = tune_layout(b);
+    char *ws = static_cast<char *>(scratch);
+    uint8_t *mask = reinterpret_cast<uint8_t *>(ws + L.mask);
+    TileList tl{reinterpret_cast<int32_t *>(ws + L.list),
+                reinterpret_cast<int32_t *>(ws + L.count), 0};
+    double *partials = reinterpret_cast<double *>(ws + L.partials);
+    double *chi2 = reinterpret_cast<double *>(ws + L.chi2);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    LMZ_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(hipErrorOutOfMemory, "hipEventCreate");
+    }
+    const int reps = 3;
+    const lmz_schedule saved = b->sched;
+    lmz_schedule best = saved;
+    hipError_t err = hipSuccess;
+    for (int kind = 0; kind < 2 && err == hipSuccess; ++kind) {
+        float best_ms = 1e30f;
+        for (int nt : {64, 128, 256}) {
+            const int occ = occupancy(kind == 0, nt);
+            int last = -1;
+            for (int mult : {1, 2, 4}) {
+                const int bpc = std::min(occ * mult, 64);
+                if (bpc == last) continue;
+                last = bpc;
+                tl.grid = cus * bpc;
+                lmz_schedule sc = b->sched;
+                if (kind == 0) { sc.jac_threads = nt; sc.jac_blocks_per_cu = bpc; }
+                else { sc.fwd_threads = nt; sc.fwd_blocks_per_cu = bpc; }
+                b->sched = sc;
+                // the LM launches over every tile in its first iterations and
+                // over a thinning set of features later: time both, the
+                // sparse one (every 8th feature) weighted to equal work
+                float total = 0.f;
+                for (int r = 0; r <= 2 * reps && err == hipSuccess; ++r) {
+                    const int stride = (r & 1) ? 8 : 1;
+                    hipLaunchKernelGGL(k_mask_every, dim3((b->nfeat + 255) / 256),
+                                       dim3(256), 0, stream, mask, b->nfeat, stride);
+                    launch_tile_list(b, mask, tl, stream);
+                    (void)hipEventRecord(e0, stream);
+                    if (kind == 0)
+                        launch_jacobian(b, params, nullptr, 0, nmax, coef, dcoef,
+                                        coords, flags, data, nullptr, 0LL,
+                                        nullptr, nullptr, partials, nullptr,
+                                        &tl, stream);
+                    else
+                        launch_residual(b, params, nullptr, 0, nmax, coef, coords,
+                                        flags, data, nullptr, 0LL, chi2, nullptr,
+                                        &tl, stream);
+                    (void)hipEventRecord(e1, stream);
+                    err = hipEventSynchronize(e1);
+                    float ms = 0.f;
+                    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+                    if (r > 0) total += ms * stride;   // the first launch warms up
+                }
+                if (err == hipSuccess && total / reps < best_ms) {
+                    best_ms = total / reps;
+                    if (kind == 0) {
+                        best.jac_threads = nt;
+                        best.jac_blocks_per_cu = bpc;
+                    } else {
+                        best.fwd_threads = nt;
+                        best.fwd_blocks_per_cu = bpc;
+                    }
+                }
+            }
+        }
+        b->sched = saved;
+        if (kind == 0) { b->sched.jac_threads = best.jac_threads;
+                         b->sched.jac_blocks_per_cu = best.jac_blocks_per_cu; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err != hipSuccess) {
+        b->sched = saved;
+        return hip_fail(err, "lmz_autotune");
+    }
+    b->sched = best;
+    *chosen = best;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tune_cache[key] = best;
+    }
     return LMZ_OK;
 }
 

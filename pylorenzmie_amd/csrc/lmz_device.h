@@ -20,6 +20,7 @@ struct lmz_batch {
     int32_t coef_slots = 0;
     int32_t cap_max = 0;
     int32_t all_dense = 1;
+    lmz_schedule sched{LMZ_THREADS, LMZ_THREADS, 0, 0};   // launch shapes
     lmz_feature *d_feat = nullptr;      // device copy of the feature table
     int32_t *d_tile_feature = nullptr;  // tile -> feature
 };
@@ -58,7 +59,8 @@ void launch_jacobian(const lmz_batch *b, const double *params,
                      const TileList *tl, hipStream_t stream);
 void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
                       const TileList &tl, hipStream_t stream);
-int persistent_grids(int *grid_jacobian, int *grid_forward);
+// Persistent grids of the LM's pixel launches under the batch's schedule.
+int persistent_grids(const lmz_batch *b, int *grid_jacobian, int *grid_forward);
 int hip_fail(hipError_t e, const char *what);
 int check_batch(const lmz_batch *b);
 
@@ -99,10 +101,10 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 
 // Block reduction of NV per-thread floats to NV doubles, written to
 // out[0..NV) by the first NV threads.  Fixed order: deterministic.
-template <int NV>
+template <int NV, int NT = LMZ_THREADS>
 __device__ __forceinline__ void block_reduce_store(float (&v)[NV],
                                                    double *__restrict__ out) {
-    __shared__ double red[LMZ_THREADS / 64][NV];
+    __shared__ double red[NT / 64][NV];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
 #pragma unroll
@@ -114,15 +116,15 @@ __device__ __forceinline__ void block_reduce_store(float (&v)[NV],
     if (threadIdx.x < NV) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < LMZ_THREADS / 64; ++w) s += red[w][threadIdx.x];
+        for (int w = 0; w < NT / 64; ++w) s += red[w][threadIdx.x];
         out[threadIdx.x] = s;
     }
 }
 
-template <int NV>
+template <int NV, int NT = LMZ_THREADS>
 __device__ __forceinline__ void block_reduce_store_d(double (&v)[NV],
                                                      double *__restrict__ out) {
-    __shared__ double red[LMZ_THREADS / 64][NV];
+    __shared__ double red[NT / 64][NV];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
 #pragma unroll
@@ -134,7 +136,7 @@ __device__ __forceinline__ void block_reduce_store_d(double (&v)[NV],
     if (threadIdx.x < NV) {
         double s = 0.0;
 #pragma unroll
-        for (int w = 0; w < LMZ_THREADS / 64; ++w) s += red[w][threadIdx.x];
+        for (int w = 0; w < NT / 64; ++w) s += red[w][threadIdx.x];
         out[threadIdx.x] = s;
     }
 }

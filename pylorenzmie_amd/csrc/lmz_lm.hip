@@ -342,6 +342,7 @@ struct lmz_lm_plan {
     int32_t key_flags = -1;
     int key_fixed = -1;
     lmz_lm_settings key_set{};
+    lmz_schedule key_sched{};
 };
 
 static void plan_release_graph(lmz_lm_plan *p) {
@@ -436,10 +437,6 @@ int lmz_lm_plan_create(const lmz_batch *b, lmz_lm_plan **out) {
     *out = nullptr;
     lmz_lm_plan *p = new lmz_lm_plan;
     p->batch = b;
-    if (int rc = persistent_grids(&p->grid_j, &p->grid_f)) {
-        delete p;
-        return rc;
-    }
     hipError_t e = hipStreamCreateWithFlags(&p->s, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming);
@@ -495,6 +492,7 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
     LMZ_REQUIRE((reinterpret_cast<uintptr_t>(workspace) & 255) == 0,
                 "workspace must be 256-byte aligned");
 
+    if (int rc = persistent_grids(b, &plan->grid_j, &plan->grid_f)) return rc;
     const LMLayout L = lm_layout(b);
     char *ws = static_cast<char *>(workspace);
     LMState *st = reinterpret_cast<LMState *>(ws + L.state);
@@ -538,6 +536,8 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
         const bool same = plan->exec && std::memcmp(key, plan->key_ptrs, sizeof key) == 0 &&
                           plan->key_flags == flags &&
                           plan->key_fixed == (int)fixed_ab &&
+                          std::memcmp(&b->sched, &plan->key_sched,
+                                      sizeof b->sched) == 0 &&
                           std::memcmp(&set, &plan->key_set, sizeof set) == 0;
         if (!same) {
             plan_release_graph(plan);
@@ -550,6 +550,7 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
             std::memcpy(plan->key_ptrs, key, sizeof key);
             plan->key_flags = flags;
             plan->key_fixed = (int)fixed_ab;
+            plan->key_sched = b->sched;
             plan->key_set = set;
         }
     }

@@ -41,6 +41,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lmz.Feature) == 88
     assert ctypes.sizeof(_lmz.BatchSizes) == 48
     assert ctypes.sizeof(_lmz.LMSettings) == 48
+    assert ctypes.sizeof(_lmz.Schedule) == 16
 
 
 def _feature(**kw):
@@ -79,3 +80,7 @@ def test_calls_reject_null_batch(lib):
     assert b'batch is NULL' in lib.lmz_last_error()
     assert lib.lmz_hologram(None, *([None] * 5), 3, *([None] * 5)) == 1
     assert lib.lmz_lm_run(None, None, 3, None, None, None, None, None, None) == 1
+    assert lib.lmz_batch_set_schedule(None, None) == 1
+    assert lib.lmz_autotune(None, *([None] * 5), 3, None, None, 0, None, None) == 1
+    assert lib.lmz_mie_ab_layered(None, *([None] * 7)) == 1
+    assert lib.lmz_hologram_vjp(None, *([None] * 5), 3, *([None] * 4)) == 1

@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define LMZ_VERSION 100
+#define LMZ_VERSION 101
 #define LMZ_NPARAM 8       /* stride of params rows */
 #define LMZ_NJAC 7         /* x_p y_p z_p a_p n_p k_p alpha */
 #define LMZ_NRED 36        /* 28 + 7 + 1 */

@@ -33,7 +33,7 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_version(lib):
-    assert lib.lmz_version() == 100
+    assert lib.lmz_version() == 101
 
 
 def test_struct_layouts_match_header():

@@ -252,15 +252,14 @@ LMZ_HD double row_d(const float *row, int i) {
 // recurrence becomes, exactly,
 //   Td = (d_n - d_{n-1}) + w pi_n,   tau_n = n Td - (a_n + d_{n-1}),
 //   d_{n+1} = d_n + w pi_n + (1 + 1/n) Td,   pi_{n+1} = a_{n+1} - d_{n+1},
-// whose rounding errors scale with d_n, not a_n.  It is carried in FP64
-// (the same VALU rate as unpacked FP32 on gfx950); the pixel sum uses the
-// FP32 roundings of pi_n and tau_n.
+// whose rounding errors scale with d_n, not a_n.  The forward model carries
+// it in FP64 (Angular); the Jacobian in FP32 (Angular32).
 struct Angular {
     double dm, dn, pn;
     LMZ_HD void init() { dm = 0.0; dn = 0.0; pn = 1.0; }
     // returns pi_n, tau_n of the current order and advances to n + 1
     LMZ_HD void step_d(double w, const float *row, double &pi, double &tau) {
-        const double fn = (double)row[6];
+        const double fn = row_d(row, 14);
         const double c1 = row_d(row, 8), an = row_d(row, 10), an1 = row_d(row, 12);
         const double wp = w * pn;
         const double Td = (dn - dm) + wp;
@@ -271,14 +270,23 @@ struct Angular {
         dn = dnext;
         pn = an1 - dnext;
     }
-    LMZ_HD void step(double w, const float *row, float &pi, float &tau) {
-        const double fn = (double)row[6];
-        const double c1 = row_d(row, 8), an = row_d(row, 10), an1 = row_d(row, 12);
-        const double wp = w * pn;
-        const double Td = (dn - dm) + wp;
-        pi = (float)pn;
-        tau = (float)fma(fn, Td, -(an + dm));
-        const double dnext = fma(c1, Td, dn + wp);
+};
+
+// The same recurrence in FP32 for the Jacobian: its errors scale with d_n
+// near the axis, and far from it (d_n ~ a_n) with a_n / pi_n ~ n, which
+// keeps the Jacobian's columns within 2e-5 of their scale to n ~ 280.
+// a_n = n(n+1)/2 and a_{n+1} = a_n + n + 1 are exact in FP32 for n < 4096.
+struct Angular32 {
+    float dm, dn, pn;
+    LMZ_HD void init() { dm = 0.f; dn = 0.f; pn = 1.f; }
+    LMZ_HD void step(float w, const float *row, float &pi, float &tau) {
+        const float fn = row[6], c1 = row[28];
+        const float an = 0.5f * row[7], an1 = an + fn + 1.f;
+        const float wp = w * pn;
+        const float Td = (dn - dm) + wp;
+        pi = pn;
+        tau = fmaf(fn, Td, -(an + dm));
+        const float dnext = fmaf(c1, Td, dn + wp);
         dm = dn;
         dn = dnext;
         pn = an1 - dnext;
@@ -364,6 +372,7 @@ LMZ_HD SumsD forward_sum(const PixGeomD &g, const float *coef, int nmax) {
     ang.init();
     cd x2 = g.xi_m1, x1 = g.xi_0;
     const double invR = g.invRl, w = g.w;
+#pragma unroll 2
     for (int n = 1; n <= nmax; ++n) {
         const float *cp = coef + LMZ_COEF_STRIDE * (n - 1);
         const cd PR{row_d(cp, 16), row_d(cp, 18)};
@@ -449,9 +458,9 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
     cf Tr[2], Tt[2], Tp[2];    // coefficient tangents: a_p, n_p
 #pragma unroll
     for (int q = 0; q < 2; ++q) Tr[q] = Tt[q] = Tp[q] = cf{0.f, 0.f};
-    Angular ang;
+    Angular32 ang;
     ang.init();
-    const double w = (double)g.w;
+    const float w = g.w;
     float dpm = 0.f, dpn = 0.f;   // d pi_{n-1} / dc, d pi_n / dc
     cf x2 = g.xi_m1, x1 = g.xi_0;
     const float cl = g.cl, invRl = g.invRl, invR2 = g.invR * g.invR;
@@ -459,9 +468,9 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
     for (int n = 1; n <= nmax; ++n) {
         const float *cp = coef + LMZ_COEF_STRIDE * (n - 1);
         const float *dp = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
-        const cf PR{cp[0], cp[1]}, P{cp[2], cp[3]}, Q{cp[4], cp[5]};
+        const cf P{cp[2], cp[3]}, Q{cp[4], cp[5]};
         const float fn = cp[6], nn1 = cp[7];
-        const float c1 = (float)row_d(cp, 8);
+        const float c1 = cp[28];
         float pn, tau;
         ang.step(w, cp, pn, tau);
         // tangent of Wiscombe's recurrence with respect to c
@@ -473,23 +482,25 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
         const cf xi{fmaf(cx, x1.re, -x2.re), fmaf(cx, x1.im, -x2.im)};
         const cf D{fmaf(ninv, xi.re, -x1.re), fmaf(ninv, xi.im, -x1.im)};
         const float gn = fmaf(-nn1, invR2, 1.f);
-        const cf PD = cmul(P, D), QX = cmul(Q, xi), PRX = cmul(PR, xi);
-        const cf PX = cmul(P, xi), QD = cmul(Q, D), PRD = cmul(PR, D);
+        // PR = n(n+1) P: the S_r terms use P xi and P D with pi_n n(n+1)
+        const cf PD = cmul(P, D), QX = cmul(Q, xi);
+        const cf PX = cmul(P, xi), QD = cmul(Q, D);
+        const float pnn = pn * nn1;
         // value
-        cmac(Sr, pn, PRX);
+        cmac(Sr, pnn, PX);
         cmac(St, tau, PD);
         cmac(St, -pn, QX);
         cmac(Sp, pn, PD);
         cmac(Sp, -tau, QX);
         // d/dR
-        cmac(SrR, -pn, PRD);
+        cmac(SrR, -pnn, PD);
         const float tg = tau * gn, pg = pn * gn;
         cmac(StR, tg, PX);
         cmac(StR, pn, QD);
         cmac(SpR, pg, PX);
         cmac(SpR, tau, QD);
         // d/dc
-        cmac(Src, dpn, PRX);
+        cmac(Src, dpn * nn1, PX);
         cmac(Stc, dtau, PD);
         cmac(Stc, -dpn, QX);
         cmac(Spc, dpn, PD);
@@ -635,7 +646,8 @@ LMZ_HD void write_order(int n, cd a, cd b, float *crow) {
     put_d(crow, 16, PR.re); put_d(crow, 18, PR.im);
     put_d(crow, 20, P.re);  put_d(crow, 22, P.im);
     put_d(crow, 24, Q.re);  put_d(crow, 26, Q.im);
-    for (int i = 28; i < LMZ_COEF_STRIDE; ++i) crow[i] = 0.f;
+    crow[28] = (float)(1.0 + 1.0 / n);
+    for (int i = 29; i < LMZ_COEF_STRIDE; ++i) crow[i] = 0.f;
 }
 LMZ_HD void write_dorder(int n, cd da, cd db, float *drow) {
     const double en = (2.0 * n + 1.0) / ((double)n * (n + 1.0));

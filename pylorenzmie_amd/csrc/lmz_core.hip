@@ -35,27 +35,38 @@ int check_batch(const lmz_batch *b) {
 // kernels
 // ====================================================================
 
-// One lane per feature: the FP64 recursion is sequential in n.
+// One wavefront per feature: lanes 0 and 1 run the two downward D1 chains,
+// lane 0 the upward chain, then all 64 lanes the per-order coefficients and
+// tangents (orders lane + 1, lane + 65, ...).  The scratch of feature f is
+// the contiguous block work + f * LMZ_WORK_PER_ORDER * (cap_max + 1).
 __global__ __launch_bounds__(64) void k_mie(
-    const lmz_feature *__restrict__ feat, int nfeat,
+    const lmz_feature *__restrict__ feat, int nfeat, int cap_max,
     const double *__restrict__ params, const int32_t *__restrict__ sel,
     int flip, int nbuf, int32_t *__restrict__ nmax_out,
     float *__restrict__ coef, float *__restrict__ dcoef,
     double *__restrict__ ab_out, double *__restrict__ work,
     const uint8_t *__restrict__ active) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.x;
     if (f >= nfeat) return;
     if (active && !active[f]) return;
+    const int lane = threadIdx.x;
     const lmz_feature F = feat[f];
     const double *p = params + (long long)LMZ_NPARAM * f;
     const int b = sel ? (sel[f] ^ flip) : 0;
     const long long slot0 = F.coef_offset + (long long)b * F.nmax_cap;
-    const int N = mie_feature(p[3], p[4], p[5], F.n_m, F.wavelength,
-                              F.nmax_cap, work + f, nfeat,
-                              coef ? coef + LMZ_COEF_STRIDE * slot0 : nullptr,
-                              dcoef ? dcoef + LMZ_DCOEF_STRIDE * slot0 : nullptr,
-                              ab_out ? ab_out + 4 * slot0 : nullptr);
-    nmax_out[(long long)f * nbuf + b] = N;
+    const MieSetup ms = mie_setup(p[3], p[4], p[5], F.n_m, F.wavelength,
+                                  F.nmax_cap);
+    double *w = work + (long long)f * LMZ_WORK_PER_ORDER * (cap_max + 1);
+    if (lane < 2)
+        mie_d1_down(lane ? cd{ms.x, 0.0} : ms.x * ms.m, ms.N, w, 1, 4 * lane);
+    __syncthreads();
+    if (lane == 0) mie_up_chain(ms.x, ms.N, w, 1);
+    __syncthreads();
+    for (int n = lane + 1; n <= ms.N; n += 64)
+        mie_order(ms, n, w, 1, coef ? coef + LMZ_COEF_STRIDE * slot0 : nullptr,
+                  dcoef ? dcoef + LMZ_DCOEF_STRIDE * slot0 : nullptr,
+                  ab_out ? ab_out + 4 * slot0 : nullptr);
+    if (lane == 0) nmax_out[(long long)f * nbuf + b] = ms.N;
 }
 
 // Layered spheres, one lane per feature (FP64, sequential in n).  A
@@ -413,10 +424,9 @@ void launch_mie(const lmz_batch *b, const double *params, const int32_t *sel,
                 int flip, int32_t *nmax, float *coef, float *dcoef,
                 double *ab_out, double *work, const uint8_t *active,
                 hipStream_t stream) {
-    const int grid = (b->nfeat + 63) / 64;
-    hipLaunchKernelGGL(k_mie, dim3(grid), dim3(64), 0, stream, b->d_feat,
-                       b->nfeat, params, sel, flip, b->coef_buffers, nmax,
-                       coef, dcoef, ab_out, work, active);
+    hipLaunchKernelGGL(k_mie, dim3(b->nfeat), dim3(64), 0, stream, b->d_feat,
+                       b->nfeat, b->cap_max, params, sel, flip, b->coef_buffers,
+                       nmax, coef, dcoef, ab_out, work, active);
 }
 
 // Pixel-kernel launches at the batch's threads per tile (64, 128, 256).
@@ -696,10 +706,8 @@ int lmz_mie_ab(const lmz_batch *b, const double *params, const int32_t *sel,
     LMZ_REQUIRE(coef || ab_out, "one of coef, ab_out is required");
     LMZ_REQUIRE(sel == nullptr || b->coef_buffers == 2,
                 "sel needs a batch with two coefficient buffers");
-    const int grid = (b->nfeat + 63) / 64;
-    hipLaunchKernelGGL(k_mie, dim3(grid), dim3(64), 0, stream, b->d_feat,
-                       b->nfeat, params, sel, 0, b->coef_buffers, nmax_out,
-                       coef, dcoef, ab_out, work, nullptr);
+    launch_mie(b, params, sel, 0, nmax_out, coef, dcoef, ab_out, work, nullptr,
+               stream);
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
 }

@@ -612,7 +612,7 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
 // Mie coefficients in FP64 with a_p, n_p, k_p tangents
 // ====================================================================
 // doubles of scratch per order and feature (lmz_batch_sizes.work_doubles)
-#define LMZ_WORK_PER_ORDER 8
+#define LMZ_WORK_PER_ORDER 14
 // Sphere.wiscombe_yang, theory/Sphere.py:91-123 (one layer: the rolled
 // term equals |x m|)
 LMZ_HD int wiscombe_yang(double x, cd m) {
@@ -664,46 +664,64 @@ LMZ_HD void write_dorder(int n, cd da, cd db, float *drow) {
 // (D1 downward from D1[nmax] = 0 for z = m x and z = x; Psi, zeta, Psi*zeta,
 // D3 upward; Eqs. 5 and 6), carried with tangents d/dx (real) and d/dm
 // (holomorphic), then mapped to a_p (x = k a_p), n_p and k_p
-// (m = (n_p + i k_p) / n_m).  work(n, j) = work[(n * LMZ_WORK_PER_ORDER + j)
-// * wstride].
-// Returns n_max (clamped to cap).
-LMZ_HD int mie_feature(double a_p, double n_p, double k_p, double n_m,
-                       double wavelength, int cap, double *work,
-                       long long wstride, float *coef, float *dcoef,
-                       double *ab) {
-    const double kum = 2.0 * M_PI / wavelength * n_m;   // rad / um in medium
-    const double x = kum * a_p;
-    const cd m{n_p / n_m, k_p / n_m};
-    int N = wiscombe_yang(x, m);
+// (m = (n_p + i k_p) / n_m).  It runs in three phases over a scratch table
+// W(n, j) = work[(n * LMZ_WORK_PER_ORDER + j) * wstride], n = 0..N:
+//   1. two independent downward chains, D1(m x) with dD1/dz into columns
+//      0-3 and D1(x) with dD1/dx into columns 4-7 (mie_d1_down);
+//   2. the upward chain Psi_n, zeta_n, Psi_n zeta_n, D3_n with their x
+//      derivatives into columns 8-13 (mie_up_chain);
+//   3. per order, independently, a_n, b_n and their tangents from the
+//      table (mie_order).
+// The kernel runs phase 1 on two lanes and phase 3 on all lanes of a
+// wavefront; mie_feature runs the phases in sequence.
+
+struct MieSetup {
+    double kum, x, n_m;
+    cd m;
+    int N;
+};
+
+LMZ_HD MieSetup mie_setup(double a_p, double n_p, double k_p, double n_m,
+                          double wavelength, int cap) {
+    MieSetup ms;
+    ms.kum = 2.0 * M_PI / wavelength * n_m;    // rad / um in the medium
+    ms.x = ms.kum * a_p;
+    ms.n_m = n_m;
+    ms.m = cd{n_p / n_m, k_p / n_m};
+    int N = wiscombe_yang(ms.x, ms.m);
     if (N > cap) N = cap;
     if (N < 1) N = 1;
-#define W(n, j) work[((long long)(n) * LMZ_WORK_PER_ORDER + (j)) * wstride]
-    // downward D1(z), z = m x, with dD1/dz; D1(x) with dD1/dx
-    {
-        const cd z = x * m;
-        const cd iz = cinv(z);
-        cd d1{0.0, 0.0}, dd1{0.0, 0.0};
-        double r1 = 0.0, dr1 = 0.0;
-        W(N, 0) = 0.0; W(N, 1) = 0.0; W(N, 2) = 0.0; W(N, 3) = 0.0;
-        W(N, 4) = 0.0; W(N, 5) = 0.0;
-        for (int n = N; n >= 1; --n) {
-            const cd nz = (double)n * iz;                 // n / z
-            const cd dnz = -((double)n * (iz * iz));      // -n / z^2
-            const cd inv = cinv(d1 + nz);
-            const cd d1n = nz - inv;
-            dd1 = dnz + (dd1 + dnz) * (inv * inv);
-            d1 = d1n;
-            const double nx = n / x, dnx = -n / (x * x);
-            const double rinv = 1.0 / (r1 + nx);
-            const double r1n = nx - rinv;
-            dr1 = dnx + (dr1 + dnx) * (rinv * rinv);
-            r1 = r1n;
-            W(n - 1, 0) = d1.re; W(n - 1, 1) = d1.im;
-            W(n - 1, 2) = dd1.re; W(n - 1, 3) = dd1.im;
-            W(n - 1, 4) = r1; W(n - 1, 5) = dr1;
-        }
+    ms.N = N;
+    return ms;
+}
+
+#define LMZ_W(n, j) work[((long long)(n) * LMZ_WORK_PER_ORDER + (j)) * wstride]
+
+// Phase 1: D1_n(z) downward from D1_N = 0 (Eq. 16b) with dD1_n/dz, into
+// columns col..col+3 (re, im, d re, d im) for n = 0..N.  n / z is a true
+// division, as theory/Sphere.py:196, 203 evaluate it: for real z near the
+// poles of D1 the recursion amplifies a last-bit difference in n / z to
+// 1e-9 in a_n (a_p = 9.5 um), which n * (1 / z) would introduce.
+LMZ_HD void mie_d1_down(cd z, int N, double *work, long long wstride, int col) {
+    const cd iz = cinv(z);
+    cd d1{0.0, 0.0}, dd1{0.0, 0.0};
+    LMZ_W(N, col) = 0.0; LMZ_W(N, col + 1) = 0.0;
+    LMZ_W(N, col + 2) = 0.0; LMZ_W(N, col + 3) = 0.0;
+    for (int n = N; n >= 1; --n) {
+        const cd nz = cd{(double)n, 0.0} / z;         // n / z
+        const cd dnz = -(nz * iz);                    // -n / z^2
+        const cd inv = cinv(d1 + nz);
+        dd1 = dnz + (dd1 + dnz) * (inv * inv);
+        d1 = nz - inv;
+        LMZ_W(n - 1, col) = d1.re; LMZ_W(n - 1, col + 1) = d1.im;
+        LMZ_W(n - 1, col + 2) = dd1.re; LMZ_W(n - 1, col + 3) = dd1.im;
     }
-    // upward Psi, zeta, Psi zeta, D3 with d/dx
+}
+
+// Phase 2: Psi_n(x) (Eq. 20), zeta_n(x) (Eq. 21), Psi_n zeta_n and D3_n
+// (Eqs. 18a-d) upward with d/dx; Psi, dPsi into columns 8, 9, zeta and
+// dzeta into 10-13, for n = 0..N.  Needs D1(x) in column 4, dD1/dx in 6.
+LMZ_HD void mie_up_chain(double x, int N, double *work, long long wstride) {
     double sx, cx;
     sincos(x, &sx, &cx);
     double s2x, c2x;
@@ -713,67 +731,100 @@ LMZ_HD int mie_feature(double a_p, double n_p, double k_p, double n_m,
     cd pz{0.5 * (1.0 - c2x), -0.5 * s2x};            // Eq. 18a
     cd dpz{s2x, -c2x};                               // d/dx = -i e^{2ix}
     cd d3{0.0, 1.0}, dd3{0.0, 0.0};                  // Eq. 18b
-    const cd m2 = m * m;
-    const cd im = cinv(m);
+    LMZ_W(0, 8) = psi; LMZ_W(0, 9) = dpsi;
+    LMZ_W(0, 10) = zeta.re; LMZ_W(0, 11) = zeta.im;
+    LMZ_W(0, 12) = dzeta.re; LMZ_W(0, 13) = dzeta.im;
     for (int n = 1; n <= N; ++n) {
         const double nx = n / x, dnx = -n / (x * x);
-        const double w1 = nx - W(n - 1, 4);
-        const double dw1 = dnx - W(n - 1, 5);
+        const double w1 = nx - LMZ_W(n - 1, 4);
+        const double dw1 = dnx - LMZ_W(n - 1, 6);
         const cd w3 = cd{nx, 0.0} - d3;
         const cd dw3 = cd{dnx, 0.0} - dd3;
-        const double psi_m = psi, dpsi_m = dpsi;
-        const cd zeta_m = zeta, dzeta_m = dzeta;
+        const double psi_m = psi;
+        const cd zeta_m = zeta;
         psi = psi_m * w1;                              // Eq. 20b
-        dpsi = dpsi_m * w1 + psi_m * dw1;
+        dpsi = dpsi * w1 + psi_m * dw1;
         zeta = zeta_m * w3;                            // Eq. 21b
-        dzeta = dzeta_m * w3 + zeta_m * dw3;
+        dzeta = dzeta * w3 + zeta_m * dw3;
         const cd pz_m = pz;
         pz = pz_m * (w1 * w3);                         // Eq. 18c
         dpz = dpz * (w1 * w3) + pz_m * (dw1 * w3 + w1 * dw3);
         const cd ipz = cinv(pz);
-        d3 = cd{W(n, 4), 0.0} + mul_i(ipz);            // Eq. 18d
-        dd3 = cd{W(n, 5), 0.0} - mul_i(dpz * (ipz * ipz));
-        // coefficients, Eqs. 5 and 6 (H_a = H_b = D1(m x))
-        const cd H{W(n, 0), W(n, 1)};
-        const cd dH{W(n, 2), W(n, 3)};                 // dD1/dz at z = m x
-        const cd fa = H * im + cd{nx, 0.0};
-        const cd fa_x = dH + cd{dnx, 0.0};             // (dH m)/m - n/x^2
-        const cd fa_m = x * (dH * im) - H * (im * im);
-        const cd numA = fa * cd{psi, 0.0} - cd{psi_m, 0.0};
-        const cd denA = fa * zeta - zeta_m;
-        const cd idA = cinv(denA);
-        const cd a = numA * idA;
-        const cd a_x = (fa_x * cd{psi, 0.0} + fa * cd{dpsi, 0.0} -
-                        cd{dpsi_m, 0.0} -
-                        a * (fa_x * zeta + fa * dzeta - dzeta_m)) * idA;
-        const cd a_m = fa_m * (cd{psi, 0.0} - a * zeta) * idA;
-        const cd fb = m * H + cd{nx, 0.0};
-        const cd fb_x = m2 * dH + cd{dnx, 0.0};
-        const cd fb_m = H + x * (m * dH);
-        const cd numB = fb * cd{psi, 0.0} - cd{psi_m, 0.0};
-        const cd denB = fb * zeta - zeta_m;
-        const cd idB = cinv(denB);
-        const cd b = numB * idB;
-        const cd b_x = (fb_x * cd{psi, 0.0} + fb * cd{dpsi, 0.0} -
-                        cd{dpsi_m, 0.0} -
-                        b * (fb_x * zeta + fb * dzeta - dzeta_m)) * idB;
-        const cd b_m = fb_m * (cd{psi, 0.0} - b * zeta) * idB;
-        if (coef) write_order(n, a, b, coef + LMZ_COEF_STRIDE * (n - 1));
-        if (dcoef) {
-            float *drow = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
-            write_dorder(n, kum * a_x, kum * b_x, drow);                 // a_p
-            write_dorder(n, (1.0 / n_m) * a_m, (1.0 / n_m) * b_m, drow + 8);   // n_p
-            write_dorder(n, mul_i((1.0 / n_m) * a_m), mul_i((1.0 / n_m) * b_m),
-                         drow + 16);                                     // k_p
-        }
-        if (ab) {
-            double *r = ab + 4 * (n - 1);
-            r[0] = a.re; r[1] = a.im; r[2] = b.re; r[3] = b.im;
-        }
+        d3 = cd{LMZ_W(n, 4), 0.0} + mul_i(ipz);        // Eq. 18d
+        dd3 = cd{LMZ_W(n, 6), 0.0} - mul_i(dpz * (ipz * ipz));
+        LMZ_W(n, 8) = psi; LMZ_W(n, 9) = dpsi;
+        LMZ_W(n, 10) = zeta.re; LMZ_W(n, 11) = zeta.im;
+        LMZ_W(n, 12) = dzeta.re; LMZ_W(n, 13) = dzeta.im;
     }
-#undef W
-    return N;
 }
+
+// Phase 3: a_n, b_n (Eqs. 5 and 6 with H_a = H_b = D1(m x)) and their
+// tangents for one order n >= 1, written as table rows.
+LMZ_HD void mie_order(const MieSetup &ms, int n, const double *work,
+                      long long wstride, float *coef, float *dcoef,
+                      double *ab) {
+    const double x = ms.x;
+    const cd m = ms.m;
+    const double nx = n / x, dnx = -n / (x * x);
+    const double psi = LMZ_W(n, 8), dpsi = LMZ_W(n, 9);
+    const double psi_m = LMZ_W(n - 1, 8), dpsi_m = LMZ_W(n - 1, 9);
+    const cd zeta{LMZ_W(n, 10), LMZ_W(n, 11)}, dzeta{LMZ_W(n, 12), LMZ_W(n, 13)};
+    const cd zeta_m{LMZ_W(n - 1, 10), LMZ_W(n - 1, 11)};
+    const cd dzeta_m{LMZ_W(n - 1, 12), LMZ_W(n - 1, 13)};
+    const cd H{LMZ_W(n, 0), LMZ_W(n, 1)};
+    const cd dH{LMZ_W(n, 2), LMZ_W(n, 3)};                 // dD1/dz at z = m x
+    const cd im = cinv(m);
+    const cd fa = H * im + cd{nx, 0.0};
+    const cd fa_x = dH + cd{dnx, 0.0};             // (dH m)/m - n/x^2
+    const cd fa_m = x * (dH * im) - H * (im * im);
+    const cd numA = fa * cd{psi, 0.0} - cd{psi_m, 0.0};
+    const cd denA = fa * zeta - zeta_m;
+    const cd idA = cinv(denA);
+    const cd a = numA * idA;
+    const cd a_x = (fa_x * cd{psi, 0.0} + fa * cd{dpsi, 0.0} -
+                    cd{dpsi_m, 0.0} -
+                    a * (fa_x * zeta + fa * dzeta - dzeta_m)) * idA;
+    const cd a_m = fa_m * (cd{psi, 0.0} - a * zeta) * idA;
+    const cd fb = m * H + cd{nx, 0.0};
+    const cd fb_x = (m * m) * dH + cd{dnx, 0.0};
+    const cd fb_m = H + x * (m * dH);
+    const cd numB = fb * cd{psi, 0.0} - cd{psi_m, 0.0};
+    const cd denB = fb * zeta - zeta_m;
+    const cd idB = cinv(denB);
+    const cd b = numB * idB;
+    const cd b_x = (fb_x * cd{psi, 0.0} + fb * cd{dpsi, 0.0} -
+                    cd{dpsi_m, 0.0} -
+                    b * (fb_x * zeta + fb * dzeta - dzeta_m)) * idB;
+    const cd b_m = fb_m * (cd{psi, 0.0} - b * zeta) * idB;
+    if (coef) write_order(n, a, b, coef + LMZ_COEF_STRIDE * (n - 1));
+    if (dcoef) {
+        const double inm = 1.0 / ms.n_m;
+        float *drow = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
+        write_dorder(n, ms.kum * a_x, ms.kum * b_x, drow);                 // a_p
+        write_dorder(n, inm * a_m, inm * b_m, drow + 8);                  // n_p
+        write_dorder(n, mul_i(inm * a_m), mul_i(inm * b_m), drow + 16);   // k_p
+    }
+    if (ab) {
+        double *r = ab + 4 * (n - 1);
+        r[0] = a.re; r[1] = a.im; r[2] = b.re; r[3] = b.im;
+    }
+}
+
+// All three phases in sequence on one thread.  Returns n_max (clamped to
+// cap).
+LMZ_HD int mie_feature(double a_p, double n_p, double k_p, double n_m,
+                       double wavelength, int cap, double *work,
+                       long long wstride, float *coef, float *dcoef,
+                       double *ab) {
+    const MieSetup ms = mie_setup(a_p, n_p, k_p, n_m, wavelength, cap);
+    mie_d1_down(ms.x * ms.m, ms.N, work, wstride, 0);
+    mie_d1_down(cd{ms.x, 0.0}, ms.N, work, wstride, 4);
+    mie_up_chain(ms.x, ms.N, work, wstride);
+    for (int n = 1; n <= ms.N; ++n)
+        mie_order(ms, n, work, wstride, coef, dcoef, ab);
+    return ms.N;
+}
+#undef LMZ_W
 
 // Order count of a layered sphere: Wiscombe (1980) on the outer size
 // parameter with Yang (2003) Eq. 30, N = max(n_s, |m_l x_l|, |m_l x_{l-1}|).
@@ -824,12 +875,11 @@ LMZ_HD int mie_layered_feature(const double *radius, const double *nl,
 #define W(n, j) work[((long long)(n) * LMZ_WORK_PER_ORDER + (j)) * wstride]
     // D1 downward from D1[N] = 0 into work columns (c, c + 1), orders 0..N
     auto d1_down = [&](cd z, int c) {
-        const cd iz = cinv(z);
         cd d1{0.0, 0.0};
         W(N, c) = 0.0;
         W(N, c + 1) = 0.0;
         for (int n = N; n >= 1; --n) {
-            const cd nz = (double)n * iz;
+            const cd nz = cd{(double)n, 0.0} / z;     // as mie_d1_down
             d1 = nz - cinv(d1 + nz);
             W(n - 1, c) = d1.re;
             W(n - 1, c + 1) = d1.im;

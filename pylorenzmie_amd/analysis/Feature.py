@@ -1,0 +1,72 @@
+"""One particle's crop of a hologram, mirroring analysis/Feature.py:9-140.
+
+A Feature is a Hologram with a LorenzMie model, a Mask that excludes bad
+pixels (saturated, NaN, inf; analysis/Feature.py:36-43) and an Optimizer
+on that mask.  ``optimize()`` fits this one feature; ``Frame.optimize``
+fits all features of a frame in one device batch.  Parameter estimation
+(the reference's DEEstimator) is outside this library: set the starting
+particle, or call ``estimate()`` for the error that says so.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pandas as pd
+
+from ..theory import LorenzMie
+from .Hologram import Hologram
+from .Mask import Mask
+from .Optimizer import Optimizer
+
+
+class Feature(Hologram):
+    """Crop ``hologram`` with its generative ``model`` and fitter."""
+
+    def __init__(self, hologram: Hologram, model: LorenzMie | None = None,
+                 fixed: list[str] | None = None) -> None:
+        # Hologram state copied as analysis/Feature.py:29-33 does
+        self.data = hologram.data
+        self.corner = hologram.corner
+        self._coordinates = hologram._coordinates
+        self.model = model or LorenzMie()
+        m = Mask()
+        m.shape = self.data.shape
+        m.exclude = ((self.data == np.max(self.data)) | np.isnan(self.data) |
+                     np.isinf(self.data))
+        self.optimizer = Optimizer(model=self.model, mask=m)
+        if fixed is not None:
+            self.optimizer.fixed = fixed
+
+    @property
+    def mask(self) -> Mask:
+        """Pixel mask shared with the optimizer (analysis/Feature.py:49-61)."""
+        return self.optimizer.mask
+
+    @mask.setter
+    def mask(self, mask: Mask) -> None:
+        self.optimizer.mask = mask
+
+    @property
+    def particle(self):
+        return self.model.particle
+
+    @particle.setter
+    def particle(self, particle) -> None:
+        self.model.particle = particle
+
+    def estimate(self) -> pd.Series:
+        raise NotImplementedError(
+            'parameter estimation (DEEstimator) is not part of this library: '
+            'set feature.particle to the starting parameters')
+
+    def optimize(self) -> pd.Series:
+        """Fit this feature (analysis/Feature.py:87-100); a fresh random
+        pixel subset is drawn on every call when the mask's fraction < 1."""
+        return self.optimizer.optimize(self)
+
+    def predicted(self) -> np.ndarray:
+        """Model hologram over all pixels of the crop."""
+        self.model.coordinates = self.flat_coordinates
+        return self.model.hologram().reshape(self.shape)
+
+    def residuals(self) -> np.ndarray:
+        return self.predicted() - self.data

@@ -3,7 +3,10 @@ from .Hologram import Hologram
 from .Mask import Mask
 from .Optimizer import Optimizer, hipOptimizer
 from .BatchOptimizer import BatchOptimizer
+from .Feature import Feature
+from .Frame import Frame
 from .lm import LMBatch, LMSettings
 
 __all__ = ['Hologram', 'Mask', 'Optimizer', 'hipOptimizer', 'BatchOptimizer',
+           'Feature', 'Frame',
            'LMBatch', 'LMSettings']

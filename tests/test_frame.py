@@ -1,0 +1,52 @@
+"""Frame / Feature drivers on the host (analysis/Frame.py, Feature.py):
+boxes become features with the right crops and starting positions, the
+mask drops bad pixels, and detection/estimation report that they are
+outside the library.  No GPU."""
+import numpy as np
+import pytest
+
+from pylorenzmie_amd.analysis import Feature, Frame, Hologram
+from pylorenzmie_amd.theory import Instrument
+
+
+def test_bboxes_make_features():
+    data = np.random.rand(300, 400) + 0.5
+    fr = Frame(data, instrument=Instrument(n_m=1.33))
+    fr.bboxes = [((10, 20), 64, 64), ((200, 100), 80, 70)]
+    assert len(fr.features) == 2
+    f0, f1 = fr.features
+    assert f0.shape == (64, 64) and f1.shape == (70, 70)
+    assert f1.corner == (200., 100.)
+    assert np.array_equal(f1.data, data[100:170, 200:270])
+    assert (f1.particle.x_p, f1.particle.y_p) == (235., 135.)
+    assert f1.model.instrument is fr.instrument
+    fr.data = data
+    assert fr.features == [] and fr.bboxes == []
+
+
+def test_feature_mask_excludes_bad_pixels():
+    """Saturated (= max) pixels and non-finite ones, as
+    analysis/Feature.py:39-43 (np.max of data holding a NaN is NaN, so
+    with a NaN present only the non-finite pixels are caught)."""
+    d = np.ones((50, 50))
+    d[10, 10] = d[20, 5] = 7.
+    f = Feature(Hologram(d))
+    assert f.mask.exclude[10, 10] and f.mask.exclude[20, 5]
+    assert f.mask.exclude.sum() == 2
+    assert not f.mask()[10, 10] and 0.05 < f.mask().mean() < 0.15
+    d[3, 4] = np.nan
+    d[4, 4] = np.inf
+    f = Feature(Hologram(d))
+    assert f.mask.exclude[3, 4] and f.mask.exclude[4, 4]
+    assert not f.mask()[3, 4] and not f.mask()[4, 4]
+
+
+def test_detection_and_estimation_are_outside():
+    fr = Frame(np.ones((64, 64)))
+    with pytest.raises(NotImplementedError):
+        fr.detect()
+    with pytest.raises(NotImplementedError):
+        Frame(localizer=object())
+    fr.bboxes = [((0, 0), 32, 32)]
+    with pytest.raises(NotImplementedError):
+        fr.estimate()

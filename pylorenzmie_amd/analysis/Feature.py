@@ -2,10 +2,9 @@
 
 A Feature is a Hologram with a LorenzMie model, a Mask that excludes bad
 pixels (saturated, NaN, inf; analysis/Feature.py:36-43) and an Optimizer
-on that mask.  ``optimize()`` fits this one feature; ``Frame.optimize``
-fits all features of a frame in one device batch.  Parameter estimation
-(the reference's DEEstimator) is outside this library: set the starting
-particle, or call ``estimate()`` for the error that says so.
+on that mask, and a DEEstimator for starting values.  ``optimize()`` fits
+this one feature; ``Frame.optimize`` fits all features of a frame in one
+device batch.
 """
 from __future__ import annotations
 
@@ -13,6 +12,7 @@ import numpy as np
 import pandas as pd
 
 from ..theory import LorenzMie
+from .DEEstimator import DEEstimator
 from .Hologram import Hologram
 from .Mask import Mask
 from .Optimizer import Optimizer
@@ -32,6 +32,7 @@ class Feature(Hologram):
         m.shape = self.data.shape
         m.exclude = ((self.data == np.max(self.data)) | np.isnan(self.data) |
                      np.isinf(self.data))
+        self.estimator = DEEstimator(model=self.model, exclude=m.exclude)
         self.optimizer = Optimizer(model=self.model, mask=m)
         if fixed is not None:
             self.optimizer.fixed = fixed
@@ -43,6 +44,7 @@ class Feature(Hologram):
 
     @mask.setter
     def mask(self, mask: Mask) -> None:
+        self.estimator.exclude = mask.exclude
         self.optimizer.mask = mask
 
     @property
@@ -54,9 +56,9 @@ class Feature(Hologram):
         self.model.particle = particle
 
     def estimate(self) -> pd.Series:
-        raise NotImplementedError(
-            'parameter estimation (DEEstimator) is not part of this library: '
-            'set feature.particle to the starting parameters')
+        """Starting parameters by differential evolution on the GPU
+        (DEEstimator, population-batched), as analysis/Feature.py:76-85."""
+        return self.estimator.estimate(self)
 
     def optimize(self) -> pd.Series:
         """Fit this feature (analysis/Feature.py:87-100); a fresh random

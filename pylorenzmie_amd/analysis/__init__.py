@@ -3,10 +3,11 @@ from .Hologram import Hologram
 from .Mask import Mask
 from .Optimizer import Optimizer, hipOptimizer
 from .BatchOptimizer import BatchOptimizer
+from .DEEstimator import DEEstimator
 from .Feature import Feature
 from .Frame import Frame
 from .lm import LMBatch, LMSettings
 
 __all__ = ['Hologram', 'Mask', 'Optimizer', 'hipOptimizer', 'BatchOptimizer',
-           'Feature', 'Frame',
+           'DEEstimator', 'Feature', 'Frame',
            'LMBatch', 'LMSettings']

@@ -41,12 +41,13 @@ def test_feature_mask_excludes_bad_pixels():
     assert not f.mask()[3, 4] and not f.mask()[4, 4]
 
 
-def test_detection_and_estimation_are_outside():
+def test_detection_is_outside_estimation_is_de():
     fr = Frame(np.ones((64, 64)))
     with pytest.raises(NotImplementedError):
         fr.detect()
     with pytest.raises(NotImplementedError):
         Frame(localizer=object())
     fr.bboxes = [((0, 0), 32, 32)]
-    with pytest.raises(NotImplementedError):
-        fr.estimate()
+    est = fr.features[0].estimator
+    assert est.exclude is fr.features[0].mask.exclude
+    assert set(est.bounds) == {'z_p', 'a_p', 'n_p'} and est.fraction == 0.02

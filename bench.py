@@ -206,17 +206,16 @@ def fit_agreement(lm, params, res, p0):
                        for i, k in enumerate(PARAMS)}}
 
 
-def committed_traffic(nfeat, side):
-    """HBM bytes per full-batch Jacobian launch, from the PMC summary
-    committed under profiles/ for this workload (FETCH_SIZE and WRITE_SIZE,
-    calibrated against a copy kernel), or None for other workloads."""
+def committed_figure(name, key, nfeat, side):
+    """A per-launch figure of the Jacobian kernel from a PMC summary
+    committed under profiles/ for this workload, or None for other
+    workloads: HBM bytes (FETCH_SIZE + WRITE_SIZE, calibrated against a
+    copy kernel) or counted FP32 FLOP (SQ_INSTS_VALU_FLOPS_FP32)."""
     import re
-    path = os.path.join(ROOT, 'profiles',
-                        f'k_jacobian_traffic_{nfeat}x{side}x{side}.md')
+    path = os.path.join(ROOT, 'profiles', f'{name}_{nfeat}x{side}x{side}.md')
     if not os.path.exists(path):
         return None
-    m = re.search(r'traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)',
-                  open(path).read())
+    m = re.search(key + r'\s*=\s*([0-9.eE+]+)', open(path).read())
     return float(m.group(1)) if m else None
 
 
@@ -342,7 +341,10 @@ def main():
 
     # J reads the data (4 B / pixel) and writes 36 FP64 partials per tile
     bytes_j = 4. * npx + 36 * 8. * lm.batch.ntiles
-    traffic = committed_traffic(args.nfeat, args.side)
+    traffic = committed_figure('k_jacobian_traffic', 'traffic_bytes_per_launch',
+                               args.nfeat, args.side)
+    counted = committed_figure('valu_counters', 'counted_fp32_flop_per_launch',
+                               args.nfeat, args.side)
     fd = None
     if not args.no_fd:
         log('bench: plain GPU FD form sample')
@@ -392,6 +394,8 @@ def main():
                      'pixels_per_launch': npx,
                      'traffic': traffic,
                      'algorithmic_bytes_per_launch': bytes_j,
+                     'counted_fp32_flop_per_launch': counted,
+                     'counted_achieved': (counted / tj / 1e12) if counted else None,
                      'forward_kernel': {'achieved': ach_f, 'avg_launch_ms': tf * 1e3,
                                         'algorithmic_flop_per_launch': flop_f}},
         'fit_quality': {'converged_frac': float(ok.mean()),

@@ -25,6 +25,7 @@ TUNE_FORCE = 1
 CARTESIAN = 1
 BOHREN = 2
 ACCUMULATE = 4
+FP32 = 8
 
 LM_RUNNING, LM_FTOL, LM_XTOL, LM_GTOL = 0, 1, 2, 4
 LM_MAXITER, LM_STALLED, LM_NONFINITE = 8, 16, 32

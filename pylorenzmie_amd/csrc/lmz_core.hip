@@ -152,7 +152,8 @@ __device__ __forceinline__ int next_tile(int32_t *count, int &slot, int &k) {
 }
 
 // Forward pass over tiles of LMZ_TILE pixels, in FP64.
-// MODE 0: field, 1: hologram, 2: residual (+ per-tile r^T r).
+// MODE 0: field, 1: hologram, 2: residual (+ per-tile r^T r), 3: the
+// residual in FP32 arithmetic (Cartesian; chi2 partials still FP64).
 // Without a work list each block takes tile xcd_tile(blockIdx.x); with one
 // (the LM's compacted list of active tiles, length count[0]) the grid is
 // persistent and blocks fetch list entries dynamically (next_tile) until
@@ -196,12 +197,22 @@ __global__ __launch_bounds__(NT) void k_forward(
         if (j >= fu.npix) break;
         float ax, ay, az;
         pixel_coords(fu, j, coords, npix_total, ax, ay, az);
+        const long long gi = fu.pix_offset + j;
+        if constexpr (MODE == 3) {   // FP32 residual (Cartesian)
+            PixGeom gf;
+            pixel_geometry(fu, ax, ay, az, bohren, gf);
+            const float I = intensity_f(gf, forward_sum_f(gf, cf0, fu.nmax),
+                                        fu.alpha);
+            const double r = ((double)I - (double)data[gi]) / F.noise;
+            if (outf) outf[gi] = (float)r;
+            chi = fma(r, r, chi);
+            continue;
+        }
         PixGeomD g;
         pixel_geometry_d(fu, ax, ay, az, bohren, g);
         const SumsD S = forward_sum(g, cf0, fu.nmax);
         cd E[3];
         field_from_sums(g, S, cart, E);
-        const long long gi = fu.pix_offset + j;
         if (MODE == 0) {
             const double al = fu.alphad;
 #pragma unroll
@@ -221,7 +232,7 @@ __global__ __launch_bounds__(NT) void k_forward(
             }
         }
     }
-    if (MODE == 2 && chi2_tiles) {
+    if ((MODE == 2 || MODE == 3) && chi2_tiles) {
         double v[1] = {chi};
         block_reduce_store_d<1, NT>(v, chi2_tiles + t);
         __syncthreads();   // the reduction buffer is reused by the next tile
@@ -465,6 +476,16 @@ void launch_residual(const lmz_batch *b, const double *params,
                      double *chi2_tiles, const uint8_t *active,
                      const TileList *tl, hipStream_t stream) {
     const int grid = tl ? tl->grid : b->ntiles;
+    if (flags & LMZ_FP32) {
+        launch_forward_nt<3>(b->sched.fwd_threads, grid, stream, b->d_feat,
+                             b->d_tile_feature, params, sel, flip,
+                             b->coef_buffers, nmax, coef, coords,
+                             b->npix_total, flags, data, out, out_stride,
+                             (cf *)nullptr, chi2_tiles, active,
+                             tl ? (const int32_t *)tl->list : nullptr,
+                             tl ? tl->count : nullptr);
+        return;
+    }
     launch_forward_nt<2>(b->sched.fwd_threads, grid, stream, b->d_feat,
                          b->d_tile_feature, params, sel, flip, b->coef_buffers,
                          nmax, coef, coords, b->npix_total, flags, data, out,
@@ -784,6 +805,8 @@ int lmz_hologram(const lmz_batch *b, const double *params, const int32_t *sel,
                              (int32_t *)nullptr);
     } else {
         LMZ_REQUIRE(out || chi2_tiles, "one of out, chi2_tiles is required");
+        LMZ_REQUIRE(!(flags & LMZ_FP32) || (flags & LMZ_CARTESIAN),
+                    "FP32 residuals are defined for the Cartesian hologram");
         launch_residual(b, params, sel, 0, nmax, coef, coords, flags, data, out,
                         0LL, chi2_tiles, active, nullptr, stream);
     }

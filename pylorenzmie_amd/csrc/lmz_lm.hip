@@ -482,6 +482,8 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
     LMZ_REQUIRE(set.ftol >= 0.0 && set.xtol >= 0.0 && set.gtol >= 0.0,
                 "tolerances must be >= 0");
     LMZ_REQUIRE(set.lambda0 > 0.0, "lambda0 must be > 0");
+    LMZ_REQUIRE(!(flags & LMZ_FP32),
+                "the LM's residuals are FP64 (its 1e-9 ftol test needs them)");
     int nv = 0;
     for (int i = 0; i < LMZ_NJAC; ++i) nv += set.vary[i] ? 1 : 0;
     LMZ_REQUIRE(nv >= 1, "no parameter is varied");

@@ -60,6 +60,20 @@ void chk_pixels(const lmz_feature *feat, const double *params,
     }
 }
 
+// FP32 hologram of one feature (the residual kernel's fast mode).
+void chk_holo32(const lmz_feature *feat, const double *params,
+                const float *coef, int nmax, const float *coords, int npix,
+                int flags, float *holo) {
+    const FeatUniform fu = feature_uniform(*feat, params, nmax, 0);
+    const bool bohren = flags & LMZ_BOHREN;
+    for (int j = 0; j < npix; ++j) {
+        PixGeom g;
+        pixel_geometry(fu, coords[j], coords[npix + j], coords[2 * npix + j],
+                       bohren, g);
+        holo[j] = intensity_f(g, forward_sum_f(g, coef, nmax), fu.alpha);
+    }
+}
+
 // Dense-grid pixel coordinates as the kernels derive them.
 void chk_dense_coords(int width, int npix, float *ax, float *ay) {
     lmz_feature F{};

@@ -157,3 +157,50 @@ def test_batched_uneven_crops_match_oracle(api):
         worst = max(worst, np.abs(h - truth).max())
     print(f'batched forward worst max abs error {worst:.3e}')
     assert worst <= HOLO_TOL
+
+
+def test_fp32_residual_flag(api):
+    """lmz_hologram with LMZ_FP32: intensities within 1e-4 + 2e-5 I of the
+    FP64 kernel's (measured: up to 1.6e-4 at I = 13, ~5e-5 at I ~ 1, i.e.
+    1e-3 of the noise), chi2 within 1e-5 relative; the LM refuses the flag
+    (its ftol test needs FP64 residuals)."""
+    import torch
+    from pylorenzmie_amd import _lmz
+    from pylorenzmie_amd.theory import Instrument
+    from pylorenzmie_amd.theory.batch import HologramBatch
+    rng = np.random.default_rng(4)
+    n, side = 32, 120
+    p = np.zeros((n, 7))
+    for i in range(n):
+        a = rng.uniform(0.4, 2.5)     # particles above the focal plane
+        p[i] = (59.5 + rng.uniform(-4, 4), 59.5 + rng.uniform(-4, 4),
+                a / 0.048 + rng.uniform(20, 250), a, rng.uniform(1.38, 1.65),
+                rng.uniform(0, 0.01), rng.uniform(0.8, 1.2))
+    inst = Instrument(noise=0.05, **INSTRUMENT)
+    caps = [_lmz.nmax_cap(q[3], q[4], q[5], 1.34, 0.447) for q in p]
+    hb = HologramBatch([(side, side)] * n, caps, instrument=inst)
+    hb.load(p)
+    data = hb.compute().clone() + 0.05 * torch.randn(hb.npix_total, device='cuda')
+    lib = _lmz.load_library()
+    b = hb.batch
+    out, chi = {}, {}
+    for fl in (3, 3 | _lmz.FP32):
+        r = torch.zeros(hb.npix_total, dtype=torch.float32, device='cuda')
+        c2 = torch.zeros(b.ntiles, dtype=torch.float64, device='cuda')
+        _lmz.check(lib.lmz_hologram(b.handle, _lmz.ptr(hb.params), None,
+                                    _lmz.ptr(hb.nmax), _lmz.ptr(hb.coef), None, fl,
+                                    _lmz.ptr(data), _lmz.ptr(r), _lmz.ptr(c2), None,
+                                    _lmz.stream_handle()))
+        out[fl], chi[fl] = r.cpu().numpy(), c2.sum().item()
+    d = data.cpu().numpy()
+    i64, i32 = out[3] * 0.05 + d, out[3 | _lmz.FP32] * 0.05 + d
+    err = np.abs(i32 - i64) - 2e-5 * np.abs(i64)
+    print('FP32 - FP64 max |dI|', np.abs(i32 - i64).max(), 'at I', i64[np.argmax(np.abs(i32 - i64))])
+    assert err.max() < 1e-4
+    assert chi[3 | _lmz.FP32] == pytest.approx(chi[3], rel=1e-5)
+    from pylorenzmie_amd.analysis.lm import LMBatch, LMSettings
+    lm = LMBatch([dict(data=np.ones((8, 8), np.float32), width=8, nmax_cap=40)], inst)
+    lm.flags = 3 | _lmz.FP32
+    lm.load(np.array([[4., 4., 100., 1., 1.5, 0., 1.]]))
+    with pytest.raises(_lmz.LMZError, match='FP64'):
+        lm.run(LMSettings(), ['x_p', 'y_p', 'z_p'])

@@ -31,6 +31,8 @@ def chk():
     lib.chk_pixels.argtypes = [P, P, P, P, ctypes.c_int, P, ctypes.c_int,
                                ctypes.c_int, P, P, P]
     lib.chk_dense_coords.argtypes = [ctypes.c_int, ctypes.c_int, P, P]
+    lib.chk_holo32.argtypes = [P, P, P, ctypes.c_int, P, ctypes.c_int,
+                               ctypes.c_int, P]
     return lib, Feature
 
 
@@ -193,3 +195,27 @@ def test_layered_one_layer_equals_homogeneous(chk):
         assert N1 == N
         got = np.stack([ab[:N, 0] + 1j * ab[:N, 1], ab[:N, 2] + 1j * ab[:N, 3]], 1)
         assert np.abs(ab1 - got).max() < 1e-12 * np.abs(got).max()
+
+
+@pytest.mark.parametrize('q,tol', [((100.3, 99.6, 150., 0.75, 1.45, 0.), 3e-6),
+                                   ((17.41, 11.047, 61.918, 1.595, 1.608, 0.006), 6e-5),
+                                   ((24.0, 24.0, 100., 2.0, 1.6, 0.0), 8e-5)])
+def test_fp32_residual_arithmetic(chk, q, tol):
+    """The FP32 forward (LMZ_FP32 residuals) against the FP64 oracle: about
+    1e-6 of I, up to ~4e-5 on bright forward lobes (I ~ 40-55)."""
+    lib, Feature = chk
+    c = grid3((201, 201)) if q[0] > 50 else grid3((40, 40))
+    N, coef, _, _ = mie(chk, q[3], q[4], q[5])
+    F = Feature()
+    F.npix = c.shape[1]
+    F.n_m, F.wavelength, F.magnification, F.noise = 1.34, 0.447, 0.048, 0.05
+    F.nmax_cap = 400
+    p = np.zeros(8)
+    p[:6] = q
+    p[6] = 1.
+    cc = np.ascontiguousarray(c.astype(np.float32))
+    h = np.zeros(cc.shape[1], np.float32)
+    lib.chk_holo32(ctypes.byref(F), _p(p), _p(coef), N, _p(cc), cc.shape[1], 3,
+                   _p(h))
+    truth = ref.hologram(c, dict(zip(ref.PARAMS, q)), **INSTRUMENT)
+    assert np.abs(h - truth).max() < tol

@@ -183,6 +183,31 @@ int lmz_hologram(const lmz_batch *batch, const double *params,
                  float *out, double *chi2_tiles, const uint8_t *active,
                  hipStream_t stream);
 
+/* Superposed holograms (or fields) of several particles per feature in
+ * one pass over the pixels, FP64, with an optional spherical-aberration
+ * mask per particle.
+ *   part_offset  int32 [nfeat + 1]: particles of feature f are
+ *                part_offset[f] .. part_offset[f + 1] - 1
+ *   pparams      double [nparticles][LMZ_NPARAM]: x_p y_p z_p (absolute,
+ *                any cluster origin added), a_p, n_p, k_p (unused here)
+ *   pnmax        int32 [nparticles], orders of each particle's table
+ *   pcoef        float [nparticles * pcap][coef_stride] per-order tables
+ *                (particle i at rows i * pcap), as lmz_mie_ab or
+ *                lmz_coef_from_ab write them for a batch of the particles
+ *   spherical    aberration coefficient [wavelengths], 0 for none
+ *   alpha        double [nfeat] or NULL (1)
+ *   out          float [npix_total] hologram, or field float2 [3][npix_total]
+ *                (alpha E): exactly one of the two
+ * Replaces the particle loop of LorenzMie.field (theory/LorenzMie.py:
+ * 160-181) with Cluster / Dimer particles (theory/Cluster.py:7-59,
+ * theory/Dimer.py:7-78), and the mask of theory/Aberrated.py:61-84. */
+int lmz_hologram_superposed(const lmz_batch *batch, int32_t nparticles,
+                            const int32_t *part_offset, const double *pparams,
+                            const int32_t *pnmax, const float *pcoef,
+                            int32_t pcap, const float *coords, int32_t flags,
+                            double spherical, const double *alpha, float *out,
+                            float *field, hipStream_t stream);
+
 /* Intensity of an accumulated field: out = |x + E|^2 per pixel, where the
  * field already carries alpha (lmz_field writes alpha E).  Used for
  * superposed particles (theory/LorenzMie.py:131-133 after :178-181). */

@@ -288,9 +288,26 @@ def scattered_field(coordinates, x_p, y_p, z_p, ab, k,
     return lorenzmie(ab, kdr, cartesian, bohren) * np.exp(-1.j * k * r_p[2])
 
 
+def aberration(coordinates, r_p, k, n_m, spherical):
+    """Spherical-aberration phase mask of a particle at r_p (absolute),
+    theory/Aberrated.py:61-76: exp(i (4 n_m^3 / k) s r^4 / (r^2 + z_p^2)^2)
+    with r the in-plane pixel distance and k in rad/px."""
+    if spherical == 0.:
+        return 1.
+    c = _coords3(coordinates)
+    x = c[0] - r_p[0]
+    y = c[1] - r_p[1]
+    rsq = x * x + y * y
+    zpsq = r_p[2] ** 2
+    phase = (4. * n_m ** 3 / k) * spherical * rsq ** 2 / (rsq + zpsq) ** 2
+    return np.exp(1j * phase)
+
+
 def field(coordinates, particles, n_m=1.340, wavelength=0.447,
-          magnification=0.048, cartesian=True, bohren=True):
-    """Superposed scattered field, theory/LorenzMie.py:160-181.
+          magnification=0.048, cartesian=True, bohren=True, spherical=0.):
+    """Superposed scattered field, theory/LorenzMie.py:160-181, each
+    particle's field times its aberration mask when ``spherical`` is not 0
+    (theory/Aberrated.py:78-84).
 
     ``particles`` is a dict or a list of dicts with keys x_p, y_p, z_p,
     a_p, n_p, k_p and optionally r_0, nmax, or ``ab`` (precomputed).
@@ -304,22 +321,24 @@ def field(coordinates, particles, n_m=1.340, wavelength=0.447,
         if ab is None:
             ab = mie_coefficients(p['a_p'], p['n_p'], p.get('k_p', 0.),
                                   n_m, wavelength, p.get('nmax'))
-        total = total + scattered_field(coordinates, p['x_p'], p['y_p'],
-                                        p['z_p'], ab, k,
-                                        p.get('r_0', (0., 0., 0.)),
-                                        cartesian, bohren)
+        r_0 = np.asarray(p.get('r_0', (0., 0., 0.)), float)
+        e = scattered_field(coordinates, p['x_p'], p['y_p'], p['z_p'], ab, k,
+                            r_0, cartesian, bohren)
+        r_p = np.array([p['x_p'], p['y_p'], p['z_p']]) + r_0
+        total = total + e * aberration(coordinates, r_p, k, n_m, spherical)
     return total
 
 
 def hologram(coordinates, particles, n_m=1.340, wavelength=0.447,
-             magnification=0.048, alpha=1.0, cartesian=True, bohren=True):
+             magnification=0.048, alpha=1.0, cartesian=True, bohren=True,
+             spherical=0.):
     """I = |x_hat + alpha E_s|^2 summed over components.
 
     theory/LorenzMie.py:113-133 (alpha = 1); alpha as in
     theory/obsolete/pycudaLMHologram.py:48-49.
     """
     e = alpha * field(coordinates, particles, n_m, wavelength, magnification,
-                      cartesian, bohren)
+                      cartesian, bohren, spherical)
     e[0] += 1.
     return np.sum(e.real ** 2 + e.imag ** 2, axis=0)
 

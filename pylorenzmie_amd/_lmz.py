@@ -101,6 +101,8 @@ SIGNATURES = {
     'lmz_field': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P]),
     'lmz_hologram': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
     'lmz_intensity': (_I, [_P, _P, _P, _I, _P, _P]),
+    'lmz_hologram_superposed': (_I, [_P, _I, _P, _P, _P, _P, _I, _P, _I,
+                                     ctypes.c_double, _P, _P, _P, _P]),
     'lmz_jacobian': (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P,
                           _P, _P]),
     'lmz_hologram_vjp': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),

@@ -320,6 +320,76 @@ __global__ __launch_bounds__(NT) void k_jacobian(
   }
 }
 
+// Superposed field of several particles per feature, FP64, in one pass
+// over the pixels (theory/LorenzMie.py:160-181 loops over particles with a
+// full pass each): particles part_offset[f] .. part_offset[f + 1] - 1,
+// rows of pparams (x_p y_p z_p absolute, a_p n_p k_p, -, -) with
+// coefficient tables at pcoef + LMZ_COEF_STRIDE * i * pcap.  When spherical
+// is not 0 each particle's field is multiplied by its aberration mask
+// exp(i (4 n_m^3 / k) s r^4 / (r^2 + z_p^2)^2) (theory/Aberrated.py:61-84),
+// r the in-plane pixel distance to the particle.
+// MODE 0: field alpha E, 1: hologram |x + alpha E|^2.
+template <int MODE>
+__global__ __launch_bounds__(LMZ_THREADS) void k_superpose(
+    const lmz_feature *__restrict__ feat,
+    const int32_t *__restrict__ tile_feature,
+    const int32_t *__restrict__ part_offset,
+    const double *__restrict__ pparams, const int32_t *__restrict__ pnmax,
+    const float *__restrict__ pcoef, int pcap,
+    const float *__restrict__ coords, long long npix_total, int flags,
+    double spherical, const double *__restrict__ alpha,
+    float *__restrict__ out, cf *__restrict__ field) {
+    const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int f = tile_feature[t];
+    const lmz_feature &F = feat[f];
+    const int p0 = part_offset[f], p1 = part_offset[f + 1];
+    const bool cart = flags & LMZ_CARTESIAN;
+    const bool bohren = flags & LMZ_BOHREN;
+    const double al = alpha ? alpha[f] : 1.0;
+    const double kk = wavenumber_px(F);
+    const double ab_scale = 4.0 * F.n_m * F.n_m * F.n_m / kk * spherical;
+#pragma unroll 1
+    for (int q = 0; q < LMZ_PPT; ++q) {
+        const int j = (t - F.tile_offset) * LMZ_TILE + q * LMZ_THREADS + threadIdx.x;
+        if (j >= F.npix) break;
+        cd E[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+        float ax = 0.f, ay = 0.f, az = 0.f;
+#pragma unroll 1
+        for (int i = p0; i < p1; ++i) {
+            const double *pp = pparams + (long long)LMZ_NPARAM * i;
+            const FeatUniform fu = feature_uniform(F, pp, pnmax[i], 0);
+            if (i == p0) pixel_coords(fu, j, coords, npix_total, ax, ay, az);
+            PixGeomD g;
+            pixel_geometry_d(fu, ax, ay, az, bohren, g);
+            const SumsD S = forward_sum(
+                g, pcoef + (long long)LMZ_COEF_STRIDE * i * pcap, fu.nmax);
+            cd Ei[3];
+            field_from_sums(g, S, cart, Ei);
+            cd m{1.0, 0.0};
+            if (spherical != 0.0) {
+                const double dx = ((double)ax - (double)fu.ix) - fu.fxd;
+                const double dy = ((double)ay - (double)fu.iy) - fu.fyd;
+                const double rsq = fma(dx, dx, dy * dy);
+                const double den = rsq + pp[2] * pp[2];
+                double sn, cs;
+                sincos(ab_scale * rsq * rsq / (den * den), &sn, &cs);
+                m = cd{cs, sn};
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) E[c] = E[c] + m * Ei[c];
+        }
+        const long long gi = F.pix_offset + j;
+        if (MODE == 0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                field[c * npix_total + gi] = cf{(float)(al * E[c].re),
+                                                (float)(al * E[c].im)};
+        } else {
+            out[gi] = (float)intensity(E, al);
+        }
+    }
+}
+
 // Vector-Jacobian product over one tile: sum of g dI/d(param) for the
 // seven parameters, J formed per pixel in registers and never written.
 __global__ __launch_bounds__(LMZ_THREADS) void k_vjp(
@@ -810,6 +880,36 @@ int lmz_hologram(const lmz_batch *b, const double *params, const int32_t *sel,
         launch_residual(b, params, sel, 0, nmax, coef, coords, flags, data, out,
                         0LL, chi2_tiles, active, nullptr, stream);
     }
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+int lmz_hologram_superposed(const lmz_batch *b, int32_t nparticles,
+                            const int32_t *part_offset, const double *pparams,
+                            const int32_t *pnmax, const float *pcoef,
+                            int32_t pcap, const float *coords, int32_t flags,
+                            double spherical, const double *alpha, float *out,
+                            float *field, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(nparticles >= 1 && pcap >= 1, "nparticles and pcap must be >= 1");
+    LMZ_REQUIRE(part_offset && pparams && pnmax && pcoef,
+                "part_offset, pparams, pnmax and pcoef are required");
+    LMZ_REQUIRE(coords || b->all_dense,
+                "coords is required when a feature has explicit coordinates");
+    LMZ_REQUIRE((out != nullptr) != (field != nullptr),
+                "exactly one of out (hologram) and field is required");
+    LMZ_REQUIRE(std::isfinite(spherical), "spherical must be finite");
+    if (out)
+        hipLaunchKernelGGL(k_superpose<1>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                           stream, b->d_feat, b->d_tile_feature, part_offset,
+                           pparams, pnmax, pcoef, pcap, coords, b->npix_total,
+                           flags, spherical, alpha, out, (cf *)nullptr);
+    else
+        hipLaunchKernelGGL(k_superpose<0>, dim3(b->ntiles), dim3(LMZ_THREADS), 0,
+                           stream, b->d_feat, b->d_tile_feature, part_offset,
+                           pparams, pnmax, pcoef, pcap, coords, b->npix_total,
+                           flags, spherical, alpha, (float *)nullptr,
+                           reinterpret_cast<cf *>(field));
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
 }

@@ -211,6 +211,58 @@ class LorenzMie(LMObject):
                 (_lmz.BOHREN if bohren else 0))
 
     # ------------------------------------------------------------ compute
+    def _superposed(self) -> bool:
+        """Several particles, or an aberration mask: one pass of the
+        superposition kernel instead of the single-particle kernels."""
+        return len(self.particle) != 1 or getattr(self, 'spherical', 0.) != 0.
+
+    def _particle_table(self, d):
+        """Per-particle parameters and coefficient tables of every
+        constituent (lmz_hologram_superposed's particle arrays)."""
+        t = _lmz.torch()
+        dev = _lmz.device()
+        lib = _lmz.load_library()
+        inst = self.instrument
+        parts = list(self.particle)
+        abs_ = [np.asarray(p.ab(inst.n_m, inst.wavelength), complex) for p in parts]
+        cap = max(max(a.shape[0] - 1 for a in abs_), 1)
+        feats = [_lmz.make_feature(1, 1, (0., 0., 0.), inst, cap, i)
+                 for i in range(len(parts))]
+        pb = _lmz.Batch(feats, 1)
+        rows = np.zeros((len(parts) * cap, 4))
+        pnmax = np.zeros(len(parts), np.int32)
+        pp = np.zeros((len(parts), _lmz.NPARAM))
+        for i, (p, ab) in enumerate(zip(parts, abs_)):
+            n = ab.shape[0] - 1
+            pnmax[i] = n
+            r = rows[i * cap:i * cap + n]
+            r[:, 0], r[:, 1] = ab[1:, 0].real, ab[1:, 0].imag
+            r[:, 2], r[:, 3] = ab[1:, 1].real, ab[1:, 1].imag
+            pp[i, :3] = np.asarray(p.r_p, float) + np.asarray(p.r_0, float)
+        ab_d = t.from_numpy(rows.ravel()).to(dev)
+        nin = t.from_numpy(pnmax).to(dev)
+        pn = t.zeros(len(parts), dtype=t.int32, device=dev)
+        coef = t.zeros(pb.coef_floats(), dtype=t.float32, device=dev)
+        _lmz.check(lib.lmz_coef_from_ab(pb.handle, _lmz.ptr(ab_d), _lmz.ptr(nin),
+                                        _lmz.ptr(pn), _lmz.ptr(coef),
+                                        _lmz.stream_handle()))
+        d['superposed'] = dict(batch=pb, params=t.from_numpy(pp).to(dev), nmax=pn,
+                               coef=coef, cap=cap, n=len(parts),
+                               offset=t.tensor([0, len(parts)], dtype=t.int32,
+                                               device=dev),
+                               alpha=t.tensor([self.alpha], dtype=t.float64,
+                                              device=dev))
+        return d['superposed']
+
+    def _launch_superposed(self, d, flags, out=None, field=None) -> None:
+        sp = self._particle_table(d)
+        _lmz.check(_lmz.load_library().lmz_hologram_superposed(
+            self._batch.handle, sp['n'], _lmz.ptr(sp['offset']),
+            _lmz.ptr(sp['params']), _lmz.ptr(sp['nmax']), _lmz.ptr(sp['coef']),
+            sp['cap'], _lmz.ptr(d['coords']), flags,
+            float(getattr(self, 'spherical', 0.)), _lmz.ptr(sp['alpha']),
+            _lmz.ptr(out), _lmz.ptr(field), _lmz.stream_handle()))
+
     def field(self, cartesian: bool = True, bohren: bool = True,
               device: bool = False):
         """Scattered field alpha E_s, (3, npts) complex.
@@ -224,13 +276,14 @@ class LorenzMie(LMObject):
         lib = _lmz.load_library()
         flags = self._flags(cartesian, bohren)
         stream = _lmz.stream_handle()
-        for i, particle in enumerate(self.particle):
-            self._load_particle(particle, d)
+        if self._superposed():
+            self._launch_superposed(d, flags, field=d['field'])
+        else:
+            self._load_particle(self.particle[0], d)
             _lmz.check(lib.lmz_field(
                 self._batch.handle, _lmz.ptr(d['params']), None,
                 _lmz.ptr(d['nmax']), _lmz.ptr(d['coef']), _lmz.ptr(d['coords']),
-                flags | (_lmz.ACCUMULATE if i else 0), _lmz.ptr(d['field']),
-                stream))
+                flags, _lmz.ptr(d['field']), stream))
         t = _lmz.torch()
         fld = t.view_as_complex(d['field'].view(3, self._npts, 2))
         if device:
@@ -249,17 +302,14 @@ class LorenzMie(LMObject):
         lib = _lmz.load_library()
         flags = self._flags(cartesian, bohren)
         stream = _lmz.stream_handle()
-        if len(self.particle) == 1:
+        if self._superposed():
+            self._launch_superposed(d, flags, out=d['holo'])
+        else:
             self._load_particle(self.particle[0], d)
             _lmz.check(lib.lmz_hologram(
                 self._batch.handle, _lmz.ptr(d['params']), None,
                 _lmz.ptr(d['nmax']), _lmz.ptr(d['coef']), _lmz.ptr(d['coords']),
                 flags, None, _lmz.ptr(d['holo']), None, None, stream))
-        else:
-            self.field(cartesian, bohren, device=True)
-            _lmz.check(lib.lmz_intensity(
-                self._batch.handle, _lmz.ptr(d['params']), _lmz.ptr(d['field']),
-                flags, _lmz.ptr(d['holo']), stream))
         holo = d['holo'][:self._npts]
         if device:
             return holo
@@ -269,8 +319,8 @@ class LorenzMie(LMObject):
         """(I - d) / noise of a single particle on the GPU (float32 tensor),
         the residual of analysis/Optimizer.py:257-260; ``data`` is a float32
         tensor on the device with one value per coordinate."""
-        if len(self.particle) != 1:
-            raise TypeError('residuals() supports a single particle')
+        if self._superposed():
+            raise TypeError('residuals() supports a single unaberrated particle')
         d = self._device_buffers(self._cap())
         self._load_particle(self.particle[0], d)
         lib = _lmz.load_library()

@@ -9,9 +9,13 @@ from .Particle import Particle
 from .Sphere import Sphere
 from .LayeredSphere import LayeredSphere
 from .LorenzMie import LorenzMie, PARAMS
+from .Cluster import Cluster
+from .Dimer import Dimer
+from .Aberrated import Aberrated, AberratedLorenzMie
 
 hipLorenzMie = LorenzMie
 LMHologram = LorenzMie
 
-__all__ = ['Instrument', 'Particle', 'Sphere', 'LayeredSphere', 'LorenzMie', 'hipLorenzMie',
+__all__ = ['Instrument', 'Particle', 'Sphere', 'LayeredSphere', 'Cluster',
+           'Dimer', 'Aberrated', 'AberratedLorenzMie', 'LorenzMie', 'hipLorenzMie',
            'LMHologram', 'PARAMS']

@@ -29,3 +29,8 @@ def fits_golden():
 
 
 INSTRUMENT = dict(n_m=1.34, wavelength=0.447, magnification=0.048)
+
+
+@pytest.fixture(scope='session')
+def superposed_golden():
+    return np.load(os.path.join(GOLDEN, 'superposed.npz'))

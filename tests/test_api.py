@@ -153,3 +153,44 @@ def test_optimizer_rejects_incompatible_model():
     with pytest.raises(TypeError):
         Optimizer(model=Other())
     assert hipOptimizer.method in LorenzMie.method
+
+
+# ---------------------------------------------------------------- Cluster, Dimer
+def test_cluster_moves_constituents():
+    """tests/test_cluster.py of the reference: sequence protocol and r_0."""
+    from pylorenzmie_amd.theory import Cluster
+    p1, p2 = Sphere(x_p=-1.), Sphere(x_p=2.)
+    c = Cluster(x_p=10., y_p=20., z_p=30., particles=[p1, p2])
+    assert len(c) == 2 and c[0] is p1 and list(c) == [p1, p2]
+    with pytest.raises(IndexError):
+        c[2]
+    c.x_p, c.z_p = 11., 31.
+    assert all(tuple(p.r_0) == (11., 20., 31.) for p in c)
+    q = [Sphere(), Sphere(), Sphere()]
+    c.particles = q
+    assert len(c) == 3 and tuple(q[2].r_0) == (11., 20., 31.)
+    assert set(c.properties) == {'x_p', 'y_p', 'z_p'}
+
+
+def test_dimer_geometry_matches_reference(superposed_golden):
+    """tests/test_dimer.py of the reference, and its member positions."""
+    from pylorenzmie_amd.theory import Dimer
+    g = superposed_golden
+    x, y, z, a, n, k, theta, phi = g['dimer_params']
+    d = Dimer(a_p=a, n_p=n, k_p=k, theta=theta, phi=phi, magnification=0.048)
+    assert len(d) == 2 and all(isinstance(p, Sphere) for p in d)
+    np.testing.assert_allclose([p.r_p for p in d.particles], g['dimer_member_rp'])
+    d.a_p = 0.7
+    assert all(p.a_p == 0.7 for p in d)
+    assert np.allclose(d.particles[0].r_p, -d.particles[1].r_p)
+    assert set(d.properties) == {'x_p', 'y_p', 'z_p', 'a_p', 'n_p', 'k_p',
+                                 'theta', 'phi'}
+
+
+def test_aberrated_class():
+    from pylorenzmie_amd.theory import Aberrated, AberratedLorenzMie
+    m = AberratedLorenzMie(spherical=0.5)
+    assert isinstance(m, LorenzMie) and m.properties['spherical'] == 0.5
+    m.properties = dict(spherical=-0.2, z_p=77.)
+    assert m.spherical == -0.2 and m.particle.z_p == 77.
+    assert Aberrated(LorenzMie).__qualname__ == 'AberratedLorenzMie'

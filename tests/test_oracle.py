@@ -204,3 +204,39 @@ def test_jacobian_stencils_agree():
     e = ref.field(c, p, **INSTRUMENT)
     da = 2 * e[0].real + 2 * 0.95 * np.sum(np.abs(e) ** 2, 0)
     assert np.abs(j6[6] - da).max() < 1e-8 * np.abs(da).max()
+
+
+
+# ---- superposition and aberration (reference golden) ------------------------
+def _members(g):
+    centre = g['cluster_centre']
+    return [dict(zip(ref.PARAMS[:6], m), r_0=centre) for m in g['cluster_members']]
+
+
+def test_aberrated_and_cluster_golden(superposed_golden):
+    g = superposed_golden
+    c = g['coords']
+    p = dict(zip(ref.PARAMS[:6], g['aberrated_particle']))
+    for tag, s in (('s09', 0.9), ('sm04', -0.4)):
+        h = ref.hologram(c, p, spherical=s, **INSTRUMENT)
+        assert np.abs(h - g['aberrated_' + tag]).max() < 1e-12
+    # the aberration changes the hologram, and s = 0 is the plain one
+    assert np.abs(g['aberrated_s09'] - ref.hologram(c, p, **INSTRUMENT)).max() > 1e-3
+    members = _members(g)
+    h = ref.hologram(c, members, spherical=0.5, **INSTRUMENT)
+    assert np.abs(h - g['cluster_holo']).max() < 1e-12
+    h = ref.hologram(c, members, **INSTRUMENT)
+    assert np.abs(h - g['cluster_holo_plain']).max() < 1e-12
+
+
+def test_dimer_golden(superposed_golden):
+    g = superposed_golden
+    x, y, z, a, n, k, theta, phi = g['dimer_params']
+    axis = np.array([np.cos(theta) * np.cos(phi), np.cos(theta) * np.sin(phi),
+                     np.sin(theta)])
+    off = (a / INSTRUMENT['magnification']) * axis
+    assert np.allclose(g['dimer_member_rp'], [off, -off])
+    members = [dict(x_p=o[0], y_p=o[1], z_p=o[2], a_p=a, n_p=n, k_p=k,
+                    r_0=(x, y, z)) for o in (off, -off)]
+    h = ref.hologram(g['coords'], members, **INSTRUMENT)
+    assert np.abs(h - g['dimer_holo']).max() < 1e-12

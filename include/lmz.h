@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define LMZ_VERSION 101
+#define LMZ_VERSION 102
 #define LMZ_NPARAM 8       /* stride of params rows */
 #define LMZ_NJAC 7         /* x_p y_p z_p a_p n_p k_p alpha */
 #define LMZ_NRED 36        /* 28 + 7 + 1 */
@@ -339,6 +339,75 @@ int lmz_lm_run_fixed_ab(lmz_lm_plan *plan, const float *coords, int32_t flags,
                         const double *ab, const int32_t *nmax,
                         double *params_io, double *result, void *workspace,
                         hipStream_t stream);
+
+/* ------------------------------------------------------------------ *
+ * Frames resident in device memory: background normalization and the
+ * crops that feed a batch.
+ * ------------------------------------------------------------------ */
+
+/* pixel types of frames, backgrounds and normalized output */
+#define LMZ_U8 0
+#define LMZ_U16 1
+#define LMZ_F32 2
+#define LMZ_F64 3
+#define LMZ_MAX_FILTER_SIZE 255   /* widest median-filter window */
+
+/* Bytes of scratch lmz_median needs (8-byte aligned device memory). */
+size_t lmz_median_scratch_bytes(void);
+
+/* numpy.median of n pixels of type dtype, written as a double to the device
+ * scalar *median: the mean of the two middle values for even n (float32
+ * arithmetic for LMZ_F32, float64 otherwise), NaN when a pixel is NaN or
+ * n == 0.  Exact radix select in device memory; no host synchronization.
+ * Replaces np.median(image) of utilities/normalizer.py:74. */
+int lmz_median(const void *image, int32_t dtype, int64_t n, double *median,
+               void *scratch, hipStream_t stream);
+
+/* scipy.ndimage.median_filter(image, size=size) with its default 'reflect'
+ * boundary on an (ny, nx) row-major frame: the element of rank
+ * size*size // 2 of the size x size window whose offsets run from -(size//2)
+ * to size - size//2 - 1.  Writes the background (type dtype) and/or
+ * out = image / where(background > 0, background, 1.) as LMZ_F32 or
+ * LMZ_F64 (the exact quotient rounded once).  1 <= size <=
+ * LMZ_MAX_FILTER_SIZE.  Replaces utilities/normalizer.py:71-72 and 75-76. */
+int lmz_median_filter(const void *image, int32_t dtype, int32_t ny, int32_t nx,
+                      int32_t size, void *background, void *out,
+                      int32_t out_dtype, hipStream_t stream);
+
+/* out = image / where(b > 0, b, 1.) over n pixels, b being the device scalar
+ * *background_scalar (from lmz_median), else the image reference[] of type
+ * ref_dtype, else the host scalar ref_scalar.  The quotient of the exact
+ * operands rounded once to out_dtype, which equals numpy's result in its
+ * result type.  Replaces utilities/normalizer.py:64-76 ('median' and
+ * 'reference'). */
+int lmz_normalize(const void *image, int32_t dtype, int64_t n,
+                  const double *background_scalar, const void *reference,
+                  int32_t ref_dtype, double ref_scalar, void *out,
+                  int32_t out_dtype, hipStream_t stream);
+
+/* Pixelwise median of three 8-bit frames frames[3][n] into out[n]: one
+ * level of the running background median of utilities/vmedian.py:61-75. */
+int lmz_median3_u8(const uint8_t *frames, int64_t n, uint8_t *out,
+                   hipStream_t stream);
+
+/* Fill a batch's packed data (and explicit coordinates) from an (ny, nx)
+ * float32 frame in device memory, without host copies.
+ *   xs, ys double [nx], [ny] device: the frame's column and row
+ *          coordinates (lib/meshgrid.py:10-40); needed only for
+ *          explicit-coordinate features
+ *   box    int32 [nfeat][4] device: x0, y0, w, h of each feature's crop in
+ *          the frame (the Hologram slice of analysis/Hologram.py:92-108)
+ *   index  int32 [npix_total] device, for explicit-coordinate features:
+ *          the crop-local row-major index of each pixel (the pixels a Mask
+ *          picks, analysis/Mask.py:82-105), NULL if every feature is dense
+ *   coords float [3][npix_total] (out) for explicit-coordinate features:
+ *          (xs[x], ys[y], 0) - origin, rounded once to float
+ * A dense feature's width must equal its box width.  Pixels outside the
+ * frame or the box read as NaN. */
+int lmz_gather_frame(const lmz_batch *batch, const float *frame, int32_t ny,
+                     int32_t nx, const double *xs, const double *ys,
+                     const int32_t *box, const int32_t *index, float *data,
+                     float *coords, hipStream_t stream);
 
 const char *lmz_last_error(void);
 int lmz_version(void);

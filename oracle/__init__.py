@@ -15,6 +15,9 @@ lorenzmie
 fit
     The reference's fitter semantics (scipy MINPACK LM with the
     reference's default settings, residual scaling and SVD uncertainties).
+normalize
+    Background normalization of a frame (np.median, scipy's median filter,
+    a reference image) and the running median of a video.
 independent
     Textbook restatements that share no code path with ``lorenzmie``
     (Bessel functions from scipy.special, Legendre functions from
@@ -36,4 +39,9 @@ Parity pins (what checks each function; see tests/test_oracle.py):
 * jacobian -- agreement of 5-point and 7-point stencils, and
   directional derivatives of the independent model.
 * fit -- truth recovery on synthetic data and golden reference fits.
+* normalize -- the filter against its definition (explicit reflected
+  windows and an exact rank, median_filter_brute) at odd, even and
+  frame-exceeding sizes; the result types against numpy's promotion of
+  the reference's expression; VMedian against np.median of the frames it
+  has seen (order 0) and the median of medians (order 1).
 """

@@ -26,6 +26,12 @@ CARTESIAN = 1
 BOHREN = 2
 ACCUMULATE = 4
 FP32 = 8
+MAX_FILTER_SIZE = 255
+
+# pixel types of frames (LMZ_U8 ... LMZ_F64)
+U8, U16, F32, F64 = 0, 1, 2, 3
+NP_DTYPE_CODE = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16,
+                 np.dtype(np.float32): F32, np.dtype(np.float64): F64}
 
 LM_RUNNING, LM_FTOL, LM_XTOL, LM_GTOL = 0, 1, 2, 4
 LM_MAXITER, LM_STALLED, LM_NONFINITE = 8, 16, 32
@@ -119,6 +125,13 @@ SIGNATURES = {
                         _P, _P]),
     'lmz_lm_run_fixed_ab': (_I, [_P, _P, _I, _P, ctypes.POINTER(LMSettings),
                                  _P, _P, _P, _P, _P, _P]),
+    'lmz_median_scratch_bytes': (ctypes.c_size_t, []),
+    'lmz_median': (_I, [_P, _I, ctypes.c_int64, _P, _P, _P]),
+    'lmz_median_filter': (_I, [_P, _I, _I, _I, _I, _P, _P, _I, _P]),
+    'lmz_normalize': (_I, [_P, _I, ctypes.c_int64, _P, _P, _I, ctypes.c_double,
+                           _P, _I, _P]),
+    'lmz_median3_u8': (_I, [_P, ctypes.c_int64, _P, _P]),
+    'lmz_gather_frame': (_I, [_P, _P, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     'lmz_last_error': (ctypes.c_char_p, []),
     'lmz_version': (_I, []),
 }

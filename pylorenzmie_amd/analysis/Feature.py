@@ -14,7 +14,7 @@ import pandas as pd
 from ..theory import LorenzMie
 from .DEEstimator import DEEstimator
 from .Hologram import Hologram
-from .Mask import Mask
+from .Mask import Mask, host_data
 from .Optimizer import Optimizer
 
 
@@ -29,9 +29,8 @@ class Feature(Hologram):
         self._coordinates = hologram._coordinates
         self.model = model or LorenzMie()
         m = Mask()
-        m.shape = self.data.shape
-        m.exclude = ((self.data == np.max(self.data)) | np.isnan(self.data) |
-                     np.isinf(self.data))
+        m.shape = tuple(self.data.shape)
+        m.exclude = _bad_pixels(self.data)
         self.estimator = DEEstimator(model=self.model, exclude=m.exclude)
         self.optimizer = Optimizer(model=self.model, mask=m)
         if fixed is not None:
@@ -71,4 +70,14 @@ class Feature(Hologram):
         return self.model.hologram().reshape(self.shape)
 
     def residuals(self) -> np.ndarray:
-        return self.predicted() - self.data
+        return self.predicted() - host_data(self.data)
+
+
+def _bad_pixels(data) -> np.ndarray:
+    """Saturated (== max), NaN and inf pixels (analysis/Feature.py:36-43);
+    for a crop in device memory the test runs there and only the boolean
+    mask comes back."""
+    if hasattr(data, 'is_cuda') and data.is_cuda:
+        import torch
+        return ((data == data.max()) | ~torch.isfinite(data)).cpu().numpy()
+    return (data == np.max(data)) | np.isnan(data) | np.isinf(data)

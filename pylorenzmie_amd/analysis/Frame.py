@@ -10,6 +10,12 @@ optimizer's variables and settings; each group is one batch.  Fitted
 values are written back to every feature's model and optimizer, so
 ``feature.optimizer.result`` and ``report()`` work as after a single fit.
 
+The frame may live in device memory (a torch tensor on the GPU, for
+instance the output of ``utilities.Normalizer`` on a camera frame already
+there): its features' pixels then go from the frame to the batch on the
+device (lmz_gather_frame), and only the boxes and the picked pixel indices
+come from the host.
+
 Feature detection (the reference's Localizer, which needs trackpy) is
 outside this library: set ``bboxes`` to place features.
 """
@@ -23,6 +29,8 @@ import pandas as pd
 from .. import _lmz
 from ..theory import Instrument, LorenzMie, Sphere
 from ..theory.LorenzMie import PARAMS
+from ..lib import meshgrid
+from ..utilities.normalizer import is_device_tensor
 from .Feature import Feature
 from .Hologram import Hologram
 from .lm import LMBatch
@@ -54,18 +62,32 @@ class Frame(Hologram):
     @data.setter
     def data(self, data) -> None:
         """Setting data clears features, boxes and results
-        (analysis/Frame.py:63-80)."""
+        (analysis/Frame.py:63-80).  A device tensor stays on the device as
+        float32 (the type the fitter reads)."""
         self._features = []
         self._bboxes = []
         self._results = pd.DataFrame()
-        self._data = None if data is None else np.asarray(data)
-        if self._data is not None:
-            h = Hologram(self._data, corner=self.corner)
-            self._coordinates = h._coordinates
+        if data is None:
+            self._data = None
+            self._coordinates = None
+            return
+        if is_device_tensor(data):
+            import torch
+            if data.dim() != 2:
+                raise ValueError('frame data must be 2-D')
+            self._data = data.to(torch.float32).contiguous()
+        else:
+            self._data = np.asarray(data)
+        self._coordinates = meshgrid(tuple(self._data.shape), corner=self.corner,
+                                     flatten=False)
 
     @property
     def shape(self):
-        return None if self._data is None else self._data.shape
+        return None if self._data is None else tuple(self._data.shape)
+
+    @property
+    def on_device(self) -> bool:
+        return is_device_tensor(self._data)
 
     @property
     def results(self) -> pd.DataFrame:
@@ -94,10 +116,18 @@ class Frame(Hologram):
             self._features.append(feature)
 
     def __getitem__(self, key) -> Feature:
-        """A Feature on the crop ``key`` with this frame's instrument."""
+        """A Feature on the crop ``key`` with this frame's instrument; it
+        remembers its box in the frame."""
+        sy, sx = key
+        ny, nx = self.shape
+        y0, y1, ystep = sy.indices(ny)
+        x0, x1, xstep = sx.indices(nx)
         crop = Hologram.__getitem__(self, key)
         model = LorenzMie(particle=Sphere(), instrument=self.instrument)
-        return Feature(crop, model=model)
+        feature = Feature(crop, model=model)
+        if ystep == 1 and xstep == 1 and y1 > y0 and x1 > x0:
+            feature._box = (x0, y0, x1 - x0, y1 - y0)
+        return feature
 
     # ------------------------------------------------------------ pipeline
     def detect(self) -> int:
@@ -118,8 +148,10 @@ class Frame(Hologram):
             key = (tuple(opt.variables), tuple(sorted(opt.settings.items())))
             groups.setdefault(key, []).append(i)
         rows: list = [None] * len(self._features)
+        source = self if self.on_device else None
         for idx in groups.values():
-            for i, row in zip(idx, _fit_batch([self._features[i] for i in idx])):
+            group = [self._features[i] for i in idx]
+            for i, row in zip(idx, _fit_batch(group, source)):
                 rows[i] = row
         self._results = pd.DataFrame(rows)
         return self._results
@@ -132,27 +164,69 @@ class Frame(Hologram):
         return self.optimize()
 
 
-def _fit_batch(features: list[Feature]) -> list[pd.Series]:
+def _select_on_frame(feature: Feature) -> dict:
+    """The pixels a feature's optimizer fits, as a box and crop-local
+    indices in the frame: the same mask draw and model coordinates as
+    Optimizer._select (analysis/Optimizer.py:211-215), without the data."""
+    opt, model = feature.optimizer, feature.model
+    box = getattr(feature, '_box', None)
+    if box is None:
+        raise ValueError('a feature of a device frame must be a box slice of it')
+    if opt.mask is None:
+        model.coordinates = feature.flat_coordinates
+        index = None
+    else:
+        m = opt.mask
+        m.shape = tuple(feature.shape)          # draws a fresh subset
+        picked = m()
+        index = np.flatnonzero(picked).astype(np.int32)
+        model.coordinates = feature.coordinates[:, picked]
+    npix = model._npts
+    if model._width > 0:
+        # the picked pixels form a full grid (all of the box, or a
+        # rectangle inside it): a dense crop of that rectangle
+        i0 = int(index[0]) if index is not None else 0
+        x0, y0, w, _ = box
+        box = (x0 + i0 % w, y0 + i0 // w, model._width, npix // model._width)
+        index = None
+    elif index is None:
+        index = np.arange(npix, dtype=np.int32)
+    opt._data = None                # the pixels stay in device memory
+    opt._count = npix
+    return dict(box=box, npix=npix, index=index)
+
+
+def _fit_batch(features: list[Feature], frame: Frame | None = None
+               ) -> list[pd.Series]:
     """One LMBatch over the features' masked pixels; results written back
-    to each feature's model and optimizer."""
+    to each feature's model and optimizer.  With ``frame`` (in device
+    memory) the pixels are gathered there."""
     opt0 = features[0].optimizer
     if not opt0._use_device_lm():
         return [f.optimize() for f in features]
     crops, p0 = [], []
     for f in features:
         opt, model = f.optimizer, f.model
-        opt._select(f)                        # mask pick, model.coordinates
+        if frame is None:
+            opt._select(f)                    # mask pick, model.coordinates
+            crop = dict(data=np.asarray(opt._data, np.float32),
+                        coords=model._rel)
+        else:
+            crop = _select_on_frame(f)
         sph, inst = model.particle, model.instrument
-        crops.append(dict(data=np.asarray(opt._data, np.float32),
-                          width=model._width, origin=model._origin,
-                          coords=model._rel, noise=inst.noise,
-                          nmax_cap=_lmz.nmax_cap(sph.a_p, sph.n_p, sph.k_p,
-                                                 inst.n_m, inst.wavelength)))
+        crop.update(width=model._width, origin=model._origin, noise=inst.noise,
+                    nmax_cap=_lmz.nmax_cap(sph.a_p, sph.n_p, sph.k_p,
+                                           inst.n_m, inst.wavelength))
+        crops.append(crop)
         props = model.properties
         p = np.array([props[k] for k in PARAMS], float)
         p[:3] += sph.r_0
         p0.append(p)
-    lm = LMBatch(crops, features[0].model.instrument)
+    source = None
+    if frame is not None:
+        xy = frame._coordinates
+        source = (frame.data, xy[0, 0, :], xy[1, :, 0])
+    lm = LMBatch(crops, features[0].model.instrument, frame=source)
     lm.load(np.array(p0))
     lm.run(opt0._lm_settings(), opt0.variables)
     params, res = lm.fetch()

@@ -55,6 +55,14 @@ class Mask(LMObject):
             self._mask[self.exclude] = False
 
     def apply(self, hologram: Hologram):
-        self.shape = hologram.shape
+        self.shape = tuple(hologram.shape)
         m = self._mask
-        return hologram.data[m], hologram.coordinates[:, m]
+        return host_data(hologram.data)[m], hologram.coordinates[:, m]
+
+
+def host_data(data) -> np.ndarray:
+    """The pixels of a crop as a numpy array (a copy when they live in
+    device memory, for the single-feature fits and estimates)."""
+    if hasattr(data, 'is_cuda') and data.is_cuda:
+        return data.cpu().numpy()
+    return np.asarray(data)

@@ -26,7 +26,7 @@ from ..lib import LMObject
 from ..theory import LorenzMie, Sphere
 from ..theory.LorenzMie import PARAMS
 from .Hologram import Hologram
-from .Mask import Mask
+from .Mask import Mask, host_data
 from .lm import LMBatch, LMSettings, stats_from_jtj
 
 _DEFAULT_FIXED = ('wavelength magnification numerical_aperture n_m noise '
@@ -56,6 +56,7 @@ class Optimizer(LMObject):
         self.fixed = fixed if fixed is not None else list(_DEFAULT_FIXED)
         self.robust = robust
         self._data = None
+        self._count = 0             # pixels of the last fit
         self._result = None
         self._lm_cache = None
 
@@ -130,7 +131,7 @@ class Optimizer(LMObject):
         keys.extend('success npix redchi'.split())
         redchi, unc = self._statistics()
         values = list(sum(zip(list(self._result.x), unc), ()))
-        values.extend([self._result.success, self._data.size, redchi])
+        values.extend([self._result.success, self._count, redchi])
         return pd.Series(dict(zip(keys, values)))
 
     @property
@@ -158,10 +159,11 @@ class Optimizer(LMObject):
     def _select(self, hologram: Hologram) -> None:
         """analysis/Optimizer.py:211-215."""
         if self.mask is None:
-            self._data = hologram.flat_data
+            self._data = host_data(hologram.data).ravel()
             self.model.coordinates = hologram.flat_coordinates
         else:
             self._data, self.model.coordinates = self.mask.apply(hologram)
+        self._count = self._data.size
 
     def _use_device_lm(self) -> bool:
         s = self.settings
@@ -240,7 +242,7 @@ class Optimizer(LMObject):
 
     def _statistics(self):
         res = self._result
-        npix = self._data.size
+        npix = self._count
         if getattr(res, 'jtj', None) is not None:
             return stats_from_jtj(res.jtj, 2. * res.cost, npix, res.vary)
         # analysis/Optimizer.py:262-278

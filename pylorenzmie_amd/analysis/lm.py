@@ -63,9 +63,15 @@ class LMBatch:
         width (0 for explicit coordinates), origin (x, y, z),
         coords (3, npts) float array relative to origin when width == 0,
         noise, nmax_cap.
+    ``frame``: (tensor, xs, ys) -- a float32 (ny, nx) frame in device memory
+        with its column and row coordinates; the crops then give
+        box (x0, y0, w, h), npix and, for width == 0, index (the crop-local
+        row-major indices of their pixels) instead of data and coords, and
+        lmz_gather_frame fills the batch on the device.
     """
 
-    def __init__(self, crops: list[dict], instrument, flags: int = 3):
+    def __init__(self, crops: list[dict], instrument, flags: int = 3,
+                 frame=None):
         t = _lmz.torch()
         dev = _lmz.device()
         feats = []
@@ -73,14 +79,17 @@ class LMBatch:
         datas, coords = [], []
         self.npix = []
         for c in crops:
-            data = np.asarray(c['data'], np.float32).ravel()
-            n = data.size
+            if frame is None:
+                data = np.asarray(c['data'], np.float32).ravel()
+                n = data.size
+            else:
+                data, n = None, int(c['npix'])
             feats.append(_lmz.make_feature(n, c.get('width', 0),
                                            c.get('origin', (0., 0., 0.)),
                                            instrument, c['nmax_cap'], off,
                                            noise=c.get('noise')))
             datas.append(data)
-            if c.get('width', 0) == 0:
+            if c.get('width', 0) == 0 and frame is None:
                 coords.append(np.asarray(c['coords'], np.float32).reshape(3, n))
             else:
                 coords.append(None)
@@ -89,17 +98,20 @@ class LMBatch:
         self.batch = _lmz.Batch(feats, 2)
         self.nfeat = len(crops)
         self.flags = flags
-        self.data = t.from_numpy(np.concatenate(datas)).to(dev)
-        if any(c is not None for c in coords):
-            allc = np.zeros((3, off), np.float32)
-            o = 0
-            for c, n in zip(coords, self.npix):
-                if c is not None:
-                    allc[:, o:o + n] = c
-                o += n
-            self.coords = t.from_numpy(allc).to(dev)
+        if frame is not None:
+            self._gather(crops, frame, off)
         else:
-            self.coords = None
+            self.data = t.from_numpy(np.concatenate(datas)).to(dev)
+            if any(c is not None for c in coords):
+                allc = np.zeros((3, off), np.float32)
+                o = 0
+                for c, n in zip(coords, self.npix):
+                    if c is not None:
+                        allc[:, o:o + n] = c
+                    o += n
+                self.coords = t.from_numpy(allc).to(dev)
+            else:
+                self.coords = None
         nbytes = self.batch.lm_workspace_bytes()
         self.workspace = t.empty(nbytes + 256, dtype=t.uint8, device=dev)
         base = self.workspace.data_ptr()
@@ -112,6 +124,51 @@ class LMBatch:
         _lmz.check(_lmz.load_library().lmz_lm_plan_create(
             self.batch.handle, ctypes.byref(plan)))
         self.plan = plan
+
+    def _gather(self, crops, frame, npix_total: int) -> None:
+        """Data (and explicit coordinates) of every feature read from the
+        device frame by lmz_gather_frame; only the boxes and the picked
+        pixel indices travel from the host."""
+        t = _lmz.torch()
+        image, xs, ys = frame
+        if image.dtype != t.float32 or image.dim() != 2 or not image.is_cuda:
+            raise TypeError('frame must be a float32 (ny, nx) device tensor')
+        image = image.contiguous()
+        dev = image.device
+        ny, nx = image.shape
+        if len(xs) != nx or len(ys) != ny:
+            raise ValueError('xs and ys must hold the frame\'s nx column and '
+                             'ny row coordinates')
+        box = np.zeros((self.nfeat, 4), np.int32)
+        index = None
+        explicit = any(c.get('width', 0) == 0 for c in crops)
+        if explicit:
+            index = np.zeros(npix_total, np.int32)
+        o = 0
+        for i, (c, n) in enumerate(zip(crops, self.npix)):
+            x0, y0, w, h = (int(v) for v in c['box'])
+            if not (0 <= x0 and x0 + w <= nx and 0 <= y0 and y0 + h <= ny):
+                raise ValueError(f'box {c["box"]} is outside the {nx}x{ny} frame')
+            box[i] = (x0, y0, w, h)
+            if c.get('width', 0) == 0:
+                idx = np.asarray(c['index'], np.int64)
+                if idx.size != n or (n and (idx.min() < 0 or idx.max() >= w * h)):
+                    raise ValueError('index must hold npix crop-local indices')
+                index[o:o + n] = idx
+            elif int(c['width']) != w or n != w * h:
+                raise ValueError('a dense crop must cover its box')
+            o += n
+        self.data = t.empty(npix_total, dtype=t.float32, device=dev)
+        self.coords = (t.empty((3, npix_total), dtype=t.float32, device=dev)
+                       if explicit else None)
+        self._box = t.from_numpy(box).to(dev)
+        self._index = t.from_numpy(index).to(dev) if explicit else None
+        self._xs = t.from_numpy(np.ascontiguousarray(xs, np.float64)).to(dev)
+        self._ys = t.from_numpy(np.ascontiguousarray(ys, np.float64)).to(dev)
+        _lmz.check(_lmz.load_library().lmz_gather_frame(
+            self.batch.handle, _lmz.ptr(image), ny, nx, _lmz.ptr(self._xs),
+            _lmz.ptr(self._ys), _lmz.ptr(self._box), _lmz.ptr(self._index),
+            _lmz.ptr(self.data), _lmz.ptr(self.coords), _lmz.stream_handle()))
 
     def close(self):
         if getattr(self, 'plan', None):

@@ -34,3 +34,8 @@ INSTRUMENT = dict(n_m=1.34, wavelength=0.447, magnification=0.048)
 @pytest.fixture(scope='session')
 def superposed_golden():
     return np.load(os.path.join(GOLDEN, 'superposed.npz'))
+
+
+@pytest.fixture(scope='session')
+def frame_golden():
+    return np.load(os.path.join(GOLDEN, 'frame.npz'))

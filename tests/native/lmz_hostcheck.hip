@@ -1,10 +1,13 @@
 // Host build of the per-pixel and per-feature arithmetic in
-// pylorenzmie_amd/csrc/lmz_pixel.h, for CPU tests of the kernel formulas
-// against the FP64 oracle.  Test instrument only: the library's product
+// pylorenzmie_amd/csrc/lmz_pixel.h and the frame algorithms in
+// pylorenzmie_amd/csrc/lmz_frame.h, for CPU tests of the kernel formulas
+// against the FP64 oracle, numpy and scipy.  Test instrument only: the library's product
 // path is the gfx950 kernels in liblmz.so.
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
+#include "../../pylorenzmie_amd/csrc/lmz_frame.h"
 #include "../../pylorenzmie_amd/csrc/lmz_pixel.h"
 
 using namespace lmz;
@@ -87,5 +90,127 @@ void chk_dense_coords(int width, int npix, float *ax, float *ay) {
         pixel_coords(fu, j, nullptr, 0, ax[j], ay[j], z);
     }
 }
+
+}  // extern "C"
+
+namespace {
+
+struct HostHist {
+    uint32_t w[128];
+    void add(uint32_t i, uint32_t v) { w[i] += v; }
+    uint32_t word(int i) const { return w[i]; }
+    void clear() {
+        for (int i = 0; i < 128; ++i) w[i] = 0u;
+    }
+};
+
+// The radix-select median filter as the kernel runs it: 8 x 8 output tiles,
+// selection from the first digit where the tile's keys differ.
+template <int DT>
+void medfilt_radix_host(const void *image, int ny, int nx, int size, void *bgv) {
+    using P = Pix<DT>;
+    using K = typename P::K;
+    using T = typename P::T;
+    const T *img = (const T *)image;
+    T *bg = (T *)bgv;
+    const int lo = size / 2, span = 7 + size;
+    HostHist h;
+    auto key_at = [&](int i, int j) {
+        return P::key(img[(long long)reflect_index(i, ny) * nx + reflect_index(j, nx)]);
+    };
+    for (int by = 0; by < ny; by += 8)
+        for (int bx = 0; bx < nx; bx += 8) {
+            const K kref = key_at(by - lo, bx - lo);
+            unsigned long long diff = 0ull;
+            for (int i = 0; i < span; ++i)
+                for (int j = 0; j < span; ++j)
+                    diff |= (unsigned long long)(key_at(by - lo + i, bx - lo + j) ^ kref);
+            for (int ty = 0; ty < 8; ++ty)
+                for (int tx = 0; tx < 8; ++tx) {
+                    const int x = bx + tx, y = by + ty;
+                    if (x >= nx || y >= ny) continue;
+                    K key = kref;
+                    if (diff) {
+                        const int hb = 63 - __builtin_clzll(diff);
+                        const int top = (hb / 8) * 8;
+                        const K prefix = (top + 8 >= (int)(8 * sizeof(K)))
+                                             ? (K)0 : (K)(kref & (~(K)0 << (top + 8)));
+                        auto fetch = [&](int r, int c) {
+                            return key_at(y - lo + r, x - lo + c);
+                        };
+                        key = window_select<K>(fetch, size, size * size / 2,
+                                               prefix, top, h);
+                    }
+                    bg[(long long)y * nx + x] = P::value(key);
+                }
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+// scipy.ndimage.median_filter by the device algorithms: path 0 the sliding
+// histogram (8-bit frames), path 1 the radix select.
+int chk_median_filter(const void *image, int dt, int ny, int nx, int size,
+                      void *bg, int path) {
+    if (path == 0) {
+        if (dt != LMZ_U8) return 1;
+        const uint8_t *img = (const uint8_t *)image;
+        uint8_t *out = (uint8_t *)bg;
+        const int lo = size / 2, rank = size * size / 2;
+        HostHist h;
+        for (int x = 0; x < nx; ++x)
+            for (int y0 = 0; y0 < ny; y0 += 32) {
+                auto px = [&](int yy, int c) {
+                    return img[(long long)reflect_index(yy, ny) * nx +
+                               reflect_index(x - lo + c, nx)];
+                };
+                h.clear();
+                for (int r = 0; r < size; ++r)
+                    for (int c = 0; c < size; ++c) hist_add(h, px(y0 - lo + r, c));
+                int below;
+                int m = hist_find(h, rank, &below);
+                out[(long long)y0 * nx + x] = (uint8_t)m;
+                for (int y = y0 + 1; y < std::min(y0 + 32, ny); ++y) {
+                    for (int c = 0; c < size; ++c) {
+                        slide_remove(h, px(y - 1 - lo, c), m, below);
+                        slide_add(h, px(y - 1 - lo + size, c), m, below);
+                    }
+                    slide_settle(h, rank, m, below);
+                    out[(long long)y * nx + x] = (uint8_t)m;
+                }
+            }
+        return 0;
+    }
+    switch (dt) {
+    case LMZ_U8: medfilt_radix_host<LMZ_U8>(image, ny, nx, size, bg); break;
+    case LMZ_U16: medfilt_radix_host<LMZ_U16>(image, ny, nx, size, bg); break;
+    case LMZ_F32: medfilt_radix_host<LMZ_F32>(image, ny, nx, size, bg); break;
+    default: medfilt_radix_host<LMZ_F64>(image, ny, nx, size, bg); break;
+    }
+    return 0;
+}
+
+// Order-preserving key of a value and back (lmz_frame.h's Pix<dt>).
+unsigned long long chk_key(int dt, double v) {
+    switch (dt) {
+    case LMZ_U8: return Pix<LMZ_U8>::key((uint8_t)v);
+    case LMZ_U16: return Pix<LMZ_U16>::key((uint16_t)v);
+    case LMZ_F32: return Pix<LMZ_F32>::key((float)v);
+    default: return Pix<LMZ_F64>::key(v);
+    }
+}
+
+double chk_mean_of_two(int dt, double a, double b) {
+    switch (dt) {
+    case LMZ_U8: return mean_of_two<LMZ_U8>((uint8_t)a, (uint8_t)b);
+    case LMZ_U16: return mean_of_two<LMZ_U16>((uint16_t)a, (uint16_t)b);
+    case LMZ_F32: return mean_of_two<LMZ_F32>((float)a, (float)b);
+    default: return mean_of_two<LMZ_F64>(a, b);
+    }
+}
+
+int chk_reflect(long long i, long long n) { return reflect_index(i, n); }
 
 }  // extern "C"

@@ -33,7 +33,7 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_version(lib):
-    assert lib.lmz_version() == 101
+    assert lib.lmz_version() == 102
 
 
 def test_struct_layouts_match_header():
@@ -84,3 +84,29 @@ def test_calls_reject_null_batch(lib):
     assert lib.lmz_autotune(None, *([None] * 5), 3, None, None, 0, None, None) == 1
     assert lib.lmz_mie_ab_layered(None, *([None] * 7)) == 1
     assert lib.lmz_hologram_vjp(None, *([None] * 5), 3, *([None] * 4)) == 1
+
+
+def test_frame_calls_reject_bad_arguments(lib):
+    """Frame entry points check their arguments on the host, before any
+    launch."""
+    P = ctypes.c_void_p
+    fake = P(256)                   # never dereferenced: rejected first
+    assert lib.lmz_median_scratch_bytes() >= 256 * 4 + 6 * 8
+    assert lib.lmz_median(fake, 7, 10, fake, fake, None) == 1
+    assert lib.lmz_median(fake, 2, -1, fake, fake, None) == 1
+    assert lib.lmz_median(fake, 2, 10, None, fake, None) == 1
+    assert lib.lmz_median(fake, 2, 10, fake, P(257), None) == 1
+    for ny, nx, size, odt in ((0, 5, 3, 2), (5, 5, 0, 2), (5, 5, 256, 2),
+                              (5, 5, 3, 0)):
+        assert lib.lmz_median_filter(fake, 0, ny, nx, size, None, fake, odt,
+                                     None) == 1
+    assert lib.lmz_median_filter(fake, 0, 5, 5, 3, None, None, 2, None) == 1
+    assert lib.lmz_normalize(fake, 0, 10, fake, fake, 0, 0., fake, 2, None) == 1
+    assert lib.lmz_normalize(fake, 0, 10, None, None, 0, 1., fake, 1, None) == 1
+    assert lib.lmz_normalize(fake, 0, 10, None, fake, 9, 1., fake, 2, None) == 1
+    assert lib.lmz_normalize(None, 0, 0, None, None, 0, 1., None, 2, None) == 0
+    assert lib.lmz_median3_u8(None, 5, None, None) == 1
+    assert lib.lmz_median3_u8(None, 0, None, None) == 0
+    assert lib.lmz_gather_frame(None, fake, 5, 5, None, None, fake, None, fake,
+                                None, None) == 1
+    assert b'batch' in lib.lmz_last_error()

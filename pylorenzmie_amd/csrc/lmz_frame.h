@@ -17,12 +17,13 @@
 namespace lmz {
 
 // scipy.ndimage mode 'reflect' (d c b a | a b c d | d c b a) for any i,
-// including windows wider than the image (period 2n).
-__host__ __device__ inline int reflect_index(long long i, long long n) {
-    const long long p = 2 * n;
-    long long m = i % p;
+// including windows wider than the image (period 2n).  n < 2^30.
+__host__ __device__ inline int reflect_index(int i, int n) {
+    if (i >= 0 && i < n) return i;
+    const int p = 2 * n;
+    int m = i % p;
     if (m < 0) m += p;
-    return (int)(m < n ? m : p - 1 - m);
+    return m < n ? m : p - 1 - m;
 }
 
 // Pixel types: storage T, key K (unsigned, same order as the values; every
@@ -155,11 +156,23 @@ __host__ __device__ inline int hist_find(Hist &h, int rank, int *below) {
     return 255;   // not reached for rank < total
 }
 
+// Count key k in digit (k >> s) & 255 if it agrees with prefix under
+// himask; otherwise add 0 (no branch, so that loads can run ahead).
+template <class K, class Hist>
+__host__ __device__ inline void count_key(Hist &h, K k, K prefix, K himask,
+                                          int s) {
+    const uint32_t d = (uint32_t)((k >> s) & 0xFFu);
+    const uint32_t inc = ((k ^ prefix) & himask) == 0 ? (1u << ((d & 1u) << 4))
+                                                      : 0u;
+    h.add(d >> 1, inc);
+}
+
 // Key of rank `rank` (0-based) among the keys fetch(r, c), r, c in
 // [0, size): radix select over 8-bit digits, most significant first.  The
 // keys are known to agree above bit top_shift + 8 (they all equal `prefix`
 // there), so selection starts at digit top_shift.  Counts fit the 16-bit
-// halves for size <= 255.
+// halves for size <= 255.  Keys are fetched eight at a time before they
+// are counted.
 template <class K, class Fetch, class Hist>
 __host__ __device__ inline K window_select(const Fetch &fetch, int size,
                                            int rank, K prefix, int top_shift,
@@ -169,12 +182,15 @@ __host__ __device__ inline K window_select(const Fetch &fetch, int size,
                              ? (K)0 : (K)(~(K)0 << (s + 8));
         h.clear();
         for (int r = 0; r < size; ++r) {
-#pragma unroll 4
-            for (int c = 0; c < size; ++c) {
-                const K k = fetch(r, c);
-                if (((k ^ prefix) & himask) == 0)
-                    hist_add(h, (uint32_t)((k >> s) & 0xFFu));
+            int c = 0;
+            for (; c + 8 <= size; c += 8) {
+                K k[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) k[j] = fetch(r, c + j);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) count_key(h, k[j], prefix, himask, s);
             }
+            for (; c < size; ++c) count_key(h, fetch(r, c), prefix, himask, s);
         }
         int below;
         const int b = hist_find(h, rank, &below);

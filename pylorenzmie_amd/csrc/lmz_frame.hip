@@ -178,44 +178,111 @@ __device__ inline void emit(const T *img, int nx, int x, int y, T med,
 }
 
 // Median filter of an 8-bit frame (scipy.ndimage.median_filter, mode
-// 'reflect'): one thread per output column walks FILTER_ROWS rows down,
-// keeping the window's 256-bin histogram; a step removes the row leaving
-// the window and adds the row entering it (2 * size values instead of
-// size^2) and moves the median bin by the change.
-constexpr int FILTER_ROWS = 32;
+// 'reflect'): one thread per output column walks `rows` rows down, keeping
+// the window's 256-bin histogram; a step removes the row leaving the window
+// and adds the row entering it (2 * size values instead of size^2) and
+// moves the median bin by the change.  The block's 64 + size - 1 columns of
+// each row come through LDS (reflected once, in colmap); the next step's
+// rows are loaded into registers while the current ones are counted.
+constexpr int SEG_MAX = 64 + LMZ_MAX_FILTER_SIZE - 1;
+constexpr int SEG_PER_LANE = (SEG_MAX + 63) / 64;
 
 __global__ __launch_bounds__(64) void k_medfilt_u8(
     const uint8_t *__restrict__ img, int ny, int nx, int size, int lo,
-    uint8_t *__restrict__ bg, void *__restrict__ out, int odt) {
+    int rows, uint8_t *__restrict__ bg, void *__restrict__ out, int odt) {
     __shared__ uint32_t hist[128 * 64];
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y0 = blockIdx.y * FILTER_ROWS;
-    if (x >= nx || y0 >= ny) return;
-    const int y1 = min(y0 + FILTER_ROWS, ny);
-    LdsHist h{hist + threadIdx.x};
-    const int xl = x - lo;
-    const bool interior = xl >= 0 && xl + size <= nx;
-    auto col = [&](int c) { return interior ? xl + c : reflect_index(xl + c, nx); };
-    h.clear();
+    __shared__ uint8_t seg[2][SEG_MAX + 2];
+    __shared__ int colmap[SEG_MAX];
+    const int lane = threadIdx.x;
+    const int x0 = blockIdx.x * 64, x = x0 + lane;
+    const int y0 = blockIdx.y * rows;
+    if (y0 >= ny) return;                        // uniform over the block
+    const int y1 = min(y0 + rows, ny);
+    const int w = 63 + size;
+    const bool active = x < nx;
+    for (int p = lane; p < w; p += 64) colmap[p] = reflect_index(x0 - lo + p, nx);
+    __syncthreads();
+    uint8_t ra[SEG_PER_LANE], rb[SEG_PER_LANE];
+    auto load = [&](uint8_t(&r)[SEG_PER_LANE], int row) {
+        const uint8_t *src = img + (long long)reflect_index(row, ny) * nx;
+#pragma unroll
+        for (int k = 0; k < SEG_PER_LANE; ++k) {
+            const int p = lane + 64 * k;
+            r[k] = p < w ? src[colmap[p]] : (uint8_t)0;
+        }
+    };
+    auto store = [&](const uint8_t(&r)[SEG_PER_LANE], int buf) {
+#pragma unroll
+        for (int k = 0; k < SEG_PER_LANE; ++k) {
+            const int p = lane + 64 * k;
+            if (p < w) seg[buf][p] = r[k];
+        }
+    };
+    LdsHist h{hist + lane};
+    if (active) h.clear();
+    // the window of row y0, one row at a time
+    load(ra, y0 - lo);
     for (int r = 0; r < size; ++r) {
-        const uint8_t *row = img + (long long)reflect_index(y0 - lo + r, ny) * nx;
-        for (int c = 0; c < size; ++c) hist_add(h, row[col(c)]);
+        store(ra, 0);
+        __syncthreads();
+        if (r + 1 < size) load(ra, y0 - lo + r + 1);
+        if (active) {
+            const uint8_t *sp = seg[0] + lane;
+            int c = 0;
+            for (; c + 8 <= size; c += 8) {
+                uint32_t v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = sp[c + j];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) hist_add(h, v[j]);
+            }
+            for (; c < size; ++c) hist_add(h, sp[c]);
+        }
+        __syncthreads();
     }
     const int rank = size * size / 2;
-    int below;
-    int m = hist_find(h, rank, &below);
-    emit<uint8_t>(img, nx, x, y0, (uint8_t)m, bg, out, odt);
+    int below = 0, m = 0;
+    if (active) {
+        m = hist_find(h, rank, &below);
+        emit<uint8_t>(img, nx, x, y0, (uint8_t)m, bg, out, odt);
+    }
+    // step y: row y - 1 - lo leaves, row y - 1 - lo + size enters
+    if (y0 + 1 < y1) {
+        load(ra, y0 - lo);
+        load(rb, y0 - lo + size);
+    }
     for (int y = y0 + 1; y < y1; ++y) {
-        const uint8_t *rold = img + (long long)reflect_index(y - 1 - lo, ny) * nx;
-        const uint8_t *rnew = img + (long long)reflect_index(y - 1 - lo + size, ny) * nx;
-#pragma unroll 4
-        for (int c = 0; c < size; ++c) {
-            const int cc = col(c);
-            slide_remove(h, rold[cc], m, below);
-            slide_add(h, rnew[cc], m, below);
+        store(ra, 0);
+        store(rb, 1);
+        __syncthreads();
+        if (y + 1 < y1) {
+            load(ra, y - lo);
+            load(rb, y - lo + size);
         }
-        slide_settle(h, rank, m, below);
-        emit<uint8_t>(img, nx, x, y, (uint8_t)m, bg, out, odt);
+        if (active) {
+            const uint8_t *so = seg[0] + lane, *sn = seg[1] + lane;
+            int c = 0;
+            for (; c + 8 <= size; c += 8) {
+                uint32_t vo[8], vn[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    vo[j] = so[c + j];
+                    vn[j] = sn[c + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    slide_remove(h, vo[j], m, below);
+                    slide_add(h, vn[j], m, below);
+                }
+            }
+            for (; c < size; ++c) {
+                slide_remove(h, so[c], m, below);
+                slide_add(h, sn[c], m, below);
+            }
+            slide_settle(h, rank, m, below);
+            emit<uint8_t>(img, nx, x, y, (uint8_t)m, bg, out, odt);
+        }
+        __syncthreads();
     }
 }
 
@@ -341,14 +408,18 @@ __global__ __launch_bounds__(256) void k_gather_frame(
     }
 }
 
-int grid_for(long long n, int threads) {
+int device_cus() {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
             hipSuccess)
         cus = 256;
+    return cus;
+}
+
+int grid_for(long long n, int threads) {
     const long long want = (n + threads - 1) / threads;
-    return (int)std::max(1LL, std::min(want, (long long)cus * SEL_BLOCKS_PER_CU));
+    return (int)std::max(1LL, std::min(want, (long long)device_cus() * SEL_BLOCKS_PER_CU));
 }
 
 bool valid_dtype(int dt) { return dt >= LMZ_U8 && dt <= LMZ_F64; }
@@ -457,9 +528,15 @@ int lmz_median_filter(const void *image, int32_t dtype, int32_t ny, int32_t nx,
     LMZ_REQUIRE(!out || out_dtype == LMZ_F32 || out_dtype == LMZ_F64,
                 "out_dtype must be LMZ_F32 or LMZ_F64");
     if (dtype == LMZ_U8) {
-        const dim3 grid((nx + 63) / 64, (ny + FILTER_ROWS - 1) / FILTER_ROWS);
+        // rows per block: 32, halved while the grid has fewer than four
+        // workgroups per CU (the initial window costs size^2 per column)
+        const int bx = (nx + 63) / 64;
+        const int want = 4 * device_cus();
+        int rows = 32;
+        while (rows > 8 && (long long)bx * ((ny + rows - 1) / rows) < want) rows /= 2;
+        const dim3 grid(bx, (ny + rows - 1) / rows);
         hipLaunchKernelGGL(k_medfilt_u8, grid, dim3(64), 0, stream,
-                           (const uint8_t *)image, ny, nx, size, size / 2,
+                           (const uint8_t *)image, ny, nx, size, size / 2, rows,
                            (uint8_t *)background, out, out_dtype);
         LMZ_HIP(hipGetLastError());
         return LMZ_OK;

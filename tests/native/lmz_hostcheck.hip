@@ -211,6 +211,6 @@ double chk_mean_of_two(int dt, double a, double b) {
     }
 }
 
-int chk_reflect(long long i, long long n) { return reflect_index(i, n); }
+int chk_reflect(int i, int n) { return reflect_index(i, n); }
 
 }  // extern "C"

@@ -29,7 +29,7 @@ def chk():
     lib.chk_key.argtypes = [ctypes.c_int, ctypes.c_double]
     lib.chk_mean_of_two.restype = ctypes.c_double
     lib.chk_mean_of_two.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double]
-    lib.chk_reflect.argtypes = [ctypes.c_longlong, ctypes.c_longlong]
+    lib.chk_reflect.argtypes = [ctypes.c_int, ctypes.c_int]
     return lib
 
 

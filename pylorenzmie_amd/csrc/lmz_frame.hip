@@ -43,7 +43,10 @@ __global__ __launch_bounds__(64) void k_sel_init(SelectState *st,
 }
 
 // Histogram of digit (key >> s) & 255 over the keys in the current prefix
-// set; the first pass also counts NaNs.
+// set; the first pass also counts NaNs.  Each thread counts runs of one
+// digit before adding to the block's histogram: in the leading digits
+// nearly every key of a frame falls in one bin, where atomics would
+// serialize.
 template <int DT>
 __global__ __launch_bounds__(SEL_THREADS) void k_sel_hist(
     const typename Pix<DT>::T *__restrict__ img, long long n,
@@ -59,14 +62,24 @@ __global__ __launch_bounds__(SEL_THREADS) void k_sel_hist(
     const K himask = (s + 8 >= (int)(8 * sizeof(K))) ? (K)0
                                                       : (K)(~(K)0 << (s + 8));
     unsigned long long my_nan = 0ull;
+    uint32_t run_d = 0u, run_n = 0u;
     for (long long i = (long long)blockIdx.x * SEL_THREADS + threadIdx.x; i < n;
          i += (long long)gridDim.x * SEL_THREADS) {
         const typename P::T v = img[i];
         if (first) my_nan += P::isnan(v);
         const K k = P::key(v);
-        if (((k ^ prefix) & himask) == 0)
-            atomicAdd(&h[(uint32_t)((k >> s) & 0xFFu)], 1u);
+        if (((k ^ prefix) & himask) == 0) {
+            const uint32_t d = (uint32_t)((k >> s) & 0xFFu);
+            if (d == run_d) {
+                ++run_n;
+            } else {
+                if (run_n) atomicAdd(&h[run_d], run_n);
+                run_d = d;
+                run_n = 1u;
+            }
+        }
     }
+    if (run_n) atomicAdd(&h[run_d], run_n);
     if (first && my_nan) atomicAdd(&nnan, my_nan);
     __syncthreads();
     if (h[threadIdx.x]) atomicAdd(&st->hist[threadIdx.x], h[threadIdx.x]);

@@ -177,7 +177,29 @@ def kernel_times(lm, settings, args):
     P = np.array(lm.npix, float)
     flop_j = float(np.sum(P * (296. * nm + 480.)))
     flop_f = float(np.sum(P * (64. * nm + 70.)))
-    return tj, tf, flop_j, flop_f, float(np.sum(P)), float(np.mean(nm))
+    # the standalone J^T J reduction over a Jacobian written to HBM
+    # (lmz_normal_eq: 7 J columns + the residual, 32 B per pixel)
+    npx = int(b.npix_total)
+    J = torch.empty(7 * npx, dtype=torch.float32, device=dev)
+    r = torch.empty(npx, dtype=torch.float32, device=dev)
+    neq = torch.empty(36 * b.nfeat, dtype=torch.float64, device=dev)
+    _lmz.check(lib.lmz_jacobian(b.handle, _lmz.ptr(lm.params), None, _lmz.ptr(nmax),
+                                _lmz.ptr(coef), _lmz.ptr(dcoef), None, flags,
+                                _lmz.ptr(lm.data), _lmz.ptr(J), _lmz.ptr(r), None,
+                                None, s))
+
+    def normal_eq():
+        _lmz.check(lib.lmz_normal_eq(b.handle, _lmz.ptr(J), _lmz.ptr(r),
+                                     _lmz.ptr(neq), s))
+    normal_eq()
+    ev[2].record()
+    for _ in range(max(reps, 5)):
+        normal_eq()
+    ev[3].record()
+    torch.cuda.synchronize()
+    tn = ev[2].elapsed_time(ev[3]) / max(reps, 5) * 1e-3
+    del J, r
+    return tj, tf, tn, flop_j, flop_f, float(np.sum(P)), float(np.mean(nm))
 
 
 def fit_agreement(lm, params, res, p0):
@@ -207,10 +229,10 @@ def fit_agreement(lm, params, res, p0):
 
 
 def committed_figure(name, key, nfeat, side):
-    """A per-launch figure of the Jacobian kernel from a PMC summary
-    committed under profiles/ for this workload, or None for other
-    workloads: HBM bytes (FETCH_SIZE + WRITE_SIZE, calibrated against a
-    copy kernel) or counted FP32 FLOP (SQ_INSTS_VALU_FLOPS_FP32)."""
+    """A per-launch figure of a kernel from a PMC summary committed under
+    profiles/ for this workload, or None for other workloads: HBM bytes
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel) or counted
+    FP32 FLOP (SQ_INSTS_VALU_FLOPS_FP32)."""
     import re
     path = os.path.join(ROOT, 'profiles', f'{name}_{nfeat}x{side}x{side}.md')
     if not os.path.exists(path):
@@ -333,7 +355,7 @@ def main():
     log(f"bench: {agree['frac_within_1e-4_rel_and_kp_1e-5_abs']:.4f} of "
         f"{agree['features_compared']} fits within 1e-4 of the converged minimum")
     log('bench: kernel timing')
-    tj, tf, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
+    tj, tf, tn, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
     ach_j = flop_j / tj / 1e12
     ach_f = flop_f / tf / 1e12
     log(f'bench: jacobian {tj * 1e3:.2f} ms {ach_j:.1f} TFLOP/s; '
@@ -341,6 +363,22 @@ def main():
 
     # J reads the data (4 B / pixel) and writes 36 FP64 partials per tile
     bytes_j = 4. * npx + 36 * 8. * lm.batch.ntiles
+    # reductions: J^T J from J in HBM (32 B / pixel in, 288 B / feature
+    # out); the residual chi2 (4 B / pixel in, one FP64 per tile out)
+    bytes_n = 32. * npx + 36 * 8. * lm.batch.nfeat
+    bytes_r = 4. * npx + 8. * lm.batch.ntiles
+    reductions = {
+        'normal_eq': {'kernel': 'k_normal_eq', 'bound': 'hbm',
+                      'achieved': bytes_n / tn / 1e9, 'peak': PEAK_HBM,
+                      'unit': 'GB/s', 'frac': bytes_n / tn / 1e9 / PEAK_HBM,
+                      'avg_launch_ms': tn * 1e3, 'bytes_per_launch': bytes_n,
+                      'traffic': committed_figure(
+                          'k_normal_eq_traffic', 'traffic_bytes_per_launch',
+                          args.nfeat, args.side)},
+        'residual_chi2': {'kernel': 'k_forward<2>', 'bound': 'valu (FP64)',
+                          'achieved': bytes_r / tf / 1e9, 'unit': 'GB/s',
+                          'avg_launch_ms': tf * 1e3, 'bytes_per_launch': bytes_r}}
+    log(f'bench: normal_eq {tn * 1e3:.3f} ms {bytes_n / tn / 1e9:.0f} GB/s')
     traffic = committed_figure('k_jacobian_traffic', 'traffic_bytes_per_launch',
                                args.nfeat, args.side)
     counted = committed_figure('valu_counters', 'counted_fp32_flop_per_launch',
@@ -398,6 +436,7 @@ def main():
                      'counted_achieved': (counted / tj / 1e12) if counted else None,
                      'forward_kernel': {'achieved': ach_f, 'avg_launch_ms': tf * 1e3,
                                         'algorithmic_flop_per_launch': flop_f}},
+        'reductions': reductions,
         'fit_quality': {'converged_frac': float(ok.mean()),
                         'iterations_mean': float(iters.mean()),
                         'iterations_max': float(iters.max()),

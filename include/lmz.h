@@ -219,6 +219,10 @@ int lmz_intensity(const lmz_batch *batch, const double *params,
  * recomputing the recurrences and their tangents per pixel.  Any of jac,
  * resid, partials may be NULL (resid and partials need data).  Needs the
  * dcoef tangent tables of lmz_mie_ab.
+ *   jac       float [LMZ_NJAC][npix_total], column c at c * npix_total
+ *   resid     float [npix_total], (I - data) / noise
+ *   partials  double [ntiles][LMZ_NRED], the normal equations of each tile
+ *             (J scaled by 1/noise)
  * Replaces jaxLorenzMie.jac (theory/jaxLorenzMie.py:354-395) and
  * Optimizer._analytical_jac (analysis/Optimizer.py:251-255), extended to
  * k_p and alpha. */

@@ -13,6 +13,10 @@ Also measured:
   fd      the plain GPU finite-difference form (hipOptimizer: one feature at
           a time, host MINPACK LM with forward differences, residuals copied
           to the host per evaluation) on a sample of features -> speedup;
+  fd_batched  the same batched LM with a forward-difference Jacobian of the
+          FP64 hologram (lmz_lm_settings.fd_step = 1e-5, the reference's
+          diff_step) in place of the analytic one -> the adjoint's share of
+          the speedup;
   roofline  the Jacobian kernel (dominant) and the trial-forward kernel,
           timed with HIP events on the stream they run on, against the FP32
           VALU peak, with the survey's algorithmic FLOP counts;
@@ -241,6 +245,36 @@ def committed_figure(name, key, nfeat, side):
     return float(m.group(1)) if m else None
 
 
+def run_fd_batched(lm, p0, settings, params_adjoint, res_adjoint):
+    """The same batched LM with a forward-difference Jacobian of the FP64
+    hologram in place of the analytic one (steps 1e-5 |p_j|, the
+    reference's diff_step): one warm and one timed run of all features.
+    Returns fits/s, mean iterations, and the fraction of features whose
+    fit agrees with the analytic-Jacobian fit to 1e-4 (relative; k_p 1e-5
+    absolute) among those both runs converged."""
+    import dataclasses
+    import torch
+    fd = dataclasses.replace(settings, fd_step=1e-5)
+    lm.params.copy_(p0)
+    lm.run(fd, PARAMS)
+    torch.cuda.synchronize()
+    lm.params.copy_(p0)
+    t0 = time.perf_counter()
+    lm.run(fd, PARAMS)
+    params, res = lm.fetch()
+    dt = time.perf_counter() - t0
+    ok = lambda r: ((r[:, 38].astype(int) & 7) != 0) & ((r[:, 38].astype(int) & 56) == 0)
+    both = ok(res) & ok(res_adjoint)
+    d = np.abs(params[:, :7] - params_adjoint[:, :7])
+    rel = d / np.maximum(np.abs(params_adjoint[:, :7]), 1e-300)
+    rel[:, 5] = d[:, 5] * 1e-4 / 1e-5           # k_p: absolute 1e-5
+    within = np.all(rel <= 1e-4, axis=1)
+    return dict(value=lm.nfeat / dt, unit='fits/s', fd_step=1e-5,
+                iterations_mean=float(res[:, 37].mean()),
+                agree_with_adjoint=float(within[both].mean()) if both.any() else None,
+                features_compared=int(both.sum()), seconds=dt)
+
+
 def run_fd(args, truth, start, data_host):
     """Plain GPU FD form on a sample of features, sequentially."""
     from pylorenzmie_amd.theory import LorenzMie, Sphere, Instrument
@@ -384,7 +418,13 @@ def main():
     counted = committed_figure('valu_counters', 'counted_fp32_flop_per_launch',
                                args.nfeat, args.side)
     fd = None
+    fdb = None
     if not args.no_fd:
+        log('bench: batched LM with a forward-difference Jacobian')
+        fdb = run_fd_batched(lm, p0, settings, params, res)
+        log(f"bench: batched FD {fdb['value']:.1f} fits/s, "
+            f"{fdb['iterations_mean']:.1f} iterations, "
+            f"{fdb['agree_with_adjoint']:.4f} agree with the adjoint fits")
         log('bench: plain GPU FD form sample')
         data_host = data_dev[: args.side * args.side * args.fd_sample].cpu().numpy()
         data_host = data_host.reshape(-1, args.side, args.side)
@@ -456,6 +496,8 @@ def main():
         'fit_agreement': agree,
         'fd_plain_gpu': fd,
         'speedup_vs_fd': (fits_per_s / fd['value']) if fd else None,
+        'fd_batched': fdb,
+        'speedup_vs_fd_batched': (fits_per_s / fdb['value']) if fdb else None,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line), flush=True)

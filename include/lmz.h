@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define LMZ_VERSION 102
+#define LMZ_VERSION 103
 #define LMZ_NPARAM 8       /* stride of params rows */
 #define LMZ_NJAC 7         /* x_p y_p z_p a_p n_p k_p alpha */
 #define LMZ_NRED 36        /* 28 + 7 + 1 */
@@ -298,6 +298,11 @@ typedef struct lmz_lm_settings {
     double lambda0;        /* initial damping relative to diag(J^T J) */
     uint8_t vary[LMZ_NJAC];/* 1 = fitted, 0 = held fixed */
     uint8_t use_graph;     /* capture the iteration in a HIP graph */
+    double fd_step;        /* 0: analytic Jacobian (lmz_jacobian's kernel);
+                              > 0: forward differences of the FP64 hologram
+                              with steps fd_step |p_j| (fd_step if p_j = 0),
+                              MINPACK lmdif's rule for scipy's diff_step
+                              (analysis/Optimizer.py:111) */
 } lmz_lm_settings;
 
 /* per-feature LM result, double [nfeat][LMZ_NRES] */

@@ -90,7 +90,8 @@ class LMSettings(ctypes.Structure):
                 ('gtol', ctypes.c_double),
                 ('lambda0', ctypes.c_double),
                 ('vary', ctypes.c_uint8 * NJAC),
-                ('use_graph', ctypes.c_uint8)]
+                ('use_graph', ctypes.c_uint8),
+                ('fd_step', ctypes.c_double)]
 
 
 _P = ctypes.c_void_p

@@ -26,6 +26,7 @@ class LMSettings:
     lambda0: float = 1e-3
     check_every: int = 4
     use_graph: bool = True
+    fd_step: float = 0.0      # > 0: forward-difference Jacobian (transparency)
 
 
 def stats_from_jtj(jtj_upper: np.ndarray, chi2: float, npix: int,
@@ -193,6 +194,7 @@ class LMBatch:
         for i in range(_lmz.NJAC):
             s.vary[i] = 1 if (PARAMS[i] in vary) else 0
         s.use_graph = 1 if settings.use_graph else 0
+        s.fd_step = float(settings.fd_step)
         return s
 
     def load(self, params0: np.ndarray) -> None:

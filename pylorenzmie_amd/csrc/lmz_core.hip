@@ -320,6 +320,142 @@ __global__ __launch_bounds__(NT) void k_jacobian(
   }
 }
 
+// Hologram of one pixel in FP64 (the residual kernel's arithmetic) for the
+// particle described by fu with coefficient table cf; E receives the field
+// without alpha.
+__device__ inline double hologram_d(const FeatUniform &fu, const float *cf,
+                                    float ax, float ay, float az, bool bohren,
+                                    bool cart, cd (&E)[3]) {
+    PixGeomD g;
+    pixel_geometry_d(fu, ax, ay, az, bohren, g);
+    const SumsD S = forward_sum(g, cf, fu.nmax);
+    field_from_sums(g, S, cart, E);
+    return intensity(E, fu.alphad);
+}
+
+// Steps and perturbed parameter rows of a forward-difference Jacobian, as
+// MINPACK's lmdif (scipy least_squares method 'lm' with diff_step, called
+// at analysis/Optimizer.py:221) takes them: h_j = eps |p_j|, or eps when
+// p_j = 0.  Rows fd_params[c][f] = p + h_{3+c} e_{3+c} for a_p, n_p, k_p,
+// whose coefficient tables differ from the base point's.
+__global__ __launch_bounds__(64) void k_fd_setup(
+    int nfeat, const double *__restrict__ params, double eps,
+    const uint8_t *__restrict__ active, double *__restrict__ fd_h,
+    double *__restrict__ fd_params) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfeat) return;
+    if (active && !active[f]) return;
+    const double *p = params + (long long)LMZ_NPARAM * f;
+    for (int c = 0; c < LMZ_NPARAM; ++c) {
+        double h = eps * fabs(p[c]);
+        if (h == 0.0) h = eps;
+        fd_h[(long long)LMZ_NPARAM * f + c] = h;
+    }
+    for (int q = 0; q < 3; ++q) {
+        double *row = fd_params + ((long long)q * nfeat + f) * LMZ_NPARAM;
+        for (int c = 0; c < LMZ_NPARAM; ++c) row[c] = p[c];
+        row[3 + q] += fd_h[(long long)LMZ_NPARAM * f + 3 + q];
+    }
+}
+
+// Forward-difference Jacobian over one tile: the FP64 hologram at p and at
+// p + h_j e_j for every varied j (x_p, y_p, z_p move the particle, a_p,
+// n_p, k_p use the coefficient tables of their perturbed rows, alpha needs
+// only the base field), J_j = (I_j - I) / h_j, reduced to the same per-tile
+// normal-equation partials as k_jacobian.  The finite-difference form the
+// analytic Jacobian replaces, inside the same batched LM.
+template <int NT = LMZ_THREADS>
+__global__ __launch_bounds__(NT) void k_fd_jacobian(
+    const lmz_feature *__restrict__ feat,
+    const int32_t *__restrict__ tile_feature, int nfeat,
+    const double *__restrict__ params, const int32_t *__restrict__ sel,
+    int flip, int nbuf, const int32_t *__restrict__ nmax,
+    const float *__restrict__ coef, const double *__restrict__ fd_params,
+    const int32_t *__restrict__ fd_nmax, const float *__restrict__ fd_coef,
+    long long fd_coef_stride, const double *__restrict__ fd_h, int vary,
+    const float *__restrict__ coords, long long npix_total, int flags,
+    const float *__restrict__ data, const float *__restrict__ rin,
+    long long rin_stride, double *__restrict__ partials,
+    const int32_t *__restrict__ list, int32_t *count) {
+  __shared__ int next_w;
+  int chunks_done = 0;
+  const int nwork = count ? count[0] : (int)gridDim.x;
+  for (int w = list ? next_tile(count, next_w, chunks_done)
+                    : xcd_tile((int)blockIdx.x, (int)gridDim.x);
+       w < nwork;
+       w = list ? next_tile(count, next_w, chunks_done) : nwork) {
+    const int t = list ? list[w] : w;
+    const int f = tile_feature[t];
+    const lmz_feature &F = feat[f];
+    const int b = sel ? (sel[f] ^ flip) : 0;
+    const int N = nmax[(long long)f * nbuf + b];
+    const double *p = params + (long long)LMZ_NPARAM * f;
+    const double *h = fd_h + (long long)LMZ_NPARAM * f;
+    const int slot = F.coef_offset + b * F.nmax_cap;
+    const FeatUniform fu = feature_uniform(F, p, N, slot);
+    const bool cart = flags & LMZ_CARTESIAN;
+    const bool bohren = flags & LMZ_BOHREN;
+    const float *cf0 = coef + (long long)LMZ_COEF_STRIDE * slot;
+    const float *rinf = rin + b * rin_stride;
+    float acc[LMZ_NRED];
+#pragma unroll
+    for (int i = 0; i < LMZ_NRED; ++i) acc[i] = 0.f;
+#pragma unroll 1
+    for (int q = 0; q < LMZ_TILE / NT; ++q) {
+        const int j = (t - fu.tile0) * LMZ_TILE + q * NT + threadIdx.x;
+        if (j >= fu.npix) break;
+        float ax, ay, az;
+        pixel_coords(fu, j, coords, npix_total, ax, ay, az);
+        cd E[3], Ej[3];
+        const double I0 = hologram_d(fu, cf0, ax, ay, az, bohren, cart, E);
+        float Jn[LMZ_NJAC];
+#pragma unroll 1
+        for (int c = 0; c < 3; ++c) {          // x_p, y_p, z_p
+            Jn[c] = 0.f;
+            if (!(vary & (1 << c))) continue;
+            double pc[LMZ_NPARAM];
+            for (int i = 0; i < LMZ_NPARAM; ++i) pc[i] = p[i];
+            pc[c] += h[c];
+            const FeatUniform fc = feature_uniform(F, pc, N, slot);
+            const double Ic = hologram_d(fc, cf0, ax, ay, az, bohren, cart, Ej);
+            Jn[c] = (float)((Ic - I0) / h[c] * (double)fu.inv_noise);
+        }
+#pragma unroll 1
+        for (int c = 3; c < 6; ++c) {          // a_p, n_p, k_p
+            Jn[c] = 0.f;
+            if (!(vary & (1 << c))) continue;
+            const int qn = c - 3;
+            const int Nc = fd_nmax[(long long)qn * nfeat * nbuf + (long long)f * nbuf];
+            const FeatUniform fc = feature_uniform(
+                F, fd_params + ((long long)qn * nfeat + f) * LMZ_NPARAM, Nc,
+                F.coef_offset);
+            const float *cfc = fd_coef + qn * fd_coef_stride +
+                               (long long)LMZ_COEF_STRIDE * F.coef_offset;
+            const double Ic = hologram_d(fc, cfc, ax, ay, az, bohren, cart, Ej);
+            Jn[c] = (float)((Ic - I0) / h[c] * (double)fu.inv_noise);
+        }
+        Jn[6] = 0.f;
+        if (vary & (1 << 6))                   // alpha: the same field
+            Jn[6] = (float)((intensity(E, fu.alphad + h[6]) - I0) / h[6] *
+                            (double)fu.inv_noise);
+        const long long gi = fu.pix_offset + j;
+        const float r = rinf[gi];
+        int idx = 0;
+#pragma unroll
+        for (int a = 0; a < LMZ_NJAC; ++a)
+#pragma unroll
+            for (int c = a; c < LMZ_NJAC; ++c, ++idx)
+                acc[idx] = fmaf(Jn[a], Jn[c], acc[idx]);
+#pragma unroll
+        for (int a = 0; a < LMZ_NJAC; ++a) acc[28 + a] = fmaf(Jn[a], r, acc[28 + a]);
+        acc[35] = fmaf(r, r, acc[35]);
+    }
+    (void)data;
+    block_reduce_store<LMZ_NRED, NT>(acc, partials + (long long)LMZ_NRED * t);
+    __syncthreads();   // the reduction buffer is reused by the next tile
+  }
+}
+
 // Superposed field of several particles per feature, FP64, in one pass
 // over the pixels (theory/LorenzMie.py:160-181 loops over particles with a
 // full pass each): particles part_offset[f] .. part_offset[f + 1] - 1,
@@ -578,6 +714,43 @@ void launch_jacobian(const lmz_batch *b, const double *params,
                        rin, rin_stride, jac, resid, partials, active,
                        tl ? (const int32_t *)tl->list : nullptr,
                        tl ? tl->count : nullptr);
+}
+
+void launch_fd_jacobian(const lmz_batch *b, const double *params,
+                        const int32_t *sel, int flip, const int32_t *nmax,
+                        const float *coef, const FdTables &fd, int vary,
+                        const float *coords, int flags, const float *data,
+                        const float *rin, long long rin_stride,
+                        double *partials, const uint8_t *active,
+                        const TileList *tl, double *work, hipStream_t stream) {
+    hipLaunchKernelGGL(k_fd_setup, dim3((b->nfeat + 63) / 64), dim3(64), 0,
+                       stream, b->nfeat, params, fd.eps, active, fd.h,
+                       fd.params);
+    // coefficient tables of the a_p, n_p, k_p rows (the params rows of a
+    // feature are contiguous per set, so the Mie kernel reads set q as its
+    // params array)
+    for (int q = 0; q < 3; ++q) {
+        if (!(vary & (1 << (3 + q)))) continue;
+        launch_mie(b, fd.params + (long long)q * b->nfeat * LMZ_NPARAM, nullptr,
+                   0, fd.nmax + (long long)q * b->nfeat * b->coef_buffers,
+                   fd.coef + q * fd.coef_stride, nullptr, nullptr, work, active,
+                   stream);
+    }
+    const int grid = tl ? tl->grid : b->ntiles;
+    const int nt = b->sched.jac_threads;
+    const int32_t *list = tl ? (const int32_t *)tl->list : nullptr;
+    int32_t *count = tl ? tl->count : nullptr;
+    auto go = [&](auto kern, int threads) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, stream, b->d_feat,
+                           b->d_tile_feature, b->nfeat, params, sel, flip,
+                           b->coef_buffers, nmax, coef, fd.params, fd.nmax,
+                           fd.coef, fd.coef_stride, fd.h, vary, coords,
+                           b->npix_total, flags, data, rin, rin_stride, partials,
+                           list, count);
+    };
+    if (nt == 64) go(k_fd_jacobian<64>, 64);
+    else if (nt == 128) go(k_fd_jacobian<128>, 128);
+    else go(k_fd_jacobian<256>, 256);
 }
 
 // Compacted list of the tiles of the features whose mask is set, and its

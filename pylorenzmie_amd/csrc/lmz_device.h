@@ -57,6 +57,25 @@ void launch_jacobian(const lmz_batch *b, const double *params,
                      const float *rin, long long rin_stride, float *jac,
                      float *resid, double *partials, const uint8_t *active,
                      const TileList *tl, hipStream_t stream);
+// Buffers of the forward-difference Jacobian: steps, the a_p / n_p / k_p
+// parameter rows and their coefficient tables.
+struct FdTables {
+    double eps;            // relative step
+    double *h;             // [nfeat][LMZ_NPARAM]
+    double *params;        // [3][nfeat][LMZ_NPARAM]
+    int32_t *nmax;         // [3][nfeat * coef_buffers]
+    float *coef;           // 3 tables, coef_stride floats apart
+    long long coef_stride;
+};
+// Normal-equation partials of a forward-difference Jacobian of the FP64
+// hologram, for the features in tl (or all); rin as in launch_jacobian.
+void launch_fd_jacobian(const lmz_batch *b, const double *params,
+                        const int32_t *sel, int flip, const int32_t *nmax,
+                        const float *coef, const FdTables &fd, int vary,
+                        const float *coords, int flags, const float *data,
+                        const float *rin, long long rin_stride,
+                        double *partials, const uint8_t *active,
+                        const TileList *tl, double *work, hipStream_t stream);
 void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
                       const TileList &tl, hipStream_t stream);
 // Persistent grids of the LM's pixel launches under the batch's schedule.

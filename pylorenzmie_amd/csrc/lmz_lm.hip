@@ -37,7 +37,8 @@ struct LMState {
 
 struct LMLayout {
     size_t coef, dcoef, nmax, sel, work, partials, chi2_tiles, state, ptrial,
-        mask_j, mask_f, count, resid, list_j, list_f, total;
+        mask_j, mask_f, count, resid, list_j, list_f, fd_h, fd_params, fd_nmax,
+        fd_coef, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -62,6 +63,11 @@ static LMLayout lm_layout(const lmz_batch *b) {
     L.resid = take(sizeof(float) * 2 * (size_t)b->npix_total);
     L.list_j = take(sizeof(int32_t) * (size_t)b->ntiles);
     L.list_f = take(sizeof(int32_t) * (size_t)b->ntiles);
+    // forward-difference Jacobian (settings.fd_step > 0)
+    L.fd_h = take(sizeof(double) * LMZ_NPARAM * (size_t)b->nfeat);
+    L.fd_params = take(sizeof(double) * 3 * LMZ_NPARAM * (size_t)b->nfeat);
+    L.fd_nmax = take(sizeof(int32_t) * 3 * 2 * (size_t)b->nfeat);
+    L.fd_coef = take(sizeof(float) * 3 * LMZ_COEF_STRIDE * (size_t)b->coef_slots);
     L.total = off;
     return L;
 }
@@ -405,8 +411,24 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
     // J^T J and J^T r at the current point, r from its FP64 forward pass,
     // over the tiles of the features that accepted their last step
     launch_tile_list(b, mask_j, tl_j, s);
-    launch_jacobian(b, params, sel, 0, nmax, coef, dcoef, coords, flags, data,
-                    resid, np, nullptr, nullptr, partials, nullptr, &tl_j, s);
+    if (set.fd_step > 0.0) {
+        FdTables fd;
+        fd.eps = set.fd_step;
+        fd.h = reinterpret_cast<double *>(ws + L.fd_h);
+        fd.params = reinterpret_cast<double *>(ws + L.fd_params);
+        fd.nmax = reinterpret_cast<int32_t *>(ws + L.fd_nmax);
+        fd.coef = reinterpret_cast<float *>(ws + L.fd_coef);
+        fd.coef_stride = (long long)LMZ_COEF_STRIDE * b->coef_slots;
+        int vary = 0;
+        for (int i = 0; i < LMZ_NJAC; ++i) vary |= set.vary[i] ? (1 << i) : 0;
+        launch_fd_jacobian(b, params, sel, 0, nmax, coef, fd, vary, coords,
+                           flags, data, resid, np, partials, mask_j, &tl_j,
+                           work, s);
+    } else {
+        launch_jacobian(b, params, sel, 0, nmax, coef, dcoef, coords, flags,
+                        data, resid, np, nullptr, nullptr, partials, nullptr,
+                        &tl_j, s);
+    }
     hipLaunchKernelGGL(k_lm_solve, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
                        partials, st, params, ptrial, mask_j, mask_f, set);
     if (!fixed_ab)
@@ -482,6 +504,8 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
     LMZ_REQUIRE(set.ftol >= 0.0 && set.xtol >= 0.0 && set.gtol >= 0.0,
                 "tolerances must be >= 0");
     LMZ_REQUIRE(set.lambda0 > 0.0, "lambda0 must be > 0");
+    LMZ_REQUIRE(set.fd_step >= 0.0 && set.fd_step < 1.0,
+                "fd_step must be in [0, 1): 0 for the analytic Jacobian");
     LMZ_REQUIRE(!(flags & LMZ_FP32),
                 "the LM's residuals are FP64 (its 1e-9 ftol test needs them)");
     int nv = 0;

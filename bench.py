@@ -22,6 +22,16 @@ Also measured:
           VALU peak, with the survey's algorithmic FLOP counts;
   cpu_baseline  the FP64 oracle (numpy model + scipy MINPACK LM) on one
           feature on the host.
+
+--config 1..4 measures the other configurations instead (one JSON line):
+  1  batched forward holograms (1024 crops, side U{64..400}): holograms/s,
+     FP64 TFLOP/s, error against the oracle at sampled pixels;
+  2  forward + adjoint of sum (I - d)^2 (256 crops of 256^2): time, error
+     of the gradient against FP64 central differences;
+  3  batched LM on uneven crops (side log-U[48, 512]): fits/s, the
+     Jacobian's FP32 fraction, the J^T J reduction's HBM rate;
+  4  hard regime (512 crops of 512^2, layered and large spheres): hologram
+     time and error, fits/s of both halves.
 """
 from __future__ import annotations
 
@@ -319,6 +329,279 @@ def run_cpu(args, start, data_host):
     return n / dt, n, dt
 
 
+# ---------------------------------------------------------------------------
+# The other configurations (--config N): one JSON line each with the
+# quantities the measurement plan asks of them.  Seeds are the config index.
+def _timed(fn, reps):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e-3
+
+
+def _draw_cfg(rng, sides, alpha):
+    """Parameters as configs 1-3: centre +-5 px, z U[50, 400], a_p
+    U[0.4, 3.0], n_p U[1.38, 1.65], k_p U[0, 0.01], alpha U[0.8, 1.2] or 1."""
+    p = np.zeros((len(sides), 7))
+    for i, s in enumerate(sides):
+        c = (s - 1) / 2
+        p[i] = (c + rng.uniform(-5, 5), c + rng.uniform(-5, 5),
+                rng.uniform(50., 400.), rng.uniform(0.4, 3.0),
+                rng.uniform(1.38, 1.65), rng.uniform(0., 0.01),
+                rng.uniform(0.8, 1.2) if alpha else 1.0)
+    return p
+
+
+def _oracle_error(hb, params, picks, rng, layers=None):
+    """(max |I - I_oracle|, max of the same less the float32 rounding of
+    the stored value 2^-23 I) over 2000 sampled pixels of each picked
+    feature."""
+    from oracle import lorenzmie as ref
+    host = hb.out.cpu().numpy()
+    worst, excess = 0., -np.inf
+    for i in picks:
+        ny, nx = hb.shapes[i]
+        idx = np.unique(rng.choice(ny * nx, min(2000, ny * nx), replace=False))
+        c = ref.meshgrid((ny, nx))[:, idx]
+        if layers is None:
+            want = ref.hologram(c, dict(zip(ref.PARAMS, params[i])), **INSTRUMENT)
+        else:
+            lay = np.asarray(layers[i])
+            ab = (ref.mie_coefficients(lay[0, 0], lay[0, 1], lay[0, 2], 1.34, 0.447)
+                  if len(lay) == 1 else
+                  ref.mie_coefficients_layered(lay[:, 0], lay[:, 1], lay[:, 2],
+                                               1.34, 0.447))
+            want = ref.hologram(c, dict(x_p=params[i, 0], y_p=params[i, 1],
+                                        z_p=params[i, 2], ab=ab),
+                                alpha=params[i, 6], **INSTRUMENT)
+        err = np.abs(host[hb.offsets[i] + idx] - want)
+        worst = max(worst, float(err.max()))
+        excess = max(excess, float((err - 2. ** -23 * np.abs(want)).max()))
+    return worst, excess
+
+
+def run_config(args):
+    import torch
+    from pylorenzmie_amd import _lmz
+    from pylorenzmie_amd.theory import Instrument
+    from pylorenzmie_amd.theory.batch import HologramBatch
+    cfg = args.config
+    rng = np.random.default_rng(cfg)
+    inst = Instrument(**INSTRUMENT)
+    line = {'config': cfg}
+    if cfg == 1:
+        sides = [int(rng.integers(64, 401)) for _ in range(1024)]
+        params = _draw_cfg(rng, sides, alpha=False)
+        caps = [_lmz.nmax_cap(p[3], p[4], p[5], inst.n_m, inst.wavelength)
+                for p in params]
+        hb = HologramBatch([(s, s) for s in sides], caps, instrument=inst)
+        hb.load(params)
+        t = _timed(hb.compute, 5)
+        nm = hb.nmax.cpu().numpy().astype(float)
+        P = np.array([s * s for s in sides], float)
+        flop = float(np.sum(P * (64. * nm + 70.)))
+        line.update(workload='batched forward holograms, 1024 crops of side '
+                             'U{64..400}, Mie + FP64 pixel sum',
+                    pixels=float(P.sum()), ms=t * 1e3, holograms_per_s=1024 / t,
+                    achieved_tflops=flop / t / 1e12,
+                    frac_fp64_peak=flop / t / 1e12 / 78.6)
+        e, x = _oracle_error(hb, params, range(0, 1024, 256), rng)
+        line.update(max_abs_err_vs_oracle=e, max_err_beyond_float32_rounding=x)
+    elif cfg == 2:
+        from pylorenzmie_amd.theory.autograd import hologram
+        from oracle import lorenzmie as ref
+        n, s = 256, 256
+        truth = _draw_cfg(rng, [s] * n, alpha=True)
+        q = truth.copy()
+        q[:, :3] += (2., -2., 2.)
+        q[:, 3] *= 1.05
+        q[:, 4] -= 0.02
+        caps = [_lmz.nmax_cap(p[3], p[4], p[5], inst.n_m, inst.wavelength)
+                for p in np.maximum(truth, q)]
+        hb = HologramBatch([(s, s)] * n, caps, instrument=inst)
+        hb.load(truth)
+        g = torch.Generator(device='cuda')
+        g.manual_seed(cfg)
+        data = hb.compute().clone() + 0.05 * torch.randn(
+            hb.npix_total, generator=g, device='cuda', dtype=torch.float32)
+        pq = torch.tensor(q, device='cuda', requires_grad=True)
+
+        def step():
+            pq.grad = None
+            I = hologram(pq, hb)
+            ((I.double() - data.double()) ** 2).sum().backward()
+        t = _timed(step, 5)
+        grad = pq.grad.cpu().numpy()
+        # FP64 central differences of each feature's loss, h = 1e-6 relative
+        d_host = data.cpu().numpy()
+        c = ref.meshgrid((s, s))
+        worst = 0.
+        for i in (0, 1):
+            nmax = ref.wiscombe_yang(ref.size_parameter(q[i, 3]),
+                                     (q[i, 4] + 1j * q[i, 5]) / 1.34)
+            d = d_host[hb.offsets[i]:hb.offsets[i] + s * s].astype(float)
+
+            def L(v):
+                I = ref.hologram(c, dict(x_p=v[0], y_p=v[1], z_p=v[2], a_p=v[3],
+                                         n_p=v[4], k_p=v[5], nmax=nmax),
+                                 alpha=v[6], **INSTRUMENT)
+                return np.sum((I - d) ** 2)
+            for j in range(7):
+                h = 1e-6 * max(abs(q[i, j]), 1e-3)
+                vp, vm = q[i].copy(), q[i].copy()
+                vp[j] += h
+                vm[j] -= h
+                fd = (L(vp) - L(vm)) / (2 * h)
+                worst = max(worst, abs(grad[i, j] - fd) / max(abs(fd), 1e-6))
+        line.update(workload='forward + adjoint (VJP) of L = sum (I - d)^2, '
+                             '256 crops of 256x256, alpha U[0.8, 1.2]',
+                    pixels=float(hb.npix_total), ms_forward_and_backward=t * 1e3,
+                    max_rel_err_vs_fp64_central_differences=worst)
+    elif cfg == 3:
+        from pylorenzmie_amd.analysis.lm import LMBatch, LMSettings
+        n = args.nfeat
+        sides = [int(round(np.exp(rng.uniform(np.log(48.), np.log(512.)))))
+                 for _ in range(n)]
+        truth = _draw_cfg(rng, sides, alpha=True)
+        sigma = rng.choice([0.02, 0.05, 0.1], n)
+        sg = rng.choice([-1., 1.], size=(n, 5))
+        start = truth.copy()
+        start[:, :3] += 2 * sg[:, :3]
+        start[:, 3] *= 1 + 0.05 * sg[:, 3]
+        start[:, 4] += 0.02 * sg[:, 4]
+        caps = [_lmz.nmax_cap(p[3], p[4], p[5], inst.n_m, inst.wavelength)
+                for p in np.maximum(truth, start)]
+        hb = HologramBatch([(x, x) for x in sides], caps, instrument=inst)
+        hb.load(truth)
+        g = torch.Generator(device='cuda')
+        g.manual_seed(cfg)
+        sig = torch.from_numpy(np.repeat(sigma, [x * x for x in sides]).astype(np.float32))
+        data = hb.compute().clone() + sig.cuda() * torch.randn(
+            hb.npix_total, generator=g, device='cuda', dtype=torch.float32)
+        crops = [dict(data=np.zeros((x, x), np.float32), width=x, origin=(0., 0., 0.),
+                      noise=float(sigma[i]), nmax_cap=caps[i])
+                 for i, x in enumerate(sides)]
+        lm = LMBatch(crops, inst)
+        lm.data.copy_(data)
+        settings = LMSettings(max_iter=50, ftol=args.ftol, xtol=args.xtol,
+                              gtol=args.gtol)
+        lm.load(start)
+        lm.autotune()
+        lm.load(start)
+        lm.run(settings, PARAMS)
+        lm.load(start)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lm.run(settings, PARAMS)
+        params, res = lm.fetch()
+        dt = time.perf_counter() - t0
+        st = res[:, 38].astype(int)
+        ok = ((st & 7) != 0) & ((st & 56) == 0)
+        lm.params.copy_(torch.from_numpy(np.hstack([start, np.zeros((n, 1))])).cuda())
+        tj, tf, tn, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
+        bytes_n = 32. * npx + 36 * 8. * n
+        line.update(workload=f'batched 7-parameter LM, {n} crops of side '
+                             'log-U[48, 512], sigma in {0.02, 0.05, 0.1}',
+                    pixels=float(npx), fits_per_s=n / dt, seconds=dt,
+                    converged_frac=float(ok.mean()),
+                    iterations_mean=float(res[:, 37].mean()),
+                    jacobian_ms=tj * 1e3, jacobian_tflops=flop_j / tj / 1e12,
+                    jacobian_frac_fp32_peak=flop_j / tj / 1e12 / PEAK_FP32,
+                    forward_ms=tf * 1e3, forward_tflops=flop_f / tf / 1e12,
+                    normal_eq_gbs=bytes_n / tn / 1e9,
+                    normal_eq_frac_hbm_peak=bytes_n / tn / 1e9 / PEAK_HBM)
+    elif cfg == 4:
+        from pylorenzmie_amd.analysis.lm import LMBatch, LMSettings
+        from pylorenzmie_amd.theory.LayeredSphere import layered_cap
+        n, side, mag = 512, 512, INSTRUMENT['magnification']
+        params, layers = np.zeros((n, 7)), []
+        for i in range(n):
+            a_p = rng.uniform(0.5, 10.)
+            if rng.random() < 0.5:
+                L = int(rng.integers(2, 6))
+                radii = np.sort(rng.uniform(0.05, 1., L)) * a_p
+                radii[-1] = a_p
+                nl = rng.uniform(1.33, 2.0, L)
+            else:
+                radii, nl = np.array([a_p]), rng.uniform(1.33, 2.0, 1)
+            layers.append(np.stack([radii, nl, np.zeros(len(radii))], 1))
+            c = (side - 1) / 2
+            params[i] = (c + rng.uniform(-5, 5), c + rng.uniform(-5, 5),
+                         a_p / mag + 30. + rng.uniform(0., 250.), a_p, nl[-1], 0.,
+                         rng.uniform(0.8, 1.2))
+        caps = [layered_cap(lay, 1.34, 0.447) for lay in layers]
+        hb = HologramBatch([(side, side)] * n, caps, instrument=inst)
+        hb.load(params)
+        hb.load_layers(layers)
+        t = _timed(hb.compute, 3)
+        nm = hb.nmax.cpu().numpy()
+        lay_idx = [i for i in range(n) if len(layers[i]) > 1]
+        err, excess = _oracle_error(hb, params, [int(np.argmax(nm))] + lay_idx[:3],
+                                    rng, layers)
+        g = torch.Generator(device='cuda')
+        g.manual_seed(cfg)
+        data = hb.compute().clone() + 0.05 * torch.randn(
+            hb.npix_total, generator=g, device='cuda', dtype=torch.float32)
+        inst_n = Instrument(noise=0.05, **INSTRUMENT)
+        out = {}
+        for name, idx, vary in (('layered', lay_idx, ['x_p', 'y_p', 'z_p', 'alpha']),
+                                ('homogeneous', [i for i in range(n) if len(layers[i]) == 1],
+                                 list(PARAMS))):
+            sub = data.view(n, side * side)[idx].reshape(-1)
+            crops = [dict(data=np.zeros((side, side), np.float32), width=side,
+                          origin=(0., 0., 0.),
+                          nmax_cap=(caps[i] if name == 'layered' else
+                                    _lmz.nmax_cap(1.05 * params[i, 3], params[i, 4],
+                                                  0., 1.34, 0.447)))
+                     for i in idx]
+            lm = LMBatch(crops, inst_n)
+            lm.data.copy_(sub)
+            start = params[idx].copy()
+            start[:, :3] += (2., -2., 2.)
+            if name == 'layered':
+                start[:, 6] *= 1.1
+                lm.load_layers([layers[i] for i in idx])
+            else:
+                start[:, 3] *= 1.05
+                start[:, 4] -= 0.02
+            settings = LMSettings(max_iter=50)
+            lm.load(start)
+            lm.run(settings, vary)
+            lm.load(start)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            lm.run(settings, vary)
+            p, res = lm.fetch()
+            dt = time.perf_counter() - t0
+            st = res[:, 38].astype(int)
+            ok = ((st & 7) != 0) & ((st & 56) == 0)
+            redchi = res[:, 35] / (side * side - len(vary))
+            good = ok & (redchi < 1.02)
+            out[name] = dict(features=len(idx), fits_per_s=len(idx) / dt,
+                             varied=vary, converged_frac=float(ok.mean()),
+                             at_noise_level_frac=float(good.mean()),
+                             median_abs_err_xyz=float(np.median(
+                                 np.abs(p[good, :3] - params[idx][good, :3]))))
+            lm.close()
+        line.update(workload='hard regime: 512 crops of 512x512, half layered '
+                             '(2-5 layers, fixed coefficients, x y z alpha '
+                             'fitted), half homogeneous a_p U[0.5, 10] um '
+                             '(7 parameters from config-3 starts)',
+                    nmax_max=int(nm.max()), holograms_ms=t * 1e3,
+                    max_abs_err_vs_oracle=err,
+                    max_err_beyond_float32_rounding=excess, **out)
+    else:
+        raise SystemExit('--config must be 1, 2, 3 or 4')
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument('--gpus', type=int, default=1)
@@ -344,6 +627,9 @@ def main():
     ap.add_argument('--no-fd', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--seed', type=int, default=3)
+    ap.add_argument('--config', type=int, default=0,
+                    help='1-4: measure that configuration instead of the '
+                         'headline (one JSON line)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py runs on one GPU: this build has no multi-GPU path '
@@ -359,6 +645,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         log(f'bench: copied {args.calibrate_copy} MiB device to device')
+        return
+    if args.config:
+        run_config(args)
         return
 
     truth, start = draw_features(args.nfeat, args.side, args.seed)

@@ -582,7 +582,8 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_tile_sum(
 }
 
 // Normal equations from J and r in HBM: one block per feature.
-__global__ __launch_bounds__(LMZ_THREADS) void k_normal_eq(
+template <int NT>
+__global__ __launch_bounds__(NT) void k_normal_eq(
     const lmz_feature *__restrict__ feat, const float *__restrict__ jac,
     const float *__restrict__ resid, long long npix_total,
     double *__restrict__ out) {
@@ -592,7 +593,7 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_normal_eq(
     float acc[LMZ_NRED];
 #pragma unroll
     for (int i = 0; i < LMZ_NRED; ++i) acc[i] = 0.f;
-    for (int j = threadIdx.x; j < F.npix; j += LMZ_THREADS) {
+    for (int j = threadIdx.x; j < F.npix; j += NT) {
         const long long gi = F.pix_offset + j;
         float Jn[LMZ_NJAC];
 #pragma unroll
@@ -608,7 +609,7 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_normal_eq(
         for (int a = 0; a < LMZ_NJAC; ++a) acc[28 + a] = fmaf(Jn[a], r, acc[28 + a]);
         acc[35] = fmaf(r, r, acc[35]);
     }
-    block_reduce_store<LMZ_NRED>(acc, out + (long long)LMZ_NRED * f);
+    block_reduce_store<LMZ_NRED, NT>(acc, out + (long long)LMZ_NRED * f);
 }
 
 // |x + alpha E|^2 of an accumulated field, one thread per pixel.
@@ -1292,7 +1293,9 @@ int lmz_normal_eq(const lmz_batch *b, const float *jac, const float *resid,
                   double *out, hipStream_t stream) {
     if (int rc = check_batch(b)) return rc;
     LMZ_REQUIRE(jac && resid && out, "jac, resid and out are required");
-    hipLaunchKernelGGL(k_normal_eq, dim3(b->nfeat), dim3(LMZ_THREADS), 0,
+    // one workgroup per feature, 1024 threads so that the largest crops do
+    // not trail the rest
+    hipLaunchKernelGGL(k_normal_eq<1024>, dim3(b->nfeat), dim3(1024), 0,
                        stream, b->d_feat, jac, resid, b->npix_total, out);
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;

@@ -4,10 +4,11 @@ from .Mask import Mask
 from .Optimizer import Optimizer, hipOptimizer
 from .BatchOptimizer import BatchOptimizer
 from .DEEstimator import DEEstimator
+from .RadialEstimator import RadialEstimator
 from .Feature import Feature
 from .Frame import Frame
 from .lm import LMBatch, LMSettings
 
 __all__ = ['Hologram', 'Mask', 'Optimizer', 'hipOptimizer', 'BatchOptimizer',
-           'DEEstimator', 'Feature', 'Frame',
+           'DEEstimator', 'RadialEstimator', 'Feature', 'Frame',
            'LMBatch', 'LMSettings']

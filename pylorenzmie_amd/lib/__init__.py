@@ -12,7 +12,7 @@ import logging
 
 import numpy as np
 
-__all__ = ['LMObject', 'meshgrid']
+__all__ = ['LMObject', 'meshgrid', 'azimuthal']
 
 
 def meshgrid(shape, corner=(0., 0.), flatten=True, dtype=float):

@@ -100,3 +100,72 @@ def test_frame_estimate_and_optimize(api):
     for i, q in enumerate(truth):
         for j, v in enumerate(('x_p', 'y_p', 'z_p', 'a_p', 'n_p')):
             assert abs(df.loc[i, v] - q[j]) <= 6 * df.loc[i, 'd' + v], (i, v)
+
+
+# ------------------------------------------------------- RadialEstimator
+def test_radial_estimator_contract(api):
+    """tests/test_radialestimator.py of the reference: a Series with the
+    particle's keys, x_p / y_p pinned to the coordinate means (corner
+    aware), the model's coordinates untouched, custom bounds respected."""
+    theory, analysis = api
+    shape = (101, 101)
+    model = theory.LorenzMie(coordinates=theory.LorenzMie.meshgrid(shape))
+    model.particle.r_p = [50., 50., 200.]
+    model.particle.a_p, model.particle.n_p = 0.75, 1.45
+    holo = analysis.Hologram(model.hologram().reshape(shape))
+    est = analysis.RadialEstimator(model=model, seed=0)
+    est.settings['maxiter'] = 1
+    assert est.bounds == {'z_p': (10., 600.), 'a_p': (0.25, 10.0), 'n_p': (1.0, 3.0)}
+    before = model.coordinates.copy()
+    res = est.estimate(holo)
+    assert set(model.particle.properties) <= set(res.index)
+    np.testing.assert_array_equal(model.coordinates, before)
+    assert res['x_p'] == pytest.approx(holo.coordinates[0].mean())
+    h2 = analysis.Hologram(holo.data, corner=(10., 20.))
+    res = est.estimate(h2)
+    assert res['x_p'] == pytest.approx(h2.coordinates[0].mean())
+    assert res['y_p'] == pytest.approx(h2.coordinates[1].mean())
+    custom = analysis.RadialEstimator(model=model, bounds={'z_p': (150., 250.)}, seed=0)
+    custom.settings['maxiter'] = 1
+    assert 150. <= custom.estimate(holo)['z_p'] <= 250.
+    assert set(est.properties) == {'popsize', 'bounds', 'settings'}
+
+
+def test_radial_objective_matches_oracle(api):
+    """The spoke objective: the model on (x_p + r, y_p) against the
+    azimuthal profile, per candidate, equals the FP64 oracle's."""
+    theory, analysis = api
+    from pylorenzmie_amd.analysis.DEEstimator import PopulationChi2
+    from pylorenzmie_amd.lib import azimuthal
+    d = noisy_hologram((100.0, 100.0, 150., 0.75, 1.45, 0.), seed=2)
+    prof = azimuthal.avg(d, (100., 100.))
+    spoke = np.vstack([100. + np.arange(prof.size), np.full(prof.size, 100.)])
+    inst = theory.Instrument(noise=0.05, **INSTRUMENT)
+    rng = np.random.default_rng(4)
+    cand = np.zeros((16, 7))
+    cand[:, :2] = 100.
+    cand[:, 2] = rng.uniform(20, 600, 16)
+    cand[:, 3] = rng.uniform(0.25, 3.0, 16)
+    cand[:, 4] = rng.uniform(1.0, 2.5, 16)
+    cand[:, 6] = 1.
+    got = PopulationChi2(spoke, prof, inst, cand[0], cap=120)(cand)
+    for i, q in enumerate(cand):
+        want = np.sum(((ref.hologram(spoke, dict(zip(ref.PARAMS[:6], q)),
+                                     **INSTRUMENT) - prof) / 0.05) ** 2)
+        assert got[i] == pytest.approx(want, rel=1e-5), i
+
+
+def test_radial_estimate_then_fit(api):
+    theory, analysis = api
+    truth = (100.0, 100.0, 150., 0.75, 1.45, 0.)
+    d = noisy_hologram(truth, seed=3)
+    inst = theory.Instrument(noise=0.05, **INSTRUMENT)
+    model = theory.LorenzMie(particle=theory.Sphere(), instrument=inst)
+    est = analysis.RadialEstimator(model=model, seed=2)
+    guess = est.estimate(analysis.Hologram(d))
+    print('radial estimate', dict(guess))
+    assert abs(guess['z_p'] - 150.) < 15 and abs(guess['a_p'] - 0.75) < 0.1
+    assert abs(guess['n_p'] - 1.45) < 0.05
+    res = analysis.Optimizer(model=model).optimize(analysis.Hologram(d))
+    for v, t in zip(ref.PARAMS[:5], truth):
+        assert abs(res[v] - t) <= 5 * res['d' + v], v

@@ -30,7 +30,11 @@ def median(image):
 
 
 def median_filter(image, size=51):
-    """scipy.ndimage.median_filter(image, size) (utilities/normalizer.py:72)."""
+    """scipy.ndimage.median_filter(image, size) (utilities/normalizer.py:72).
+
+    scipy 1.15's rank filter reads outside its extended line buffer when
+    the window exceeds the image by several image sizes (a 51-wide window
+    on a 3 x 2 image); there ``median_filter_brute`` is the reference."""
     return _scipy_median_filter(image, size=size)
 
 

@@ -106,15 +106,19 @@ class Normalizer:
         ny, nx = img.shape
         n = ny * nx
         method, reference = self._resolve(img.shape)
+        cast = None
         if dtype is not None:
             odt = np.dtype(dtype)
+            if odt not in (np.dtype(np.float32), np.dtype(np.float64)):
+                raise TypeError(f'output dtype must be float32 or float64, not {odt}')
         elif on_device:
             odt = np.dtype(np.float32)
         else:
             odt = result_dtype(image.dtype if hasattr(image, 'dtype') else idt,
                                method, reference)
-        if odt not in (np.dtype(np.float32), np.dtype(np.float64)):
-            raise TypeError(f'output dtype must be float32 or float64, not {odt}')
+            if odt not in (np.dtype(np.float32), np.dtype(np.float64)):
+                # float16 frames: computed in float64, rounded to numpy's type
+                cast, odt = odt, np.dtype(np.float64)
         out = t.empty((ny, nx), dtype=_torch_dtype(t, odt), device=img.device)
         lib = _lmz.load_library()
         code, ocode = _lmz.NP_DTYPE_CODE[np.dtype(idt)], _lmz.NP_DTYPE_CODE[odt]
@@ -150,7 +154,10 @@ class Normalizer:
             _lmz.check(lib.lmz_normalize(_lmz.ptr(img), code, n, _lmz.ptr(med),
                                          None, 0, 0., _lmz.ptr(out), ocode,
                                          stream))
-        return out if on_device else out.cpu().numpy()
+        if on_device:
+            return out
+        host = out.cpu().numpy()
+        return host.astype(cast) if cast is not None else host
 
     def _resolve(self, shape):
         """Method and background after the reference's shape check

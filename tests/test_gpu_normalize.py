@@ -315,3 +315,76 @@ def test_gather_out_of_box_reads_nan(torch):
     d = lm.data.cpu().numpy().reshape(5, 4)
     assert np.isnan(d[:, 2:]).all() and np.isfinite(d[:, :2]).all()
     lm.close()
+
+
+# ------------------------------------------------------------ edge cases
+def test_filter_one_row_one_column_and_tiny(torch, lib):
+    """Frames narrower than a tile and windows far wider than the frame,
+    against the filter's definition (periodic 'reflect' windows).  scipy
+    1.15's rank filter reads uninitialized memory once the window exceeds
+    the frame by several frame sizes (a 51-wide window on a 3 x 2 image
+    holds values like 3e-322), so it is no reference there."""
+    rng = np.random.default_rng(13)
+    for shape in ((1, 1), (1, 37), (29, 1), (3, 2), (65, 1), (1, 70)):
+        for dt in (np.uint8, np.float32):
+            img = _img(rng, dt, shape)
+            for size in (1, 2, 5, 51):
+                np.testing.assert_array_equal(_filter_gpu(torch, lib, img, size),
+                                              ref.median_filter_brute(img, size),
+                                              err_msg=f'{shape} {dt} {size}')
+
+
+def test_normalizer_rejects_what_it_cannot_do(torch):
+    from pylorenzmie_amd import _lmz
+    from pylorenzmie_amd.utilities import Normalizer
+    with pytest.raises(ValueError):
+        Normalizer('filter', size=0)(np.ones((4, 4), np.uint8))
+    with pytest.raises(ValueError):
+        Normalizer('filter', size=_lmz.MAX_FILTER_SIZE + 1)(np.ones((4, 4), np.uint8))
+    with pytest.raises(ValueError):
+        Normalizer()(np.ones(5, np.uint8))
+    with pytest.raises(TypeError):
+        Normalizer()(np.ones((4, 4), np.uint8), dtype=np.int32)
+    # int16 frames are converted exactly and give numpy's float64 result;
+    # float16 frames come back as float16, computed in float64
+    img = (np.arange(30).reshape(5, 6) - 7).astype(np.int16)
+    got = Normalizer('median')(img)
+    want = ref.normalize(img, 'median')
+    assert got.dtype == want.dtype
+    np.testing.assert_array_equal(got, want)
+    img = (np.arange(30).reshape(5, 6) + 3).astype(np.float16)
+    got = Normalizer('median')(img)
+    want = ref.normalize(img, 'median')
+    assert got.dtype == want.dtype == np.float16
+    np.testing.assert_allclose(got.astype(float), want.astype(float), rtol=1e-3)
+
+
+def test_frame_device_boxes_at_edges(torch):
+    """Boxes clipped by the frame's edge (numpy slicing) on a device frame
+    give the same batch as on a host frame."""
+    from pylorenzmie_amd.analysis import Frame
+    from pylorenzmie_amd.analysis import lm as lmmod
+    from pylorenzmie_amd.theory import Instrument
+    rng = np.random.default_rng(5)
+    data = (1. + 0.05 * rng.standard_normal((120, 150))).astype(np.float32)
+    inst = Instrument(noise=0.05)
+    seen = []
+    orig = lmmod.LMBatch.fetch
+
+    def spy(self):
+        seen.append(self.data.cpu().numpy().copy())
+        return orig(self)
+    lmmod.LMBatch.fetch = spy
+    try:
+        for d in (data, torch.from_numpy(data).cuda()):
+            f = Frame(d, instrument=inst)
+            f.bboxes = [((100, 80), 80, 80), ((0, 0), 50, 50)]
+            for ft in f.features:
+                ft.optimizer.mask = None
+                ft.particle.z_p = 100.
+                ft.optimizer.settings['max_nfev'] = 3
+            f.optimize()
+    finally:
+        lmmod.LMBatch.fetch = orig
+    assert seen[0].size == 40 * 50 + 50 * 50
+    np.testing.assert_array_equal(seen[1], seen[0])

@@ -229,4 +229,56 @@ __host__ __device__ inline void slide_settle(Hist &h, int rank, int &m,
     }
 }
 
+// Two-level sliding median of 16-bit values: a 256-bin histogram H of the
+// high bytes of the window, and a 256-bin histogram F of the low bytes of
+// the values whose high byte is m, the high byte of the median.  `bc`
+// counts values with high byte < m, `bf` values with high byte m and low
+// byte < mf, the low byte of the median.  When the median's high byte moves,
+// F is rebuilt from the window (refill), which a slowly varying
+// background rarely asks for.
+struct Slide16 {
+    int m = 0, bc = 0, mf = 0, bf = 0;
+};
+
+template <class Hist>
+__host__ __device__ inline void slide16_add(Hist &H, Hist &F, Slide16 &st,
+                                            uint32_t v) {
+    const int h = (int)(v >> 8), l = (int)(v & 0xFFu);
+    hist_add(H, (uint32_t)h);
+    st.bc += (h < st.m);
+    if (h == st.m) {
+        hist_add(F, (uint32_t)l);
+        st.bf += (l < st.mf);
+    }
+}
+template <class Hist>
+__host__ __device__ inline void slide16_remove(Hist &H, Hist &F, Slide16 &st,
+                                               uint32_t v) {
+    const int h = (int)(v >> 8), l = (int)(v & 0xFFu);
+    hist_sub(H, (uint32_t)h);
+    st.bc -= (h < st.m);
+    if (h == st.m) {
+        hist_sub(F, (uint32_t)l);
+        st.bf -= (l < st.mf);
+    }
+}
+// Median of rank `rank` after updates: settle the high byte; if it moved,
+// refill(m) rebuilds F for the new high byte; then settle the low byte.
+template <class Hist, class Refill>
+__host__ __device__ inline uint32_t slide16_settle(Hist &H, Hist &F,
+                                                   Slide16 &st, int rank,
+                                                   const Refill &refill) {
+    const int m0 = st.m;
+    slide_settle(H, rank, st.m, st.bc);
+    const int r = rank - st.bc;
+    if (st.m != m0) {
+        F.clear();
+        refill(st.m);
+        st.mf = hist_find(F, r, &st.bf);
+    } else {
+        slide_settle(F, r, st.mf, st.bf);
+    }
+    return ((uint32_t)st.m << 8) | (uint32_t)st.mf;
+}
+
 }  // namespace lmz

@@ -299,6 +299,101 @@ __global__ __launch_bounds__(64) void k_medfilt_u8(
     }
 }
 
+// Median filter of a 16-bit frame: as k_medfilt_u8, with the two-level
+// histogram of Slide16 (high bytes of the window; low bytes of the values
+// sharing the median's high byte), refilled from the frame when the
+// median's high byte moves.
+__global__ __launch_bounds__(64) void k_medfilt_u16(
+    const uint16_t *__restrict__ img, int ny, int nx, int size, int lo,
+    int rows, uint16_t *__restrict__ bg, void *__restrict__ out, int odt) {
+    __shared__ uint32_t histH[128 * 64];
+    __shared__ uint32_t histF[128 * 64];
+    __shared__ uint16_t seg[2][SEG_MAX + 2];
+    __shared__ int colmap[SEG_MAX];
+    const int lane = threadIdx.x;
+    const int x0 = blockIdx.x * 64, x = x0 + lane;
+    const int y0 = blockIdx.y * rows;
+    if (y0 >= ny) return;                        // uniform over the block
+    const int y1 = min(y0 + rows, ny);
+    const int w = 63 + size;
+    const bool active = x < nx;
+    for (int p = lane; p < w; p += 64) colmap[p] = reflect_index(x0 - lo + p, nx);
+    __syncthreads();
+    uint16_t ra[SEG_PER_LANE], rb[SEG_PER_LANE];
+    auto load = [&](uint16_t(&r)[SEG_PER_LANE], int row) {
+        const uint16_t *src = img + (long long)reflect_index(row, ny) * nx;
+#pragma unroll
+        for (int k = 0; k < SEG_PER_LANE; ++k) {
+            const int p = lane + 64 * k;
+            r[k] = p < w ? src[colmap[p]] : (uint16_t)0;
+        }
+    };
+    auto store = [&](const uint16_t(&r)[SEG_PER_LANE], int buf) {
+#pragma unroll
+        for (int k = 0; k < SEG_PER_LANE; ++k) {
+            const int p = lane + 64 * k;
+            if (p < w) seg[buf][p] = r[k];
+        }
+    };
+    LdsHist H{histH + lane}, F{histF + lane};
+    if (active) H.clear();
+    load(ra, y0 - lo);
+    for (int r = 0; r < size; ++r) {
+        store(ra, 0);
+        __syncthreads();
+        if (r + 1 < size) load(ra, y0 - lo + r + 1);
+        if (active) {
+            const uint16_t *sp = seg[0] + lane;
+            for (int c = 0; c < size; ++c) hist_add(H, (uint32_t)(sp[c] >> 8));
+        }
+        __syncthreads();
+    }
+    const int rank = size * size / 2;
+    Slide16 st;
+    int ycur = y0;
+    // low bytes of the window values whose high byte is m, from the frame
+    auto refill = [&](int m) {
+        for (int r = 0; r < size; ++r) {
+            const uint16_t *row = img + (long long)reflect_index(ycur - lo + r, ny) * nx;
+            for (int c = 0; c < size; ++c) {
+                const uint32_t v = row[colmap[lane + c]];
+                if ((int)(v >> 8) == m) hist_add(F, v & 0xFFu);
+            }
+        }
+    };
+    if (active) {
+        st.m = hist_find(H, rank, &st.bc);
+        F.clear();
+        refill(st.m);
+        st.mf = hist_find(F, rank - st.bc, &st.bf);
+        emit<uint16_t>(img, nx, x, y0, (uint16_t)((st.m << 8) | st.mf), bg, out, odt);
+    }
+    if (y0 + 1 < y1) {
+        load(ra, y0 - lo);
+        load(rb, y0 - lo + size);
+    }
+    for (int y = y0 + 1; y < y1; ++y) {
+        store(ra, 0);
+        store(rb, 1);
+        __syncthreads();
+        if (y + 1 < y1) {
+            load(ra, y - lo);
+            load(rb, y - lo + size);
+        }
+        if (active) {
+            const uint16_t *so = seg[0] + lane, *sn = seg[1] + lane;
+            for (int c = 0; c < size; ++c) {
+                slide16_remove(H, F, st, so[c]);
+                slide16_add(H, F, st, sn[c]);
+            }
+            ycur = y;
+            const uint32_t v = slide16_settle(H, F, st, rank, refill);
+            emit<uint16_t>(img, nx, x, y, (uint16_t)v, bg, out, odt);
+        }
+        __syncthreads();
+    }
+}
+
 // Median filter for 16-bit, float32 and float64 frames: one thread per
 // output pixel of an 8 x 8 tile, radix select over the window's keys (8-bit
 // digits, starting below the bits all of the tile's keys share).  STAGED:
@@ -554,10 +649,19 @@ int lmz_median_filter(const void *image, int32_t dtype, int32_t ny, int32_t nx,
         LMZ_HIP(hipGetLastError());
         return LMZ_OK;
     }
+    if (dtype == LMZ_U16) {
+        const int bx = (nx + 63) / 64;
+        const int want = 2 * device_cus();
+        int rows = 32;
+        while (rows > 8 && (long long)bx * ((ny + rows - 1) / rows) < want) rows /= 2;
+        const dim3 grid(bx, (ny + rows - 1) / rows);
+        hipLaunchKernelGGL(k_medfilt_u16, grid, dim3(64), 0, stream,
+                           (const uint16_t *)image, ny, nx, size, size / 2, rows,
+                           (uint16_t *)background, out, out_dtype);
+        LMZ_HIP(hipGetLastError());
+        return LMZ_OK;
+    }
     switch (dtype) {
-    case LMZ_U16:
-        return medfilt_radix<LMZ_U16>(image, ny, nx, size, background, out,
-                                      out_dtype, stream);
     case LMZ_F32:
         return medfilt_radix<LMZ_F32>(image, ny, nx, size, background, out,
                                       out_dtype, stream);

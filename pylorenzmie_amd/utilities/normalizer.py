@@ -8,7 +8,8 @@ background, ``where(background > 0, background, 1.)``:
   select in device memory, then lmz_normalize);
 * ``'filter'`` -- scipy.ndimage.median_filter(image, size) with its 'reflect'
   boundary (lmz_median_filter: a sliding 256-bin histogram for 8-bit frames,
-  a per-pixel radix select otherwise), fused with the division;
+  a two-level one for 16-bit frames, a per-pixel radix select for float
+  frames), fused with the division;
 * ``'reference'`` -- a background image or scalar (lmz_normalize); an image
   of another shape falls back to the median with a warning, as the
   reference does.

@@ -151,9 +151,51 @@ void medfilt_radix_host(const void *image, int ny, int nx, int size, void *bgv) 
 extern "C" {
 
 // scipy.ndimage.median_filter by the device algorithms: path 0 the sliding
-// histogram (8-bit frames), path 1 the radix select.
+// histogram (8-bit frames), path 1 the radix select, path 2 the two-level
+// sliding histogram (16-bit frames).
 int chk_median_filter(const void *image, int dt, int ny, int nx, int size,
                       void *bg, int path) {
+    if (path == 2) {
+        if (dt != LMZ_U16) return 1;
+        const uint16_t *img = (const uint16_t *)image;
+        uint16_t *out = (uint16_t *)bg;
+        const int lo = size / 2, rank = size * size / 2;
+        HostHist H, F;
+        for (int x = 0; x < nx; ++x)
+            for (int y0 = 0; y0 < ny; y0 += 32) {
+                auto px = [&](int yy, int c) {
+                    return (uint32_t)img[(long long)reflect_index(yy, ny) * nx +
+                                         reflect_index(x - lo + c, nx)];
+                };
+                int ycur = y0;
+                auto refill = [&](int m) {
+                    for (int r = 0; r < size; ++r)
+                        for (int c = 0; c < size; ++c) {
+                            const uint32_t v = px(ycur - lo + r, c);
+                            if ((int)(v >> 8) == m) hist_add(F, v & 0xFFu);
+                        }
+                };
+                H.clear();
+                for (int r = 0; r < size; ++r)
+                    for (int c = 0; c < size; ++c) hist_add(H, px(y0 - lo + r, c) >> 8);
+                Slide16 st;
+                st.m = hist_find(H, rank, &st.bc);
+                F.clear();
+                refill(st.m);
+                st.mf = hist_find(F, rank - st.bc, &st.bf);
+                out[(long long)y0 * nx + x] = (uint16_t)((st.m << 8) | st.mf);
+                for (int y = y0 + 1; y < std::min(y0 + 32, ny); ++y) {
+                    for (int c = 0; c < size; ++c) {
+                        slide16_remove(H, F, st, px(y - 1 - lo, c));
+                        slide16_add(H, F, st, px(y - 1 - lo + size, c));
+                    }
+                    ycur = y;
+                    out[(long long)y * nx + x] =
+                        (uint16_t)slide16_settle(H, F, st, rank, refill);
+                }
+            }
+        return 0;
+    }
     if (path == 0) {
         if (dt != LMZ_U8) return 1;
         const uint8_t *img = (const uint8_t *)image;

@@ -164,7 +164,7 @@ def test_filter_algorithms_match_scipy(chk, dt, shape, size):
     for smooth in ((False, True) if dt >= 2 else (False,)):
         img = _image(rng, dt, shape, smooth)
         want = median_filter(img, size=size)
-        paths = (0, 1) if dt == 0 else (1,)
+        paths = {0: (0, 1), 1: (1, 2)}.get(dt, (1,))
         for path in paths:
             got = _host_filter(chk, img, dt, size, path)
             np.testing.assert_array_equal(got, want, err_msg=f'path {path}')
@@ -180,6 +180,11 @@ def test_filter_algorithms_on_the_camera_frame(chk, frame_golden):
     f32 = (img.astype(np.float32) / np.float32(100.))
     np.testing.assert_array_equal(_host_filter(chk, f32, 2, 51, 1),
                                   median_filter(f32, size=51))
+    # 16-bit: the frame scaled to 12 bits, so the median's high byte moves
+    u16 = (img.astype(np.uint16) * 16 + np.arange(img.size, dtype=np.uint16).reshape(
+        img.shape) % 16)
+    np.testing.assert_array_equal(_host_filter(chk, u16, 1, 51, 2),
+                                  median_filter(u16, size=51))
 
 
 def test_filter_planted_mistake_is_caught(chk):

@@ -23,6 +23,7 @@ from .. import _lmz
 from ..lib import LMObject
 from ..theory import LorenzMie
 from ..theory.LorenzMie import PARAMS
+from .BaseEstimator import BaseEstimator
 from .Hologram import Hologram
 from .Mask import Mask
 
@@ -94,7 +95,7 @@ class PopulationChi2:
 
 
 @dataclass
-class DEEstimator(LMObject):
+class DEEstimator(BaseEstimator):
     """Estimate z_p, a_p, n_p (the ``bounds`` keys) by differential
     evolution on a ``fraction`` of the pixels; x_p, y_p pinned to the
     crop's centre (analysis/DEEstimator.py:124-180).  The population is

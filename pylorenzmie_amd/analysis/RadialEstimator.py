@@ -21,12 +21,13 @@ from ..lib import LMObject, azimuthal
 from ..theory import LorenzMie
 from ..theory.LorenzMie import PARAMS
 from .DEEstimator import DEFAULT_BOUNDS, PopulationChi2
+from .BaseEstimator import BaseEstimator
 from .Hologram import Hologram
 from .Mask import host_data
 
 
 @dataclass
-class RadialEstimator(LMObject):
+class RadialEstimator(BaseEstimator):
     """Estimate the ``bounds`` keys (default z_p, a_p, n_p) by differential
     evolution on the azimuthal profile (analysis/RadialEstimator.py:43-166);
     x_p and y_p are pinned to the crop's mean coordinates."""

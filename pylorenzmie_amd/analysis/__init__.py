@@ -3,6 +3,7 @@ from .Hologram import Hologram
 from .Mask import Mask
 from .Optimizer import Optimizer, hipOptimizer
 from .BatchOptimizer import BatchOptimizer
+from .BaseEstimator import BaseEstimator
 from .DEEstimator import DEEstimator
 from .RadialEstimator import RadialEstimator
 from .Feature import Feature
@@ -10,5 +11,5 @@ from .Frame import Frame
 from .lm import LMBatch, LMSettings
 
 __all__ = ['Hologram', 'Mask', 'Optimizer', 'hipOptimizer', 'BatchOptimizer',
-           'DEEstimator', 'RadialEstimator', 'Feature', 'Frame',
+           'BaseEstimator', 'DEEstimator', 'RadialEstimator', 'Feature', 'Frame',
            'LMBatch', 'LMSettings']

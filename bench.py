@@ -23,6 +23,11 @@ Also measured:
   cpu_baseline  the FP64 oracle (numpy model + scipy MINPACK LM) on one
           feature on the host.
 
+--extra superposed | frame | de measures a path around the fitter instead:
+  the five-sphere hologram of the reference's MultiSphere example, a
+  frame's features fitted as one batch from device memory, and the DE
+  estimator's GPU population (one JSON line each).
+
 --config 1..4 measures the other configurations instead (one JSON line):
   1  batched forward holograms (1024 crops, side U{64..400}): holograms/s,
      FP64 TFLOP/s, error against the oracle at sampled pixels;
@@ -602,6 +607,155 @@ def run_config(args):
     print(json.dumps(line), flush=True)
 
 
+# ---------------------------------------------------------------------------
+# Rows around the path (--extra NAME): one JSON line each, with the FP64
+# oracle on one host core beside the GPU figure where the reference's own
+# path is a CPU one.
+def run_extra(args):
+    import torch
+    from pylorenzmie_amd.theory import Instrument, LorenzMie, Sphere
+    from oracle import lorenzmie as ref
+    line = {'extra': args.extra}
+    if args.extra == 'superposed':
+        # docs/MultiSphere.ipynb: five default spheres on a 480 x 640 grid
+        # (4.43 s per hologram() there, hardware unstated)
+        shape = (480, 640)
+        parts = [Sphere(x_p=100. * (i + 1), y_p=240., z_p=100. + 50. * i)
+                 for i in range(5)]
+        m = LorenzMie(coordinates=LorenzMie.meshgrid(shape), particle=parts,
+                      instrument=Instrument())
+        h = m.hologram()
+        torch.cuda.synchronize()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            h = m.hologram()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        c = ref.meshgrid(shape)
+        dicts = [dict(x_p=p.x_p, y_p=p.y_p, z_p=p.z_p, a_p=p.a_p, n_p=p.n_p,
+                      k_p=p.k_p) for p in parts]
+        t0 = time.perf_counter()
+        want = ref.hologram(c, dicts, **INSTRUMENT)
+        tc = time.perf_counter() - t0
+        line.update(workload='hologram of 5 spheres (a_p 1 um, n_p 1.5) on a '
+                             '480 x 640 grid, one fused pass',
+                    ms_per_hologram=t * 1e3, max_abs_err_vs_oracle=float(
+                        np.abs(h - want).max()),
+                    cpu_baseline={'value': tc * 1e3, 'unit': 'ms per hologram',
+                                  'cores': 1, 'kind': 'port'},
+                    reference_notebook_ms=4430.)
+    elif args.extra == 'frame':
+        from pylorenzmie_amd.analysis import Frame
+        rng = np.random.default_rng(11)
+        ny, nx, side, k = 1024, 1280, 201, 16
+        inst = Instrument(noise=0.05, **INSTRUMENT)
+        truth, boxes = [], []
+        for i in range(k):
+            x0, y0 = 40 + 300 * (i % 4), 20 + 250 * (i // 4)
+            truth.append((x0 + 100 + rng.uniform(-2, 2), y0 + 100 + rng.uniform(-2, 2),
+                          rng.uniform(100., 250.), rng.uniform(0.5, 1.5),
+                          rng.uniform(1.4, 1.6), 0.))
+            boxes.append(((x0, y0), side, side))
+        frame = np.ones((ny, nx))
+        for q, ((x0, y0), _, _) in zip(truth, boxes):
+            cc = ref.meshgrid((side, side), corner=(x0, y0))
+            frame[y0:y0 + side, x0:x0 + side] = ref.hologram(
+                cc, dict(zip(ref.PARAMS[:6], q)), **INSTRUMENT).reshape(side, side)
+        frame += 0.05 * rng.standard_normal(frame.shape)
+        dev = torch.from_numpy(frame.astype(np.float32)).cuda()
+
+        def setup(data):
+            f = Frame(data, instrument=inst)
+            f.bboxes = boxes
+            for ft, q in zip(f.features, truth):
+                ft.particle.x_p += 1.5
+                ft.particle.z_p, ft.particle.a_p, ft.particle.n_p = \
+                    q[2] * 1.02, q[3] * 1.03, q[4] - 0.01
+                ft.optimizer.settings.update(ftol=1e-9, xtol=1e-9, gtol=1e-10)
+            return f
+        f = setup(dev)
+        f.optimize()
+        torch.cuda.synchronize()
+        f = setup(dev)
+        np.random.seed(1)
+        t0 = time.perf_counter()
+        batched = f.optimize()
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - t0
+        f2 = setup(frame.astype(np.float32))
+        np.random.seed(1)
+        t0 = time.perf_counter()
+        seq = [ft.optimize() for ft in f2.features]
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t0
+        agree = np.mean([all(abs(batched.iloc[i][v] - seq[i][v]) <=
+                             1e-4 * max(abs(seq[i][v]), 1e-3)
+                             for v in ('x_p', 'y_p', 'z_p', 'a_p', 'n_p'))
+                         for i in range(k)])
+        line.update(workload=f'{k} features of 201x201 (10% masks) in a '
+                             '1280x1024 frame in device memory',
+                    frame_optimize_ms=tb * 1e3, fits_per_s=k / tb,
+                    sequential_feature_optimize_ms=ts * 1e3,
+                    speedup_vs_sequential=ts / tb, agreement=float(agree),
+                    success=float(batched['success'].mean()))
+    elif args.extra == 'de':
+        from pylorenzmie_amd.analysis import DEEstimator, Hologram, Mask
+        from pylorenzmie_amd.analysis.DEEstimator import PopulationChi2
+        q = (100.3, 99.6, 150., 0.75, 1.45, 0.)
+        c = ref.meshgrid((201, 201))
+        rng = np.random.default_rng(2)
+        d = (ref.hologram(c, dict(zip(ref.PARAMS[:6], q)), **INSTRUMENT) +
+             0.05 * rng.standard_normal(201 * 201)).reshape(201, 201)
+        inst = Instrument(noise=0.05, **INSTRUMENT)
+        est = DEEstimator(model=LorenzMie(particle=Sphere(), instrument=inst),
+                          seed=1)
+        np.random.seed(1)
+        est.estimate(Hologram(d))
+        torch.cuda.synchronize()
+        np.random.seed(1)
+        t0 = time.perf_counter()
+        g = est.estimate(Hologram(d))
+        te = time.perf_counter() - t0
+        # one generation's objective: the population on the GPU vs the oracle
+        np.random.seed(3)
+        data, coords = Mask(fraction=0.02).apply(Hologram(d))
+        pop = np.zeros((30, 7))
+        pop[:, :2] = q[:2]
+        pop[:, 2] = rng.uniform(10., 600., 30)
+        pop[:, 3] = rng.uniform(0.25, 10., 30)
+        pop[:, 4] = rng.uniform(1.0, 3.0, 30)
+        pop[:, 6] = 1.
+        chi2 = PopulationChi2(coords, data, inst, pop[0],
+                              cap=_lmz_cap(10., 3.0))
+        chi2(pop)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            chi2(pop)
+        tg = (time.perf_counter() - t0) / 10
+        t0 = time.perf_counter()
+        for p in pop[:6]:
+            np.sum(((ref.hologram(coords, dict(zip(ref.PARAMS[:6], p)),
+                                  **INSTRUMENT) - data) / 0.05) ** 2)
+        tc = (time.perf_counter() - t0) / 6
+        line.update(workload='DE estimate of z_p, a_p, n_p on a 201x201 crop '
+                             '(2% mask, popsize 10, default bounds)',
+                    estimate_ms=te * 1e3,
+                    estimate={k2: float(g[k2]) for k2 in ('z_p', 'a_p', 'n_p')},
+                    generation_ms=tg * 1e3, candidates_per_s=30 / tg,
+                    cpu_baseline={'value': 1. / tc, 'unit': 'candidates/s',
+                                  'cores': 1, 'kind': 'port'})
+    else:
+        raise SystemExit('--extra must be superposed, frame or de')
+    print(json.dumps(line), flush=True)
+
+
+def _lmz_cap(a_p, n_p):
+    from pylorenzmie_amd import _lmz
+    return _lmz.nmax_cap(a_p, n_p, 0., INSTRUMENT['n_m'], INSTRUMENT['wavelength'])
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument('--gpus', type=int, default=1)
@@ -627,6 +781,9 @@ def main():
     ap.add_argument('--no-fd', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--seed', type=int, default=3)
+    ap.add_argument('--extra', default='',
+                    help='superposed | frame | de: measure that path instead '
+                         'of the headline (one JSON line)')
     ap.add_argument('--config', type=int, default=0,
                     help='1-4: measure that configuration instead of the '
                          'headline (one JSON line)')
@@ -648,6 +805,9 @@ def main():
         return
     if args.config:
         run_config(args)
+        return
+    if args.extra:
+        run_extra(args)
         return
 
     truth, start = draw_features(args.nfeat, args.side, args.seed)

@@ -374,9 +374,9 @@ __global__ __launch_bounds__(NT) void k_fd_jacobian(
     const int32_t *__restrict__ fd_nmax, const float *__restrict__ fd_coef,
     long long fd_coef_stride, const double *__restrict__ fd_h, int vary,
     const float *__restrict__ coords, long long npix_total, int flags,
-    const float *__restrict__ data, const float *__restrict__ rin,
-    long long rin_stride, double *__restrict__ partials,
-    const int32_t *__restrict__ list, int32_t *count) {
+    const float *__restrict__ rin, long long rin_stride,
+    double *__restrict__ partials, const int32_t *__restrict__ list,
+    int32_t *count) {
   __shared__ int next_w;
   int chunks_done = 0;
   const int nwork = count ? count[0] : (int)gridDim.x;
@@ -450,7 +450,6 @@ __global__ __launch_bounds__(NT) void k_fd_jacobian(
         for (int a = 0; a < LMZ_NJAC; ++a) acc[28 + a] = fmaf(Jn[a], r, acc[28 + a]);
         acc[35] = fmaf(r, r, acc[35]);
     }
-    (void)data;
     block_reduce_store<LMZ_NRED, NT>(acc, partials + (long long)LMZ_NRED * t);
     __syncthreads();   // the reduction buffer is reused by the next tile
   }
@@ -720,10 +719,10 @@ void launch_jacobian(const lmz_batch *b, const double *params,
 void launch_fd_jacobian(const lmz_batch *b, const double *params,
                         const int32_t *sel, int flip, const int32_t *nmax,
                         const float *coef, const FdTables &fd, int vary,
-                        const float *coords, int flags, const float *data,
-                        const float *rin, long long rin_stride,
-                        double *partials, const uint8_t *active,
-                        const TileList *tl, double *work, hipStream_t stream) {
+                        const float *coords, int flags, const float *rin,
+                        long long rin_stride, double *partials,
+                        const uint8_t *active, const TileList *tl,
+                        double *work, hipStream_t stream) {
     hipLaunchKernelGGL(k_fd_setup, dim3((b->nfeat + 63) / 64), dim3(64), 0,
                        stream, b->nfeat, params, fd.eps, active, fd.h,
                        fd.params);
@@ -746,7 +745,7 @@ void launch_fd_jacobian(const lmz_batch *b, const double *params,
                            b->d_tile_feature, b->nfeat, params, sel, flip,
                            b->coef_buffers, nmax, coef, fd.params, fd.nmax,
                            fd.coef, fd.coef_stride, fd.h, vary, coords,
-                           b->npix_total, flags, data, rin, rin_stride, partials,
+                           b->npix_total, flags, rin, rin_stride, partials,
                            list, count);
     };
     if (nt == 64) go(k_fd_jacobian<64>, 64);

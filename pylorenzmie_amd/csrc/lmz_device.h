@@ -72,10 +72,10 @@ struct FdTables {
 void launch_fd_jacobian(const lmz_batch *b, const double *params,
                         const int32_t *sel, int flip, const int32_t *nmax,
                         const float *coef, const FdTables &fd, int vary,
-                        const float *coords, int flags, const float *data,
-                        const float *rin, long long rin_stride,
-                        double *partials, const uint8_t *active,
-                        const TileList *tl, double *work, hipStream_t stream);
+                        const float *coords, int flags, const float *rin,
+                        long long rin_stride, double *partials,
+                        const uint8_t *active, const TileList *tl,
+                        double *work, hipStream_t stream);
 void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
                       const TileList &tl, hipStream_t stream);
 // Persistent grids of the LM's pixel launches under the batch's schedule.

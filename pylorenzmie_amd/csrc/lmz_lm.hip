@@ -422,8 +422,7 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
         int vary = 0;
         for (int i = 0; i < LMZ_NJAC; ++i) vary |= set.vary[i] ? (1 << i) : 0;
         launch_fd_jacobian(b, params, sel, 0, nmax, coef, fd, vary, coords,
-                           flags, data, resid, np, partials, mask_j, &tl_j,
-                           work, s);
+                           flags, resid, np, partials, mask_j, &tl_j, work, s);
     } else {
         launch_jacobian(b, params, sel, 0, nmax, coef, dcoef, coords, flags,
                         data, resid, np, nullptr, nullptr, partials, nullptr,

@@ -693,12 +693,22 @@ def run_extra(args):
                              1e-4 * max(abs(seq[i][v]), 1e-3)
                              for v in ('x_p', 'y_p', 'z_p', 'a_p', 'n_p'))
                          for i in range(k)])
+        # the DE estimates of all features, one search after another
+        fe = setup(dev)
+        for i, ft in enumerate(fe.features):
+            ft.estimator.seed = i
+        np.random.seed(2)
+        t0 = time.perf_counter()
+        fe.estimate()
+        torch.cuda.synchronize()
+        est_ms = (time.perf_counter() - t0) * 1e3
         line.update(workload=f'{k} features of 201x201 (10% masks) in a '
                              '1280x1024 frame in device memory',
                     frame_optimize_ms=tb * 1e3, fits_per_s=k / tb,
                     sequential_feature_optimize_ms=ts * 1e3,
                     speedup_vs_sequential=ts / tb, agreement=float(agree),
-                    success=float(batched['success'].mean()))
+                    success=float(batched['success'].mean()),
+                    frame_estimate_ms=est_ms)
     elif args.extra == 'de':
         from pylorenzmie_amd.analysis import DEEstimator, Hologram, Mask
         from pylorenzmie_amd.analysis.DEEstimator import PopulationChi2

@@ -125,6 +125,17 @@ class DEEstimator(BaseEstimator):
 
     def estimate(self, hologram: Hologram) -> pd.Series:
         from scipy.optimize import differential_evolution
+        prob = self._problem(hologram)
+        chi2 = PopulationChi2(prob['coords'], prob['data'],
+                              self.model.instrument, prob['base'], prob['cap'])
+        result = differential_evolution(_objective(prob, chi2),
+                                        list(self.bounds.values()),
+                                        **self._de_settings())
+        return self._finish(result.x)
+
+    def _problem(self, hologram: Hologram) -> dict:
+        """Pin x_p, y_p to the crop's mean coordinates, draw the mask and
+        collect what the objective needs (analysis/DEEstimator.py:124-150)."""
         model = self.model
         model.particle.x_p = float(hologram.coordinates[0].mean())
         model.particle.y_p = float(hologram.coordinates[1].mean())
@@ -143,23 +154,33 @@ class DEEstimator(BaseEstimator):
         hi = {k: v[1] for k, v in self.bounds.items()}
         cap = _lmz.nmax_cap(hi.get('a_p', base[3]), hi.get('n_p', base[4]),
                             hi.get('k_p', base[5]), inst.n_m, inst.wavelength)
-        chi2 = PopulationChi2(coords, data, inst, base, cap)
-        cols = [PARAMS.index(v) for v in variables]
+        return dict(data=data, coords=coords, base=base, cap=cap,
+                    cols=[PARAMS.index(v) for v in variables], instrument=inst)
 
-        def objective(x):
-            x = np.atleast_2d(np.asarray(x, float).T)      # (S, nvar)
-            p = np.repeat(base[None, :], x.shape[0], axis=0)
-            p[:, cols] = x
-            return chi2(p)
-
+    def _de_settings(self, seed=None) -> dict:
         settings = {k: v for k, v in self.settings.items() if k != 'workers'}
         settings['updating'] = 'deferred'
-        result = differential_evolution(objective, list(self.bounds.values()),
-                                        popsize=self.popsize, seed=self.seed,
-                                        vectorized=True, **settings)
-        found = dict(zip(variables, result.x))
+        return dict(popsize=self.popsize,
+                    seed=self.seed if seed is None else seed,
+                    vectorized=True, **settings)
+
+    def _finish(self, x) -> pd.Series:
+        model = self.model
+        found = dict(zip(self.bounds.keys(), x))
         for k in ('x_p', 'y_p', 'z_p'):
             if k in found:
                 found[k] -= model.particle.r_0['xyz'.index(k[0])]
         model.properties = found
         return pd.Series(model.particle.properties)
+
+
+def _objective(prob: dict, chi2):
+    """scipy's vectorized objective: columns of x are candidates."""
+    base, cols = prob['base'], prob['cols']
+
+    def objective(x):
+        x = np.atleast_2d(np.asarray(x, float).T)      # (S, nvar)
+        p = np.repeat(base[None, :], x.shape[0], axis=0)
+        p[:, cols] = x
+        return chi2(p)
+    return objective

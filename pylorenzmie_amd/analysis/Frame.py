@@ -135,6 +135,9 @@ class Frame(Hologram):
                                   'of this library: set bboxes')
 
     def estimate(self) -> None:
+        """Starting values of every feature, one DE search after another
+        (analysis/Frame.py:158-161), each generation's population on the
+        GPU."""
         for feature in self.features:
             feature.estimate()
 

@@ -310,23 +310,18 @@ def run_fd(args, truth, start, data_host):
 
 
 def run_cpu(args, start, data_host):
-    """FP64 oracle: numpy model + scipy MINPACK LM, as analysis/Optimizer.py
-    drives it, on one feature."""
-    from scipy.optimize import least_squares
-    from oracle import lorenzmie as ref
+    """FP64 oracle: numpy model + scipy MINPACK LM at the reference's
+    default settings with its SVD uncertainties (oracle/fit.py, as
+    analysis/Optimizer.py:196-278 drives it), one feature at a time."""
+    from oracle import fit, lorenzmie as ref
     side = args.side
     c = ref.meshgrid((side, side))
     n = 0
     t0 = time.perf_counter()
     while True:
         i = n % len(start)
-        d = data_host[i].ravel().astype(float)
-
-        def resid(v):
-            p = dict(zip(PARAMS[:6], v[:6]))
-            return (ref.hologram(c, p, alpha=v[6], **INSTRUMENT) - d) / args.sigma
-        least_squares(resid, start[i], method='lm', ftol=1e-4, xtol=1e-6,
-                      gtol=1e-6, max_nfev=2000, diff_step=1e-5, x_scale='jac')
+        fit.fit(data_host[i], c, dict(zip(PARAMS, start[i])), variables=PARAMS,
+                instrument=INSTRUMENT, noise=args.sigma)
         n += 1
         if time.perf_counter() - t0 > args.cpu_seconds or n >= args.cpu_max_fits:
             break

@@ -38,7 +38,10 @@ Parity pins (what checks each function; see tests/test_oracle.py):
   and alpha = 0 giving I == 1, the alpha closed form, golden vectors.
 * jacobian -- agreement of 5-point and 7-point stencils, and
   directional derivatives of the independent model.
-* fit -- truth recovery on synthetic data and golden reference fits.
+* fit -- the reference Optimizer's golden fits (values, SVD
+  uncertainties, reduced chi2 at default and tight tolerances) to 1e-9
+  relative (identical in practice); a finite-difference step other than
+  1e-5 or a reduced chi2 over npix misses them.
 * normalize -- the filter against its definition (explicit reflected
   windows and an exact rank, median_filter_brute) at odd, even and
   frame-exceeding sizes; the result types against numpy's promotion of

@@ -240,3 +240,44 @@ def test_dimer_golden(superposed_golden):
                     r_0=(x, y, z)) for o in (off, -off)]
     h = ref.hologram(g['coords'], members, **INSTRUMENT)
     assert np.abs(h - g['dimer_holo']).max() < 1e-12
+
+
+# ---- the fitter (oracle/fit.py) against the reference's fits ---------------
+@pytest.mark.parametrize('name', ['crop00', 'crop10', 'crop07'])
+def test_fit_reproduces_reference_fits(fits_golden, name):
+    """MINPACK LM on the oracle's hologram at the reference's settings lands
+    where the reference's Optimizer did: values, SVD uncertainties, reduced
+    chi2, success and npix, at default and tight tolerances."""
+    from oracle import fit
+    z, meta = fits_golden
+    data = z[name + '_data']
+    c = ref.meshgrid(data.shape)
+    start = dict(zip(fit.VARS6, z[name + '_start']))
+    for mode, extra in (('default', {}), ('tight', meta['tight_settings'])):
+        r = fit.fit(data, c, start, instrument=INSTRUMENT,
+                    noise=float(z[name + '_sigma']), settings=extra)
+        g = meta['fits'][f'{name}_{mode}']
+        for k in ('redchi',) + tuple(fit.VARS6) + tuple('d' + v for v in fit.VARS6):
+            assert r[k] == pytest.approx(g[k], rel=1e-9, abs=1e-12), (mode, k)
+        assert r['success'] == g['success'] and r['npix'] == g['npix']
+
+
+def test_fit_planted_mistakes_are_caught(fits_golden, monkeypatch):
+    """A finite-difference step other than the reference's 1e-5, or the
+    reduced chi2 taken over npix instead of npix - nvar, misses the
+    reference's numbers at the tolerance used above."""
+    from oracle import fit
+    z, meta = fits_golden
+    data = z['crop00_data']
+    c = ref.meshgrid(data.shape)
+    start = dict(zip(fit.VARS6, z['crop00_start']))
+    g = meta['fits']['crop00_default']
+    kw = dict(instrument=INSTRUMENT, noise=float(z['crop00_sigma']))
+    r = fit.fit(data, c, start, settings={'diff_step': 1e-3}, **kw)
+    assert max(abs(r[v] - g[v]) / max(abs(g[v]), 1e-3) for v in fit.VARS6) > 1e-9
+    good = fit.statistics
+    monkeypatch.setattr(fit, 'statistics',
+                        lambda jac, cost, npix, nvar: good(jac, cost, npix, 0))
+    r = fit.fit(data, c, start, **kw)
+    assert r['x_p'] == pytest.approx(g['x_p'], rel=1e-9)
+    assert r['redchi'] != pytest.approx(g['redchi'], rel=1e-9)

@@ -82,8 +82,9 @@ class Frame(Hologram):
                                      flatten=False)
 
     @property
-    def shape(self):
-        return None if self._data is None else tuple(self._data.shape)
+    def shape(self) -> tuple[int, int]:
+        """(ny, nx), or (0, 0) without data (analysis/Frame.py:81-86)."""
+        return (0, 0) if self._data is None else tuple(self._data.shape)
 
     @property
     def on_device(self) -> bool:
@@ -104,7 +105,11 @@ class Frame(Hologram):
     @bboxes.setter
     def bboxes(self, bboxes) -> None:
         """Square crops ((x0, y0), w, h) with a sphere at their centre
-        (analysis/Frame.py:103-115)."""
+        (analysis/Frame.py:103-115); a single box is taken as a list of
+        one."""
+        if (isinstance(bboxes, tuple) and len(bboxes) == 3
+                and isinstance(bboxes[0], tuple)):
+            bboxes = [bboxes]
         self._features = []
         self._bboxes = list(bboxes)
         for (x0, y0), w, h in self._bboxes:
@@ -131,6 +136,12 @@ class Frame(Hologram):
 
     # ------------------------------------------------------------ pipeline
     def detect(self) -> int:
+        """Without data: no features, returns 0 (analysis/Frame.py:133-145).
+        Detection itself (the Localizer) is not part of this library."""
+        if self._data is None:
+            self._features = []
+            self._bboxes = []
+            return 0
         raise NotImplementedError('feature detection (Localizer) is not part '
                                   'of this library: set bboxes')
 

@@ -39,3 +39,10 @@ def superposed_golden():
 @pytest.fixture(scope='session')
 def frame_golden():
     return np.load(os.path.join(GOLDEN, 'frame.npz'))
+
+
+@pytest.fixture(scope='session')
+def crop_golden():
+    """The reference's test hologram tests/data/crop.png and the oracle's
+    fits of it (tests/golden/make_golden_crop.py)."""
+    return np.load(os.path.join(GOLDEN, 'crop.npz'))

@@ -1,0 +1,408 @@
+"""The reference's own unit tests of the host-side classes, case by case:
+tests/test_hologram.py, test_mask.py, test_instrument.py,
+test_particle.py, test_sphere.py, test_lorenzmie.py, test_optimizer.py,
+test_feature.py and test_frame.py.  Each test names the reference cases
+it covers.  Cases that compute on the GPU (Sphere.ab, field, hologram,
+optimize, estimate, predicted, residuals) are in tests/test_gpu_contracts.py
+and tests/test_gpu_crop.py; nothing here touches the GPU."""
+import numpy as np
+import pandas as pd
+import pytest
+
+from pylorenzmie_amd.analysis import Feature, Frame, Hologram, Mask, Optimizer
+from pylorenzmie_amd.lib import LMObject
+from pylorenzmie_amd.theory import Instrument, LorenzMie, Particle, Sphere
+
+
+# ------------------------------------------------------------ Hologram
+@pytest.fixture
+def holo():
+    data = np.random.default_rng(0).random((50, 60))
+    return data, Hologram(data)
+
+
+def test_hologram_construction(holo):
+    """test_shape, test_default_corner, test_custom_corner."""
+    data, h = holo
+    assert h.shape == (50, 60)
+    assert h.corner == (0., 0.)
+    h2 = Hologram(data, corner=(10., 20.))
+    assert h2.coordinates[0, 0, 0] == pytest.approx(10.)
+    assert h2.coordinates[1, 0, 0] == pytest.approx(20.)
+
+
+def test_hologram_coordinates_and_flat_views(holo):
+    """test_coordinates_shape, _origin, test_flat_data_shape, _is_view,
+    test_flat_coordinates_shape, _is_view."""
+    _, h = holo
+    assert h.coordinates.shape == (2, 50, 60)
+    assert h.coordinates[0, 0, 0] == 0. and h.coordinates[1, 0, 0] == 0.
+    assert h.flat_data.shape == (3000,)
+    assert np.shares_memory(h.flat_data, h.data)
+    assert h.flat_coordinates.shape == (2, 3000)
+    assert np.shares_memory(h.flat_coordinates, h.coordinates)
+
+
+def test_hologram_getitem(holo):
+    """test_getitem_shape, _corner, _coordinates_consistent,
+    _corner_accumulates, _none_start."""
+    data, h = holo
+    crop = h[10:30, 15:45]
+    assert crop.shape == (20, 30)
+    assert crop.corner == (15., 10.)
+    np.testing.assert_array_equal(crop.flat_coordinates,
+                                  h.coordinates[:, 10:30, 15:45].reshape(2, -1))
+    assert Hologram(data, corner=(5., 3.))[10:30, 15:45].corner == (20., 13.)
+    crop = h[:30, :45]
+    assert crop.corner == (0., 0.) and crop.shape == (30, 45)
+
+
+def test_hologram_equality(holo):
+    """test_equality, test_inequality_data, test_inequality_corner."""
+    data, h = holo
+    assert h == Hologram(data.copy())
+    assert h != Hologram(data + 1.)
+    assert h != Hologram(data, corner=(1., 0.))
+
+
+# ------------------------------------------------------------ Mask
+@pytest.fixture
+def mask():
+    np.random.seed(1)
+    return Mask(shape=(128, 128))
+
+
+@pytest.mark.parametrize('value', [0.2, 1.])
+def test_mask_fraction(mask, value):
+    """test_fraction, test_fraction_unity."""
+    mask.fraction = value
+    assert mask.fraction == value
+    assert round(abs(value - mask().sum() / mask().size), 1) == 0
+
+
+def test_mask_exclude_shape_update(mask):
+    """test_exclude, test_shape_sets_mask, test_update_regenerates_mask,
+    test_exclude_suppresses_pixels."""
+    exclude = np.zeros((128, 128), bool)
+    mask.exclude = exclude
+    assert exclude.size == mask().size
+    assert mask().shape == (128, 128)
+    m1 = mask().copy()
+    mask.update()
+    assert not np.array_equal(m1, mask())
+    mask.fraction = 1.
+    mask.exclude = np.ones((128, 128), bool)
+    assert mask().sum() == 0
+
+
+def test_mask_properties_and_json(mask):
+    """test_properties, test_json_roundtrip."""
+    assert mask.properties['fraction'] == mask.fraction
+    mask.fraction = 0.3
+    s = mask.to_json()
+    mask.fraction = 0.1
+    mask.from_json(s)
+    assert mask.fraction == pytest.approx(0.3)
+
+
+def test_mask_apply(mask):
+    """test_apply_returns_tuple, _data_shape, _coords_shape, _sets_shape,
+    _selects_correct_pixels, _respects_exclude."""
+    rng = np.random.default_rng(2)
+    h = Hologram(rng.random((128, 128)))
+    result = mask.apply(h)
+    assert isinstance(result, tuple) and len(result) == 2
+    data, coords = result
+    assert data.ndim == 1 and data.shape[0] == coords.shape[1]
+    assert coords.shape[0] == 2
+    m = mask()
+    np.testing.assert_array_equal(data, h.data[m])
+    np.testing.assert_array_equal(coords, h.coordinates[:, m])
+    mask.apply(Hologram(rng.random((64, 80))))
+    assert mask.shape == (64, 80)
+    exclude = np.zeros((128, 128), bool)
+    exclude[0, :] = True
+    mask.fraction = 1.
+    mask.exclude = exclude
+    _, coords = mask.apply(Hologram(rng.random((128, 128))))
+    assert not np.any(coords[1] == 0.)
+
+
+def test_mask_exclude_before_shape_change(mask):
+    """test_exclude_before_shape_change_does_not_raise."""
+    exclude = np.zeros((129, 128), bool)
+    exclude[0, :] = True
+    mask.exclude = exclude
+    mask.fraction = 1.
+    _, coords = mask.apply(Hologram(np.random.default_rng(3).random((129, 128))))
+    assert not np.any(coords[1] == 0.)
+
+
+# ------------------------------------------------------------ Instrument
+def test_instrument_defaults_and_setters():
+    """test_default_* and test_set_*."""
+    inst = Instrument()
+    assert inst.wavelength == pytest.approx(0.447)
+    assert inst.magnification == pytest.approx(0.048)
+    assert inst.numerical_aperture == pytest.approx(1.45)
+    assert inst.noise == pytest.approx(0.05)
+    assert inst.n_m == pytest.approx(1.340)
+    for name, value in (('wavelength', 0.532), ('magnification', 0.120),
+                        ('numerical_aperture', 1.20), ('noise', 0.1),
+                        ('n_m', 1.34)):
+        setattr(inst, name, value)
+        assert getattr(inst, name) == pytest.approx(value)
+
+
+def test_instrument_properties_serialization_repr():
+    """test_properties_keys, _roundtrip, test_json_roundtrip,
+    test_pandas_roundtrip, test_repr."""
+    inst = Instrument()
+    for key in ('wavelength', 'magnification', 'numerical_aperture', 'noise',
+                'n_m'):
+        assert key in inst.properties
+    inst.n_m = 1.339
+    b = Instrument()
+    b.properties = inst.properties
+    assert b.n_m == pytest.approx(1.339)
+    inst.n_m = 1.341
+    b = Instrument()
+    b.from_json(inst.to_json())
+    assert b.n_m == pytest.approx(1.341)
+    b = Instrument()
+    b.from_pandas(inst.to_pandas())
+    assert b.n_m == pytest.approx(1.341)
+    assert isinstance(repr(inst), str)
+
+
+def test_instrument_wavenumber():
+    """test_wavenumber_default, _in_vacuum, _unscaled, _vacuum_unscaled."""
+    inst = Instrument()
+    lam, nm, mag = inst.wavelength, inst.n_m, inst.magnification
+    assert inst.wavenumber() == pytest.approx(2 * np.pi / lam * nm * mag)
+    assert inst.wavenumber(in_medium=False) == pytest.approx(2 * np.pi / lam * mag)
+    assert inst.wavenumber(scaled=False) == pytest.approx(2 * np.pi / lam * nm)
+    assert inst.wavenumber(in_medium=False, scaled=False) == \
+        pytest.approx(2 * np.pi / lam)
+
+
+# ------------------------------------------------------------ Particle
+def test_particle_positions():
+    """test_default_values, test_r_p_setter/_getter, test_r_0_setter/_getter."""
+    p = Particle()
+    assert (p.x_p, p.y_p, p.z_p, p.x_0, p.y_0, p.z_0) == (0., 0., 100., 0., 0., 0.)
+    p.r_p = [10., 20., 30.]
+    assert (p.x_p, p.y_p, p.z_p) == (10., 20., 30.)
+    p.x_p, p.y_p, p.z_p = 1., 2., 3.
+    np.testing.assert_array_equal(p.r_p, [1., 2., 3.])
+    p.r_0 = [1., 2., 3.]
+    assert (p.x_0, p.y_0, p.z_0) == (1., 2., 3.)
+    p.x_0, p.y_0, p.z_0 = 4., 5., 6.
+    np.testing.assert_array_equal(p.r_0, [4., 5., 6.])
+
+
+def test_particle_properties_and_serialization():
+    """test_properties_getter/_setter, test_to_json_roundtrip,
+    test_to_pandas_roundtrip."""
+    p = Particle()
+    p.x_p, p.y_p, p.z_p = 5., 6., 7.
+    assert p.properties == {'x_p': 5., 'y_p': 6., 'z_p': 7.}
+    p.properties = {'x_p': 9., 'y_p': 8., 'z_p': 7.}
+    assert (p.x_p, p.y_p, p.z_p) == (9., 8., 7.)
+    p.r_p = [1., 2., 3.]
+    b = Particle()
+    b.from_json(p.to_json())
+    assert (b.x_p, b.y_p, b.z_p) == (1., 2., 3.)
+    b = Particle()
+    b.from_pandas(p.to_pandas())
+    assert (b.x_p, b.y_p, b.z_p) == (1., 2., 3.)
+
+
+def test_particle_ab_and_sequence():
+    """test_ab_shape, test_ab_values, test_ab_accepts_parameters, test_len,
+    test_iter, test_iter_reentrant, test_getitem, _out_of_range, test_repr."""
+    p = Particle()
+    ab = p.ab()
+    assert ab.shape == (1, 2) and ab.dtype == complex
+    np.testing.assert_array_equal(ab, np.ones((1, 2), complex))
+    assert p.ab(n_m=1.33 + 0.01j, wavelength=0.447).shape == (1, 2)
+    assert len(p) == 1
+    assert list(p) == [p] and list(p)[0] is p
+    assert list(p) == list(p)
+    assert p[0] is p
+    with pytest.raises(IndexError):
+        p[1]
+    assert isinstance(repr(p), str)
+
+
+# ------------------------------------------------------------ Sphere (host)
+def test_sphere_host_cases():
+    """test_default_values, test_d_p_getter/_setter, test_repr,
+    test_properties_keys/_roundtrip, test_wiscombe_yang_*,
+    test_neves_pisignano_*."""
+    s = Sphere()
+    assert (s.a_p, s.n_p, s.k_p) == (1., 1.5, 0.)
+    s = Sphere(a_p=1., n_p=1.4, k_p=0.)
+    assert s.d_p == pytest.approx(2. * s.a_p)
+    s.d_p = 3.
+    assert s.a_p == pytest.approx(1.5)
+    assert isinstance(repr(s), str)
+    props = s.properties
+    for key in ('x_p', 'y_p', 'z_p', 'a_p', 'n_p', 'k_p'):
+        assert key in props
+    props['a_p'] += 0.25
+    s.properties = props
+    assert s.a_p == props['a_p']
+    assert Sphere.wiscombe_yang(1., 10. + 0.j) == 10
+    assert Sphere.wiscombe_yang(10., 10. + 0.j) == 100
+    assert Sphere.wiscombe_yang(4210., 10. + 0.j) == 42100
+    assert isinstance(Sphere.neves_pisignano(10.), int)
+    assert Sphere.neves_pisignano(10.) == 11
+
+
+# ------------------------------------------------------------ LorenzMie (host)
+def test_lorenzmie_host_cases():
+    """test_repr, test_meshgrid, test_coordinates_none, _point, _1d, _2d,
+    _3dlist, test_properties, test_serialize."""
+    m = LorenzMie()
+    assert isinstance(repr(m), str)
+    xy = m.meshgrid((32, 24), flatten=False)
+    assert xy.shape == (2, 32, 24)
+    assert m.meshgrid((32, 24), flatten=True).shape == (2, 32 * 24)
+    m.coordinates = None
+    assert isinstance(m.coordinates, np.ndarray)
+    point = np.array([1, 2, 3]).reshape((3, 1))
+    m.coordinates = point
+    assert np.allclose(m.coordinates, point)
+    m.coordinates = np.arange(256)
+    assert m.coordinates.shape[0] == 3
+    c = m.meshgrid((256, 256))
+    m.coordinates = c
+    assert m.coordinates.shape[0] == 3 and np.allclose(m.coordinates[0:2], c)
+    c = [[1, 1, 1, 1], [2, 2, 2, 2], [3, 3, 3, 3]]
+    m.coordinates = c
+    assert np.allclose(m.coordinates, np.array(c))
+    m.properties = dict(x_p=-42)
+    assert m.particle.x_p == -42
+    m.particle.n_p = 1.5
+    s = m.to_json()
+    m.particle.n_p = 1.4
+    m.from_json(s)
+    assert m.particle.n_p == 1.5
+
+
+# ------------------------------------------------------------ Optimizer
+@pytest.fixture
+def optimizer():
+    shape = (390, 390)
+    model = LorenzMie(coordinates=LMObject.meshgrid(shape))
+    model.instrument.wavelength = 0.447
+    model.instrument.magnification = 0.048
+    model.instrument.n_m = 1.34
+    model.particle.r_p = [shape[0] // 2, shape[1] // 2, 330]
+    model.particle.a_p = 1.1
+    model.particle.n_p = 1.4
+    return Optimizer(model=model)
+
+
+def test_optimizer_configuration_cases(optimizer):
+    """test_result_none, test_mask_default_none, test_mask_set,
+    test_metadata, test_properties, test_default_fixed_includes_noise,
+    _numerical_aperture, test_fixed_list_is_copied,
+    test_robust_false_uses_lm, test_robust_true_uses_trf,
+    test_report_before_optimize_raises."""
+    opt = optimizer
+    assert opt.result is None
+    assert opt.mask is None
+    m = Mask(fraction=0.1)
+    assert Optimizer(model=opt.model, mask=m).mask is m
+    assert isinstance(opt.metadata, pd.Series)
+    props = opt.properties
+    opt.properties = props
+    assert 'settings' in props
+    assert 'noise' in opt.fixed and 'numerical_aperture' in opt.fixed
+    fixed = ['wavelength']
+    o2 = Optimizer(model=opt.model, fixed=fixed)
+    fixed.append('n_m')
+    assert 'n_m' not in o2.fixed
+    opt.robust = False
+    assert not opt.robust and opt.settings['method'] == 'lm'
+    opt.robust = True
+    assert opt.robust and opt.settings['method'] == 'trf'
+    with pytest.raises(RuntimeError):
+        opt.report()
+
+
+def test_optimizer_variables_and_fixed(optimizer):
+    """test_variables_setter_updates_fixed,
+    test_fixed_setter_updates_variables."""
+    opt = optimizer
+    variables = ['x_p', 'y_p', 'z_p', 'a_p']
+    opt.variables = variables
+    for v in variables:
+        assert v in opt.variables and v not in opt.fixed
+    for f in opt.fixed:
+        assert f not in variables
+    fixed = list(opt.model.properties)[:2]
+    opt.fixed = fixed
+    for f in fixed:
+        assert f in opt.fixed and f not in opt.variables
+
+
+# ------------------------------------------------------------ Feature (host)
+@pytest.fixture
+def feature():
+    data = np.random.default_rng(4).random((64, 80)) + 0.5
+    model = LorenzMie()
+    model.instrument.wavelength = 0.447
+    model.instrument.magnification = 0.048
+    model.instrument.n_m = 1.34
+    h, w = data.shape
+    model.particle.r_p = [w / 2., h / 2., 330.]
+    model.particle.a_p = 1.1
+    model.particle.n_p = 1.4
+    f = Feature(Hologram(data), model=model)
+    f.mask.fraction = 0.1
+    return f
+
+
+def test_feature_host_cases(feature):
+    """test_exclude_shared, test_is_hologram, test_model,
+    test_getsetparticle."""
+    f = feature
+    assert f.estimator.exclude is f.optimizer.mask.exclude
+    assert isinstance(f, Hologram)
+    model = LorenzMie()
+    f.model = model
+    assert f.model is model
+    p = f.particle
+    p.z_p = 200.
+    f.particle = p
+    assert f.particle.z_p == 200.
+
+
+# ------------------------------------------------------------ Frame (host)
+def test_frame_host_cases(frame_golden):
+    """test_is_hologram, test_data, test_shape_from_data,
+    test_shape_no_data, test_bboxes, test_bboxes_propagate_instrument,
+    test_detect_no_data, test_getitem_returns_feature, test_getitem_corner
+    (on a crop of the reference's tests/data/image0010.png)."""
+    data = frame_golden['image'].astype(float) / 100.
+    frame = Frame()
+    assert isinstance(frame, Hologram)
+    assert frame.shape == (0, 0)
+    frame.data = data
+    assert data.size == frame.coordinates.size // 2
+    assert frame.shape == data.shape
+    frame.bboxes = ((100, 200), 30, 40)
+    assert len(frame.features) == 1
+    frame = Frame()
+    frame.instrument.wavelength = 0.447
+    frame.data = data
+    frame.bboxes = ((100, 200), 100, 100)
+    assert frame.features[0].model.instrument.wavelength == pytest.approx(0.447)
+    crop = frame[50:150, 50:150]
+    assert isinstance(crop, Feature)
+    assert crop.corner == (50., 50.)
+    frame.data = None
+    assert frame.detect() == 0

@@ -1,0 +1,102 @@
+"""The reference's own unit tests of the classes that compute on the GPU,
+case by case: tests/test_sphere.py (ab and mie_coefficients),
+test_particle.py, test_lorenzmie.py (field and hologram with every flag)
+and test_feature.py (estimate, predicted, residuals on the reference's
+test hologram tests/data/crop.png, carried as tests/golden/crop.npz).
+The values behind these shapes are checked against the oracle and the
+reference's golden vectors in tests/test_gpu_forward.py."""
+import numpy as np
+import pandas as pd
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def api():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no GPU')
+    from pylorenzmie_amd import analysis, theory
+    return theory, analysis
+
+
+def test_sphere_ab_cases(api):
+    """test_ab_shape, test_ab_dtype, test_ab_zeroth_order,
+    test_mie_coefficients_shape, _dtype, _zeroth_order."""
+    theory, _ = api
+    s = theory.Sphere(a_p=1., n_p=1.4, k_p=0.)
+    for ab in (s.ab(1.34, 0.447),
+               theory.Sphere.mie_coefficients(s.a_p, s.n_p, s.k_p, 1.34, 0.447)):
+        assert ab.ndim == 2 and ab.shape[1] == 2
+        assert ab.dtype == np.complex128
+        np.testing.assert_array_equal(ab[0, :], 0j)
+    np.testing.assert_array_equal(s.ab(1.34, 0.447),
+                                  theory.Sphere.mie_coefficients(1., 1.4, 0., 1.34, 0.447))
+
+
+def test_lorenzmie_field_cases(api):
+    """test_field_nocoordinates, test_field, test_field_bohren,
+    test_field_cartesian, test_field_both, test_hologram."""
+    theory, _ = api
+    m = theory.LorenzMie()
+    m.coordinates = None
+    assert isinstance(m.field(), np.ndarray)
+    p = m.particle
+    p.a_p, p.n_p = 1., 1.4
+    p.r_p = [64, 64, 100]
+    c = m.meshgrid([128, 128])
+    m.coordinates = c
+    fields = {}
+    for bohren in (False, True):
+        for cartesian in (False, True):
+            f = m.field(bohren=bohren, cartesian=cartesian)
+            assert f.shape[1] == c.shape[1]
+            fields[bohren, cartesian] = f.copy()
+    # every call refills the model's one field buffer, as the reference's
+    # field() returns self._field (theory/LorenzMie.py:178-181)
+    assert m.field() is m.field()
+    # the flags change the field (each is checked against the reference's
+    # golden fields elsewhere)
+    assert not np.allclose(fields[False, False], fields[True, True])
+    holo = m.hologram()
+    assert holo.size == c.shape[1]
+
+
+@pytest.fixture
+def feature(api, crop_golden):
+    """tests/test_feature.py:16-32 of the reference."""
+    theory, analysis = api
+    data = crop_golden['image'].astype(float) / 100.
+    model = theory.LorenzMie()
+    model.instrument.wavelength = 0.447
+    model.instrument.magnification = 0.048
+    model.instrument.n_m = 1.34
+    h, w = data.shape
+    model.particle.r_p = [w / 2., h / 2., 330.]
+    model.particle.a_p = 1.1
+    model.particle.n_p = 1.4
+    f = analysis.Feature(analysis.Hologram(data), model=model)
+    f.mask.fraction = 0.1
+    return f
+
+
+def test_feature_estimate_cases(feature):
+    """test_estimate_returns_series, test_estimate_sets_z_p,
+    test_estimate_sets_a_p, test_estimate_uses_coordinates_for_position."""
+    result = feature.estimate()
+    assert isinstance(result, pd.Series)
+    assert feature.particle.z_p > 0
+    assert feature.particle.a_p > 0
+    assert feature.particle.x_p == pytest.approx(
+        float(feature.coordinates[0].mean()), abs=1e-5)
+    assert feature.particle.y_p == pytest.approx(
+        float(feature.coordinates[1].mean()), abs=1e-5)
+
+
+def test_feature_predicted_and_residuals(feature):
+    """test_predicted_shape, test_residuals_shape."""
+    assert feature.predicted().shape == feature.shape
+    r = feature.residuals()
+    assert r.shape == feature.shape
+    np.testing.assert_allclose(r, feature.predicted() - feature.data)

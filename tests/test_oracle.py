@@ -281,3 +281,38 @@ def test_fit_planted_mistakes_are_caught(fits_golden, monkeypatch):
     r = fit.fit(data, c, start, **kw)
     assert r['x_p'] == pytest.approx(g['x_p'], rel=1e-9)
     assert r['redchi'] != pytest.approx(g['redchi'], rel=1e-9)
+
+
+def _gauss_newton_step(data, c, values, noise=0.05):
+    """One Gauss-Newton step of the 5-variable fit from ``values``, with a
+    central-difference Jacobian of the oracle's hologram."""
+    names = ['x_p', 'y_p', 'z_p', 'a_p', 'n_p']
+    base = dict(zip(names, values), k_p=0.)
+    r = (ref.hologram(c, base, **INSTRUMENT) - data) / noise
+    steps = [1e-3, 1e-3, 1e-3, 1e-5, 1e-5]
+    J = np.empty((r.size, 5))
+    for j, (v, h) in enumerate(zip(names, steps)):
+        hp = ref.hologram(c, dict(base, **{v: base[v] + h}), **INSTRUMENT)
+        hm = ref.hologram(c, dict(base, **{v: base[v] - h}), **INSTRUMENT)
+        J[:, j] = (hp - hm) / (2. * h * noise)
+    return -np.linalg.solve(J.T @ J, J.T @ r)
+
+
+def test_crop_fixture_is_the_minimum(crop_golden):
+    """The stored tight fit of the reference's test hologram
+    (tests/data/crop.png) is the minimum of the oracle's chi2: a
+    Gauss-Newton step from it moves no parameter by 2% of its uncertainty
+    (MINPACK stops where its forward-difference Jacobian, step 1e-5 |p|,
+    has a zero gradient, which on this real hologram, reduced chi2 4.2,
+    lies 0.2-0.9% of an uncertainty from the exact minimum), while from the
+    default-tolerance fit it still moves one by more than 0.1 of its
+    uncertainty."""
+    g = crop_golden
+    data = g['image'].astype(float)
+    data = (data / data.mean()).ravel()
+    c = ref.meshgrid(g['image'].shape)
+    unc = g['tight_uncertainties']
+    step = _gauss_newton_step(data, c, g['tight_values'])
+    assert np.all(np.abs(step) < 2e-2 * unc), step / unc
+    step = _gauss_newton_step(data, c, g['default_values'])
+    assert np.abs(step / unc).max() > 0.1, step / unc

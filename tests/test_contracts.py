@@ -1,10 +1,12 @@
 """The reference's own unit tests of the host-side classes, case by case:
 tests/test_hologram.py, test_mask.py, test_instrument.py,
 test_particle.py, test_sphere.py, test_lorenzmie.py, test_optimizer.py,
-test_feature.py and test_frame.py.  Each test names the reference cases
-it covers.  Cases that compute on the GPU (Sphere.ab, field, hologram,
-optimize, estimate, predicted, residuals) are in tests/test_gpu_contracts.py
-and tests/test_gpu_crop.py; nothing here touches the GPU."""
+test_feature.py, test_frame.py, test_cluster.py, test_dimer.py,
+test_deestimator.py and test_radialestimator.py.  Each test names the
+reference cases it covers.  Cases that compute on the GPU (Sphere.ab,
+field, hologram, optimize, estimate, predicted, residuals) are in
+tests/test_gpu_contracts.py and tests/test_gpu_crop.py; nothing here
+touches the GPU."""
 import numpy as np
 import pandas as pd
 import pytest
@@ -406,3 +408,72 @@ def test_frame_host_cases(frame_golden):
     assert crop.corner == (50., 50.)
     frame.data = None
     assert frame.detect() == 0
+
+
+# ------------------------------------------------------------ Cluster, Dimer
+def test_cluster_cases():
+    """tests/test_cluster.py: test_len, test_getitem, test_iteration,
+    test_update_positions, test_setattr_particles."""
+    from pylorenzmie_amd.theory import Cluster
+    p1 = Particle(x_p=10., y_p=20., z_p=30.)
+    p2 = Particle(x_p=40., y_p=50., z_p=60.)
+    c = Cluster(particles=[p1, p2])
+    assert len(c) == 2
+    assert c[0] == p1 and c[1] == p2
+    with pytest.raises(IndexError):
+        c[2]
+    assert list(c) == [p1, p2]
+    c.x_p, c.y_p, c.z_p = 100., 200., 300.
+    for p in c:
+        assert tuple(p.r_0) == (100., 200., 300.)
+    new = [Particle(x_p=1., y_p=2., z_p=3.), Particle(x_p=4., y_p=5., z_p=6.)]
+    c.particles = new
+    assert len(c) == 2 and c[0] == new[0] and c[1] == new[1]
+
+
+def test_dimer_cases():
+    """tests/test_dimer.py: test_initialization, test_update_positions,
+    test_setattr_particles, test_invalid_particle_count."""
+    from pylorenzmie_amd.theory import Dimer
+    d = Dimer()
+    assert len(d) == 2
+    for p in d:
+        assert isinstance(p, Sphere)
+        assert (p.a_p, p.n_p, p.k_p) == (d.a_p, d.n_p, d.k_p)
+    a_p, theta, phi, mag = 1.0, np.pi / 3, np.pi / 4, 0.05
+    d.a_p, d.theta, d.phi, d.magnification = a_p, theta, phi, mag
+    r_p = (a_p / mag) * np.array([np.cos(theta) * np.cos(phi),
+                                  np.cos(theta) * np.sin(phi), np.sin(theta)])
+    assert np.allclose(d.particles[0].r_p, r_p)
+    assert np.allclose(d.particles[1].r_p, -r_p)
+    d.a_p, d.n_p, d.k_p = 1.5, 1.55, 0.1
+    for p in d:
+        assert (p.a_p, p.n_p, p.k_p) == (1.5, 1.55, 0.1)
+    d.particles = [Sphere(a_p=0.75, n_p=1.45, k_p=0.)]
+    d.update_positions()
+    assert len(d) == 1
+
+
+# ------------------------------------------------------------ estimators (host)
+@pytest.mark.parametrize('name', ['DEEstimator', 'RadialEstimator'])
+def test_estimator_host_cases(name):
+    """tests/test_deestimator.py and test_radialestimator.py:
+    test_is_base_estimator, test_default_bounds_keys,
+    test_default_bounds_valid_ranges, test_default_bounds_match_de,
+    test_properties_keys, test_exclude_none_by_default."""
+    from pylorenzmie_amd import analysis
+    from pylorenzmie_amd.analysis.DEEstimator import DEFAULT_BOUNDS
+    model = LorenzMie(coordinates=LMObject.meshgrid((101, 101)))
+    est = getattr(analysis, name)(model=model, seed=0)
+    assert isinstance(est, analysis.BaseEstimator)
+    for key in ('z_p', 'a_p', 'n_p'):
+        assert key in DEFAULT_BOUNDS and key in est.bounds
+    for key, (lo, hi) in DEFAULT_BOUNDS.items():
+        assert lo < hi, key
+    assert est.bounds == DEFAULT_BOUNDS
+    keys = ('popsize', 'bounds', 'settings')
+    if name == 'DEEstimator':
+        keys += ('fraction',)
+        assert est.exclude is None
+    for key in keys:
+        assert key in est.properties

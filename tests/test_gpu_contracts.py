@@ -100,3 +100,62 @@ def test_feature_predicted_and_residuals(feature):
     r = feature.residuals()
     assert r.shape == feature.shape
     np.testing.assert_allclose(r, feature.predicted() - feature.data)
+
+
+@pytest.fixture
+def estimator_case(api):
+    """tests/test_deestimator.py:13-24 and test_radialestimator.py:13-24."""
+    theory, analysis = api
+    shape = (101, 101)
+    model = theory.LorenzMie()
+    model.coordinates = theory.LorenzMie.meshgrid(shape)
+    model.particle.r_p = [50., 50., 200.]
+    model.particle.a_p = 0.75
+    model.particle.n_p = 1.45
+    hologram = analysis.Hologram(model.hologram().reshape(shape))
+    return model, hologram
+
+
+def _quick(est):
+    est.settings['maxiter'] = 1
+    est.settings['workers'] = 1
+    return est
+
+
+@pytest.mark.parametrize('name', ['DEEstimator', 'RadialEstimator'])
+def test_estimator_cases(api, estimator_case, name):
+    """test_returns_series, test_result_has_particle_keys,
+    test_xy_pinned_to_coordinate_means, test_model_coordinates_restored,
+    test_model_coordinates_restored_on_exception, test_custom_bounds,
+    test_corner_aware (radial), test_exclude_filters_pixels (DE)."""
+    _, analysis = api
+    cls = getattr(analysis, name)
+    model, hologram = estimator_case
+    est = _quick(cls(model=model, seed=0))
+    before = model.coordinates.copy()
+    result = est.estimate(hologram)
+    assert isinstance(result, pd.Series)
+    for key in ('x_p', 'y_p', 'z_p', 'a_p', 'n_p'):
+        assert key in result
+    assert result['x_p'] == pytest.approx(float(hologram.coordinates[0].mean()),
+                                          abs=5e-4)
+    assert result['y_p'] == pytest.approx(float(hologram.coordinates[1].mean()),
+                                          abs=5e-4)
+    np.testing.assert_array_equal(model.coordinates, before)
+    bad = cls(model=model, bounds={'z_p': (500., 100.)}, seed=0)
+    try:
+        bad.estimate(hologram)
+    except Exception:
+        pass
+    np.testing.assert_array_equal(model.coordinates, before)
+    custom = _quick(cls(model=model, bounds={'z_p': (150., 250.)}, seed=0))
+    assert 150. <= custom.estimate(hologram)['z_p'] <= 250.
+    h = analysis.Hologram(hologram.data, corner=(10., 20.))
+    result = est.estimate(h)
+    assert result['x_p'] == pytest.approx(float(h.coordinates[0].mean()), abs=5e-4)
+    assert result['y_p'] == pytest.approx(float(h.coordinates[1].mean()), abs=5e-4)
+    if name == 'DEEstimator':
+        exclude = np.zeros(hologram.shape, dtype=bool)
+        exclude[:, hologram.shape[1] // 2:] = True
+        est = _quick(cls(model=model, seed=0, exclude=exclude))
+        assert isinstance(est.estimate(hologram), pd.Series)

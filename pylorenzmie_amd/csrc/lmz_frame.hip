@@ -413,8 +413,9 @@ __global__ __launch_bounds__(64) void k_medfilt_radix(
     const int span = 7 + size;
     const int bx = blockIdx.x * 8, by = blockIdx.y * 8;
     const int lane = threadIdx.x;
-    int *rowoff = reinterpret_cast<int *>(dyn);        // unstaged tables
-    int *colidx = rowoff + span;
+    // unstaged tables: 64-bit row offsets (frames may exceed 2^31 pixels)
+    long long *rowoff = reinterpret_cast<long long *>(dyn);
+    int *colidx = reinterpret_cast<int *>(rowoff + span);
     K *tile = reinterpret_cast<K *>(dyn);              // staged keys
     const K kref = P::key(img[(long long)reflect_index(by - lo, ny) * nx +
                               reflect_index(bx - lo, nx)]);
@@ -429,14 +430,14 @@ __global__ __launch_bounds__(64) void k_medfilt_radix(
         }
     } else {
         for (int i = lane; i < span; i += 64) {
-            rowoff[i] = reflect_index(by - lo + i, ny) * nx;
+            rowoff[i] = (long long)reflect_index(by - lo + i, ny) * nx;
             colidx[i] = reflect_index(bx - lo + i, nx);
         }
         __syncthreads();
         for (int idx = lane; idx < span * span; idx += 64) {
             const int i = idx / span, j = idx - i * span;
-            diff |= (unsigned long long)(P::key(img[(long long)rowoff[i] +
-                                                    colidx[j]]) ^ kref);
+            diff |= (unsigned long long)(P::key(img[rowoff[i] + colidx[j]]) ^
+                                         kref);
         }
     }
     for (int off = 32; off > 0; off >>= 1) diff |= __shfl_xor(diff, off);
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(64) void k_medfilt_radix(
             key = window_select<K>(fetch, size, rank, prefix, top, h);
         } else {
             auto fetch = [&](int r, int c) {
-                return P::key(img[(long long)rowoff[ty + r] + colidx[tx + c]]);
+                return P::key(img[rowoff[ty + r] + colidx[tx + c]]);
             };
             key = window_select<K>(fetch, size, rank, prefix, top, h);
         }
@@ -593,8 +594,8 @@ int medfilt_radix(const void *image, int ny, int nx, int size, void *bgv,
                            stream, img, ny, nx, size, lo, stride, bg, out, odt);
     } else {
         hipLaunchKernelGGL((k_medfilt_radix<DT, false>), grid, dim3(64),
-                           2 * span * sizeof(int), stream, img, ny, nx, size, lo,
-                           stride, bg, out, odt);
+                           span * (sizeof(long long) + sizeof(int)), stream, img,
+                           ny, nx, size, lo, stride, bg, out, odt);
     }
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;

@@ -329,6 +329,31 @@ def run_cpu(args, start, data_host):
     return n / dt, n, dt
 
 
+def run_cpu_pool(args, start, data_dev):
+    """The same oracle fits on args.cpu_workers host cores (oracle/cpu_pool.py,
+    one process per core, two features each), started as a child process
+    that never touches the GPU.  The sample goes through a file."""
+    import subprocess
+    import tempfile
+    n = 2 * args.cpu_workers
+    data = data_dev[: args.side * args.side * n].cpu().numpy()
+    data = data.reshape(n, args.side, args.side)
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, 'sample.npz')
+        np.savez(path, data=data, start=start[:n], sigma=np.array(args.sigma))
+        env = dict(os.environ, OMP_NUM_THREADS='1', MKL_NUM_THREADS='1',
+                   OPENBLAS_NUM_THREADS='1')
+        out = subprocess.run([sys.executable, '-m', 'oracle.cpu_pool', path,
+                              str(args.cpu_workers)], capture_output=True,
+                             text=True, env=env, cwd=ROOT, timeout=900,
+                             check=True)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return dict(value=r['value'], unit='fits/s', cores=r['workers'],
+                kind='port', sample=f"{r['fits']} 7-parameter fits of "
+                f"{args.side}^2 crops, {r['workers']} processes, "
+                f"{r['seconds']:.1f} s wall")
+
+
 # ---------------------------------------------------------------------------
 # The other configurations (--config N): one JSON line each with the
 # quantities the measurement plan asks of them.  Seeds are the config index.
@@ -775,6 +800,9 @@ def main():
     ap.add_argument('--fd-sample', type=int, default=8)
     ap.add_argument('--cpu-seconds', type=float, default=15.)
     ap.add_argument('--cpu-max-fits', type=int, default=4)
+    ap.add_argument('--cpu-workers', type=int, default=0,
+                    help='also time the oracle on this many host cores '
+                         '(one process each, 2 features each; 0: skip)')
     ap.add_argument('--kernel-reps', type=int, default=3)
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='also run one device copy of this many MiB (PMC '
@@ -898,6 +926,11 @@ def main():
                           f'FP64 numpy oracle + scipy MINPACK LM (forward '
                           f'differences), {t:.1f} s')
         log(f'bench: CPU oracle {rate:.4f} fits/s')
+    cpu_pool = None
+    if args.cpu_workers > 0:
+        cpu_pool = run_cpu_pool(args, start, data_dev)
+        log(f"bench: CPU oracle on {cpu_pool['cores']} cores "
+            f"{cpu_pool['value']:.4f} fits/s")
 
     line = {
         'metric': '7-param sphere fits/sec at 201² crops, adjoint vs FD-Jacobian, 1 GPU',
@@ -953,6 +986,7 @@ def main():
         'fd_batched': fdb,
         'speedup_vs_fd_batched': (fits_per_s / fdb['value']) if fdb else None,
         'cpu_baseline': cpu,
+        'cpu_baseline_cores': cpu_pool,
     }
     print(json.dumps(line), flush=True)
 

@@ -15,6 +15,9 @@ lorenzmie
 fit
     The reference's fitter semantics (scipy MINPACK LM with the
     reference's default settings, residual scaling and SVD uncertainties).
+cpu_pool
+    ``fit`` over a sample of features on several host processes, the
+    multi-core CPU baseline (started by bench.py as a child process).
 normalize
     Background normalization of a frame (np.median, scipy's median filter,
     a reference image) and the running median of a video.

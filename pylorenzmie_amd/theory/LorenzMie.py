@@ -15,7 +15,6 @@ import numpy as np
 from .. import _lmz
 from ..lib import LMObject
 from .Instrument import Instrument
-from .Particle import Particle
 from .Sphere import Sphere
 
 PARAMS = ('x_p', 'y_p', 'z_p', 'a_p', 'n_p', 'k_p', 'alpha')

@@ -6,7 +6,6 @@ forms evaluated on the host, as the reference does.
 """
 from __future__ import annotations
 
-import ctypes
 import threading
 from dataclasses import dataclass
 

@@ -776,8 +776,55 @@ def run_extra(args):
                     generation_ms=tg * 1e3, candidates_per_s=30 / tg,
                     cpu_baseline={'value': 1. / tc, 'unit': 'candidates/s',
                                   'cores': 1, 'kind': 'port'})
+    elif args.extra == 'batch':
+        # TorchLorenzMieBatch.batch_hologram (theory/torchLorenzMieBatch.py:
+        # 258-283): 256 holograms on one 201 x 201 grid, 1-3 spheres each
+        from pylorenzmie_amd.theory import LorenzMieBatch
+        rng = np.random.default_rng(4)
+        B, side = 256, 201
+        lists = []
+        for _ in range(B):
+            lists.append([Sphere(x_p=rng.uniform(60., 140.), y_p=rng.uniform(60., 140.),
+                                 z_p=rng.uniform(80., 300.), a_p=rng.uniform(0.4, 1.5),
+                                 n_p=rng.uniform(1.38, 1.65))
+                          for _ in range(int(rng.integers(1, 4)))])
+        m = LorenzMieBatch(coordinates=LorenzMie.meshgrid((side, side)),
+                           instrument=Instrument())
+        h = m.batch_hologram(lists)
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            h = m.batch_hologram(lists)
+        tb = (time.perf_counter() - t0) / reps
+        # the same holograms one LorenzMie.hologram() call at a time
+        m1 = LorenzMie(coordinates=LorenzMie.meshgrid((side, side)),
+                       instrument=Instrument())
+        t0 = time.perf_counter()
+        for plist in lists[:32]:
+            m1.particle = plist
+            m1.hologram()
+        ts = (time.perf_counter() - t0) / 32
+        c = ref.meshgrid((side, side))
+        err, tc = 0., 0.
+        for i in range(0, B, 64):
+            dicts = [dict(x_p=p.x_p, y_p=p.y_p, z_p=p.z_p, a_p=p.a_p,
+                          n_p=p.n_p, k_p=p.k_p) for p in lists[i]]
+            t0 = time.perf_counter()
+            want = ref.hologram(c, dicts, **INSTRUMENT)
+            tc += time.perf_counter() - t0
+            err = max(err, float(np.abs(h[i] - want).max()))
+        tc /= len(range(0, B, 64))
+        line.update(workload=f'LorenzMieBatch.batch_hologram: {B} holograms '
+                             f'of 1-3 spheres (a_p 0.4-1.5 um) on one '
+                             f'{side} x {side} grid, one pass',
+                    ms_per_batch=tb * 1e3, holograms_per_s=B / tb,
+                    sequential_ms_per_hologram=ts * 1e3,
+                    speedup_vs_sequential=ts * B / tb,
+                    max_abs_err_vs_oracle=err,
+                    cpu_baseline={'value': 1. / tc, 'unit': 'holograms/s',
+                                  'cores': 1, 'kind': 'port'})
     else:
-        raise SystemExit('--extra must be superposed, frame or de')
+        raise SystemExit('--extra must be superposed, frame, de or batch')
     print(json.dumps(line), flush=True)
 
 
@@ -815,7 +862,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--seed', type=int, default=3)
     ap.add_argument('--extra', default='',
-                    help='superposed | frame | de: measure that path instead '
+                    help='superposed | frame | de | batch: measure that path instead '
                          'of the headline (one JSON line)')
     ap.add_argument('--config', type=int, default=0,
                     help='1-4: measure that configuration instead of the '

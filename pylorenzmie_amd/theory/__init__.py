@@ -12,10 +12,12 @@ from .LorenzMie import LorenzMie, PARAMS
 from .Cluster import Cluster
 from .Dimer import Dimer
 from .Aberrated import Aberrated, AberratedLorenzMie
+from .LorenzMieBatch import LorenzMieBatch
 
 hipLorenzMie = LorenzMie
 LMHologram = LorenzMie
+hipLorenzMieBatch = LorenzMieBatch     # theory/torchLorenzMieBatch.py:218
 
 __all__ = ['Instrument', 'Particle', 'Sphere', 'LayeredSphere', 'Cluster',
            'Dimer', 'Aberrated', 'AberratedLorenzMie', 'LorenzMie', 'hipLorenzMie',
-           'LMHologram', 'PARAMS']
+           'LMHologram', 'LorenzMieBatch', 'hipLorenzMieBatch', 'PARAMS']

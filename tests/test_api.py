@@ -205,3 +205,13 @@ def test_estimators_share_the_base_class():
         est = cls(model=LorenzMie())
         assert isinstance(est, BaseEstimator)
         assert est.bounds == DEFAULT_BOUNDS and est.bounds is not DEFAULT_BOUNDS
+
+
+def test_lorenzmie_batch_without_particles():
+    """theory/torchLorenzMieBatch.py:313-316 of the reference: nothing to
+    scatter gives unit holograms; no device work is needed for that."""
+    from pylorenzmie_amd.theory import LorenzMieBatch, hipLorenzMieBatch
+    assert issubclass(LorenzMieBatch, LorenzMie) and hipLorenzMieBatch is LorenzMieBatch
+    m = LorenzMieBatch(coordinates=LorenzMie.meshgrid((8, 6)))
+    assert m.batch_hologram([]).shape == (0, 48)
+    assert np.array_equal(m.batch_hologram([[], []]), np.ones((2, 48)))

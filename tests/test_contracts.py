@@ -2,7 +2,8 @@
 tests/test_hologram.py, test_mask.py, test_instrument.py,
 test_particle.py, test_sphere.py, test_lorenzmie.py, test_optimizer.py,
 test_feature.py, test_frame.py, test_cluster.py, test_dimer.py,
-test_deestimator.py and test_radialestimator.py.  Each test names the
+test_deestimator.py, test_radialestimator.py and test_radialmask.py.
+Each test names the
 reference cases it covers.  Cases that compute on the GPU (Sphere.ab,
 field, hologram, optimize, estimate, predicted, residuals) are in
 tests/test_gpu_contracts.py and tests/test_gpu_crop.py; nothing here
@@ -477,3 +478,82 @@ def test_estimator_host_cases(name):
         assert est.exclude is None
     for key in keys:
         assert key in est.properties
+
+
+# ------------------------------------------------------------ RadialMask
+def test_radial_mask_cases():
+    """tests/test_radialmask.py: test_output_shape, _dtype,
+    test_non_square_shape, test_center_excluded_at_low_fraction,
+    test_center_selected_at_high_fraction, test_update_regenerates,
+    test_inherits_exclude, test_any_pixels_selected."""
+    from pylorenzmie_amd.analysis import RadialMask
+    np.random.seed(2)
+    shape = (101, 121)
+    h, w = shape
+    m = RadialMask(shape=shape, fraction=0.5)
+    assert m().shape == shape and m().dtype == bool
+    assert RadialMask(shape=(61, 101), fraction=0.3)().shape == (61, 101)
+    m = RadialMask(shape=shape, fraction=0.2)
+    hits = 0
+    for _ in range(20):
+        m.update()
+        hits += int(m()[h // 2, w // 2])
+    assert hits == 0
+    m = RadialMask(shape=shape, fraction=0.8)
+    hits = 0
+    for _ in range(20):
+        m.update()
+        hits += int(m()[h // 2, w // 2])
+    assert hits > 0
+    m = RadialMask(shape=shape, fraction=0.3)
+    m.update()
+    m1 = m().copy()
+    m.update()
+    assert not np.array_equal(m1, m())
+    exclude = np.zeros(shape, dtype=bool)
+    exclude[h // 2, w // 2] = True
+    assert not RadialMask(shape=shape, fraction=0.9, exclude=exclude)()[h // 2, w // 2]
+    assert RadialMask(shape=shape, fraction=0.3)().any()
+
+
+@pytest.mark.parametrize('fraction', [0.15, 0.35, 0.6, 0.85])
+def test_radial_mask_probability_law(fraction):
+    """Each pixel's selection rate over many draws is min(1, hypot(a + b u,
+    a + b v)) with u, v in [-1, 1] (analysis/RadialMask.py:21-33): the
+    centre-to-edge law for either branch of (a, b), which a branch taken
+    for the wrong fraction or unscaled positions would miss."""
+    from pylorenzmie_amd.analysis import RadialMask
+    np.random.seed(3)
+    shape = (9, 13)
+    m = RadialMask(shape=shape, fraction=fraction)
+    rate = np.zeros(shape)
+    n = 4000
+    for _ in range(n):
+        m.update()
+        rate += m()
+    rate /= n
+    f = fraction
+    a, b = (0., 2. * f) if f < 0.5 else (2. * f - 1., 2. * (1. - f))
+    y, x = np.mgrid[:shape[0], :shape[1]]
+    u = 2. * x / (shape[1] - 1.) - 1.
+    v = 2. * y / (shape[0] - 1.) - 1.
+    want = np.minimum(1., np.hypot(a + b * v, a + b * u))
+    assert np.abs(rate - want).max() < 0.035
+    wrong = np.minimum(1., np.hypot(2. * f * v, 2. * f * u)) if f >= 0.5 else \
+        np.minimum(1., np.hypot(2. * f - 1. + 2. * (1. - f) * v,
+                                2. * f - 1. + 2. * (1. - f) * u))
+    assert np.abs(rate - wrong).max() > 0.1
+
+
+def test_optimizer_takes_a_radial_mask():
+    """A RadialMask plugs into the Optimizer as the reference's Mask does
+    (analysis/Optimizer.py:211-215): the pixels it keeps are the fit's."""
+    from pylorenzmie_amd.analysis import RadialMask
+    np.random.seed(4)
+    model = LorenzMie(coordinates=LMObject.meshgrid((40, 50)))
+    opt = Optimizer(model=model, mask=RadialMask(fraction=0.3))
+    h = Hologram(np.random.default_rng(5).random((40, 50)))
+    opt._select(h)
+    keep = opt.mask()
+    assert keep.shape == (40, 50) and opt._count == keep.sum()
+    np.testing.assert_array_equal(opt._data, h.data[keep])

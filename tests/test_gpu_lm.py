@@ -79,6 +79,23 @@ def test_crop_fits_match_reference(api, fits_golden):
     assert n_same >= 8
 
 
+def test_iterations_match_minpack(api, fits_golden):
+    """At the reference's default settings the GPU LM needs no more
+    iterations than the reference's MINPACK run on the same fit, give or
+    take one, wherever that run took few steps: MINPACK's count is
+    (nfev - 1) / 7 for 6 parameters with forward differences
+    (analysis/Optimizer.py:221).  The multimodal crop01 and crop02 are left
+    out (their step counts depend on the path)."""
+    fx, meta = fits_golden
+    names = ['cfg0'] + [f'crop{i:02d}' for i in range(12) if i not in (1, 2)]
+    for name in names:
+        opt, res = fit_one(api, fx[f'{name}_data'], fx[f'{name}_start'],
+                           fx[f'{name}_sigma'])
+        ref_it = (meta['fits'][f'{name}_default']['nfev'] - 1) / 7.
+        print(name, 'gpu', opt._result.nit, 'minpack', ref_it)
+        assert opt._result.nit <= ref_it + 1, name
+
+
 def test_optimizer_contract(api, fits_golden):
     fx, _ = fits_golden
     opt, res = fit_one(api, fx['cfg0_data'], fx['cfg0_start'], 0.05,

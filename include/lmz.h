@@ -64,10 +64,12 @@ extern "C" {
 #define LMZ_CARTESIAN 1    /* project onto x, y, z (theory/LorenzMie.py:331) */
 #define LMZ_BOHREN 2       /* +z along propagation (theory/LorenzMie.py:249) */
 #define LMZ_ACCUMULATE 4   /* field += instead of field = */
-#define LMZ_FP32 8         /* residuals (lmz_hologram with data) in FP32
-                              arithmetic: 1.6x faster, errors up to
-                              1e-4 + 2e-5 I (Cartesian only; not for the
-                              LM) */
+#define LMZ_FP32 8         /* lmz_hologram (hologram, or residuals with
+                              data) in FP32 arithmetic: 1.6x faster,
+                              errors up to 1e-4 + 2e-5 I (Cartesian only;
+                              not for the LM; cupyLorenzMie's
+                              double_precision = False,
+                              theory/cupyLorenzMie.py:46-65) */
 
 /* One particle and the pixels it is evaluated on.  Fields marked (out) are
  * written by lmz_batch_create into the caller's array. */

@@ -153,7 +153,8 @@ __device__ __forceinline__ int next_tile(int32_t *count, int &slot, int &k) {
 
 // Forward pass over tiles of LMZ_TILE pixels, in FP64.
 // MODE 0: field, 1: hologram, 2: residual (+ per-tile r^T r), 3: the
-// residual in FP32 arithmetic (Cartesian; chi2 partials still FP64).
+// residual in FP32 arithmetic (Cartesian; chi2 partials still FP64), or
+// without data the FP32 hologram into out.
 // Without a work list each block takes tile xcd_tile(blockIdx.x); with one
 // (the LM's compacted list of active tiles, length count[0]) the grid is
 // persistent and blocks fetch list entries dynamically (next_tile) until
@@ -198,11 +199,15 @@ __global__ __launch_bounds__(NT) void k_forward(
         float ax, ay, az;
         pixel_coords(fu, j, coords, npix_total, ax, ay, az);
         const long long gi = fu.pix_offset + j;
-        if constexpr (MODE == 3) {   // FP32 residual (Cartesian)
+        if constexpr (MODE == 3) {   // FP32 arithmetic (Cartesian)
             PixGeom gf;
             pixel_geometry(fu, ax, ay, az, bohren, gf);
             const float I = intensity_f(gf, forward_sum_f(gf, cf0, fu.nmax),
                                         fu.alpha);
+            if (data == nullptr) {   // the hologram itself
+                outf[gi] = I;
+                continue;
+            }
             const double r = ((double)I - (double)data[gi]) / F.noise;
             if (outf) outf[gi] = (float)r;
             chi = fma(r, r, chi);
@@ -1038,7 +1043,17 @@ int lmz_hologram(const lmz_batch *b, const double *params, const int32_t *sel,
                  double *chi2_tiles, const uint8_t *active,
                  hipStream_t stream) {
     if (int rc = forward_args(b, params, nmax, coef, coords, sel)) return rc;
-    if (data == nullptr) {
+    if (data == nullptr && (flags & LMZ_FP32)) {
+        LMZ_REQUIRE(out != nullptr, "out is NULL");
+        LMZ_REQUIRE(flags & LMZ_CARTESIAN,
+                    "the FP32 hologram is defined for the Cartesian field");
+        launch_forward_nt<3>(b->sched.fwd_threads, b->ntiles, stream, b->d_feat,
+                             b->d_tile_feature, params, sel, 0, b->coef_buffers,
+                             nmax, coef, coords, b->npix_total, flags,
+                             (const float *)nullptr, out, 0LL, (cf *)nullptr,
+                             (double *)nullptr, active, (const int32_t *)nullptr,
+                             (int32_t *)nullptr);
+    } else if (data == nullptr) {
         LMZ_REQUIRE(out != nullptr, "out is NULL");
         launch_forward_nt<1>(b->sched.fwd_threads, b->ntiles, stream, b->d_feat,
                              b->d_tile_feature, params, sel, 0, b->coef_buffers,

@@ -52,10 +52,14 @@ class LorenzMie(LMObject):
     jac_params: frozenset = frozenset(PARAMS)
 
     def __init__(self, coordinates=None, particle=None, instrument=None,
-                 alpha: float = 1.0) -> None:
+                 alpha: float = 1.0, double_precision: bool = True) -> None:
         self._batch = None
         self._batch_key = None
         self.alpha = float(alpha)
+        # cupyLorenzMie's switch (theory/cupyLorenzMie.py:46-65): False
+        # computes hologram() of one particle in FP32 arithmetic (errors up
+        # to 1e-4 + 2e-5 I, Cartesian only); the field stays FP64
+        self.double_precision = bool(double_precision)
         self.coordinates = coordinates
         self.particle = particle or Sphere()
         self.instrument = instrument or Instrument()
@@ -305,6 +309,8 @@ class LorenzMie(LMObject):
             self._launch_superposed(d, flags, out=d['holo'])
         else:
             self._load_particle(self.particle[0], d)
+            if not self.double_precision and cartesian:
+                flags |= _lmz.FP32
             _lmz.check(lib.lmz_hologram(
                 self._batch.handle, _lmz.ptr(d['params']), None,
                 _lmz.ptr(d['nmax']), _lmz.ptr(d['coef']), _lmz.ptr(d['coords']),

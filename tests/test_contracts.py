@@ -557,3 +557,15 @@ def test_optimizer_takes_a_radial_mask():
     keep = opt.mask()
     assert keep.shape == (40, 50) and opt._count == keep.sum()
     np.testing.assert_array_equal(opt._data, h.data[keep])
+
+
+def test_double_precision_switch():
+    """tests/test_cupylorenzmie.py: test_doubleprecision (the switch of
+    theory/cupyLorenzMie.py:46-65)."""
+    m = LorenzMie()
+    assert m.double_precision
+    m.double_precision = False
+    assert not m.double_precision
+    m.double_precision = True
+    assert m.double_precision
+    assert not LorenzMie(double_precision=False).double_precision

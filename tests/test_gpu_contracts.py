@@ -159,3 +159,27 @@ def test_estimator_cases(api, estimator_case, name):
         exclude[:, hologram.shape[1] // 2:] = True
         est = _quick(cls(model=model, seed=0, exclude=exclude))
         assert isinstance(est.estimate(hologram), pd.Series)
+
+
+def test_single_precision_hologram(api):
+    """double_precision = False (theory/cupyLorenzMie.py:46-65): the
+    hologram in FP32 arithmetic, within 1e-4 + 2e-5 I of the FP64 one and
+    of the oracle; a non-Cartesian hologram stays FP64."""
+    from oracle import lorenzmie as ref
+    theory, _ = api
+    p = theory.Sphere(x_p=64.3, y_p=63.6, z_p=120., a_p=1.2, n_p=1.5, k_p=0.001)
+    m = theory.LorenzMie(coordinates=theory.LorenzMie.meshgrid((128, 128)),
+                         particle=p)
+    h64 = m.hologram()
+    m.double_precision = False
+    h32 = m.hologram()
+    assert not np.array_equal(h32, h64)
+    assert np.all(np.abs(h32 - h64) <= 1e-4 + 2e-5 * h64)
+    want = ref.hologram(ref.meshgrid((128, 128)),
+                        dict(x_p=64.3, y_p=63.6, z_p=120., a_p=1.2, n_p=1.5,
+                             k_p=0.001), n_m=1.34, wavelength=0.447,
+                        magnification=0.048)
+    assert np.all(np.abs(h32 - want) <= 1e-4 + 2e-5 * want)
+    np.testing.assert_array_equal(m.hologram(cartesian=False),
+                                  theory.LorenzMie(coordinates=m.coordinates,
+                                                   particle=p).hologram(cartesian=False))

@@ -19,11 +19,13 @@ Also measured:
           the speedup;
   roofline  the Jacobian kernel (dominant) and the trial-forward kernel,
           timed with HIP events on the stream they run on, against the FP32
-          VALU peak, with the survey's algorithmic FLOP counts;
+          VALU peak, with algorithmic FLOP counts per pixel of 296 N + 480
+          (Jacobian) and 64 N + 70 (forward), N the order count;
   cpu_baseline  the FP64 oracle (numpy model + scipy MINPACK LM) on one
-          feature on the host.
+          feature on the host (--cpu-workers N: also on N host cores).
 
---extra superposed | frame | de measures a path around the fitter instead:
+--extra superposed | frame | de | batch measures a path around the fitter
+instead:
   the five-sphere hologram of the reference's MultiSphere example, a
   frame's features fitted as one batch from device memory, and the DE
   estimator's GPU population (one JSON line each).
@@ -869,9 +871,9 @@ def main():
                          'headline (one JSON line)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py runs on one GPU: this build has no multi-GPU path '
-              '(features shard across GPUs with no communication; see '
-              'DESIGN.md).  Use --gpus 1.', file=sys.stderr)
+        print('bench.py runs on one GPU: there is no multi-GPU path '
+              '(features would shard across GPUs with no communication).  '
+              'Use --gpus 1.', file=sys.stderr)
         sys.exit(2)
     import torch
     assert torch.cuda.is_available(), 'bench.py needs a GPU'

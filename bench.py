@@ -27,8 +27,9 @@ Also measured:
 --extra superposed | frame | de | batch measures a path around the fitter
 instead:
   the five-sphere hologram of the reference's MultiSphere example, a
-  frame's features fitted as one batch from device memory, and the DE
-  estimator's GPU population (one JSON line each).
+  frame's features fitted as one batch from device memory, the DE
+  estimator's GPU population, and LorenzMieBatch.batch_hologram of 256
+  particle lists (one JSON line each).
 
 --config 1..4 measures the other configurations instead (one JSON line):
   1  batched forward holograms (1024 crops, side U{64..400}): holograms/s,

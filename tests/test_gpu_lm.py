@@ -177,3 +177,43 @@ def test_nonfinite_data_reports_failure(api):
     data[3, 4] = np.nan
     res = fit_one(api, data, (20., 20., 100., 1.0, 1.5, 0.), 0.05)[1]
     assert not res['success']
+
+
+def test_robust_fit_matches_oracle(api):
+    """robust=True (analysis/Optimizer.py:120-127: trf with the Cauchy
+    loss) on a hologram with 40 outlier pixels: the GPU residuals and
+    analytic Jacobian under scipy's trf reach the robust minimum that the
+    oracle's fitter (the same settings, forward differences as the
+    reference's numpy model) reaches, and it lies nearer the truth than
+    the plain least-squares fit, which the outliers pull."""
+    from oracle import fit as ofit, lorenzmie as ref
+    theory, analysis = api
+    truth = dict(x_p=50.3, y_p=49.6, z_p=140., a_p=0.8, n_p=1.47, k_p=0.)
+    c = ref.meshgrid((101, 101))
+    rng = np.random.default_rng(21)
+    d = ref.hologram(c, truth, **INSTRUMENT) + 0.05 * rng.standard_normal(c.shape[1])
+    bad = rng.choice(d.size, 40, replace=False)
+    d[bad] += 3.
+    d = d.reshape(101, 101)
+    start = dict(truth, x_p=51.5, y_p=48.8, z_p=143., a_p=0.82, n_p=1.46)
+    V5 = ['x_p', 'y_p', 'z_p', 'a_p', 'n_p']
+    tight = {'ftol': 1e-10, 'xtol': 1e-10, 'gtol': 1e-10}
+    fits = {}
+    for robust in (True, False):
+        inst = theory.Instrument(noise=0.05, **INSTRUMENT)
+        m = theory.LorenzMie(coordinates=theory.LorenzMie.meshgrid((101, 101)),
+                             particle=theory.Sphere(**start), instrument=inst)
+        opt = analysis.Optimizer(model=m, robust=robust)
+        opt.variables = V5
+        opt.settings.update(tight)
+        fits[robust] = opt.optimize(analysis.Hologram(d))
+    settings = dict(tight, method='trf', loss='cauchy')
+    want = ofit.fit(d, c, start, variables=V5, instrument=INSTRUMENT,
+                    noise=0.05, settings=settings)
+    got = fits[True]
+    for v in V5:
+        print(v, 'gpu', got[v], 'oracle', want[v], 'lm', fits[False][v])
+        assert abs(got[v] - want[v]) <= max(1e-4 * abs(want[v]),
+                                            0.05 * want['d' + v]), v
+    err = lambda r: sum(((r[v] - truth[v]) / want['d' + v]) ** 2 for v in V5)
+    assert err(got) < err(fits[False])

@@ -245,3 +245,38 @@ def test_large_homogeneous_lm(torch):
     assert good.mean() >= 0.9
     rel = np.abs(p[good, 3:5] - params[good, 3:5]) / params[good, 3:5]
     assert np.median(rel[:, 0]) < 1e-3 and np.median(rel[:, 1]) < 1e-3
+
+
+def test_optimizer_with_layered_particle(torch):
+    """The drop-in Optimizer on a model whose particle is a core-shell
+    LayeredSphere: the fit runs scipy's MINPACK on the host with the GPU's
+    FP64 residuals (analysis/Optimizer.py:216-221, no analytic Jacobian for
+    a non-Sphere particle) and reaches the minimum that the oracle's fitter
+    reaches with the same coefficients held, to 1e-4 relative."""
+    from pylorenzmie_amd import analysis, theory
+    from oracle import fit as ofit
+    lay = np.array([[0.45, 1.70, 0.0], [0.95, 1.42, 0.001]])
+    ab = oracle_ab(lay)
+    truth = dict(x_p=40.4, y_p=39.7, z_p=130., ab=ab)
+    c = ref.meshgrid((81, 81))
+    rng = np.random.default_rng(31)
+    d = (ref.hologram(c, truth, **INSTRUMENT) +
+         0.05 * rng.standard_normal(c.shape[1])).reshape(81, 81)
+    start = dict(x_p=41.2, y_p=39.0, z_p=133.)
+    p = theory.LayeredSphere(a_p=list(lay[:, 0]), n_p=list(lay[:, 1]),
+                             k_p=list(lay[:, 2]), **start)
+    m = theory.LorenzMie(coordinates=theory.LorenzMie.meshgrid((81, 81)),
+                         particle=p, instrument=theory.Instrument(noise=0.05,
+                                                                  **INSTRUMENT))
+    opt = analysis.Optimizer(model=m)
+    opt.variables = ['x_p', 'y_p', 'z_p']
+    tight = {'ftol': 1e-10, 'xtol': 1e-10, 'gtol': 1e-10}
+    opt.settings.update(tight)
+    res = opt.optimize(analysis.Hologram(d))
+    want = ofit.fit(d, c, dict(start, ab=ab), variables=['x_p', 'y_p', 'z_p'],
+                    instrument=INSTRUMENT, noise=0.05, settings=tight)
+    assert bool(res['success'])
+    for v in ('x_p', 'y_p', 'z_p'):
+        print(v, res[v], want[v])
+        assert abs(res[v] - want[v]) <= 1e-4 * abs(want[v]), v
+        assert abs(res['d' + v] - want['d' + v]) <= 0.02 * want['d' + v], v

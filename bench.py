@@ -799,6 +799,13 @@ def run_extra(args):
         for _ in range(reps):
             h = m.batch_hologram(lists)
         tb = (time.perf_counter() - t0) / reps
+        m.batch_hologram(lists, device=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            m.batch_hologram(lists, device=True)
+        torch.cuda.synchronize()
+        td = (time.perf_counter() - t0) / reps
         # the same holograms one LorenzMie.hologram() call at a time
         m1 = LorenzMie(coordinates=LorenzMie.meshgrid((side, side)),
                        instrument=Instrument())
@@ -821,6 +828,8 @@ def run_extra(args):
                              f'of 1-3 spheres (a_p 0.4-1.5 um) on one '
                              f'{side} x {side} grid, one pass',
                     ms_per_batch=tb * 1e3, holograms_per_s=B / tb,
+                    ms_per_batch_on_device=td * 1e3,
+                    holograms_per_s_on_device=B / td,
                     sequential_ms_per_hologram=ts * 1e3,
                     speedup_vs_sequential=ts * B / tb,
                     max_abs_err_vs_oracle=err,

@@ -27,13 +27,20 @@ class LorenzMieBatch(LorenzMie):
     members."""
 
     def batch_hologram(self, particle_lists: list, cartesian: bool = True,
-                       bohren: bool = True) -> np.ndarray:
+                       bohren: bool = True, device: bool = False):
         """(B, npts) float64 holograms |x + alpha E_s|^2, one per particle
-        list (theory/torchLorenzMieBatch.py:258-283)."""
+        list (theory/torchLorenzMieBatch.py:258-283); ``device=True``
+        returns the float32 tensor on the GPU instead, as hologram() does
+        (no copy to the host)."""
         B, P = len(particle_lists), self._npts
         out = self._run_batch(particle_lists, cartesian, bohren, hologram=True)
         if out is None:
+            if device:
+                t = _lmz.torch()
+                return t.ones(B, P, dtype=t.float32, device=_lmz.device())
             return np.ones((B, P))
+        if device:
+            return out.view(B, P)
         return out.view(B, P).double().cpu().numpy()
 
     def batch_field(self, particle_lists: list, cartesian: bool = True,

@@ -115,3 +115,14 @@ def test_aberrated_batch(theory):
     c = ref.meshgrid((48, 48))
     for i, plist in enumerate(lists):
         assert np.abs(h[i] - _oracle(c, plist, spherical=0.6)).max() < 1e-5, i
+
+
+def test_batch_hologram_on_device(case):
+    """device=True: the same holograms as a float32 tensor on the GPU."""
+    import torch
+    _, model, lists = case
+    h = model.batch_hologram(lists)
+    hd = model.batch_hologram(lists, device=True)
+    assert isinstance(hd, torch.Tensor) and hd.dtype == torch.float32 and hd.is_cuda
+    np.testing.assert_array_equal(hd.cpu().numpy().astype(np.float64), h)
+    assert tuple(model.batch_hologram([[]], device=True).shape) == (1, h.shape[1])

@@ -15,9 +15,22 @@ import numpy as np
 from .. import _lmz
 from ..lib import LMObject
 from .Instrument import Instrument
+from .Particle import Particle
 from .Sphere import Sphere
 
 PARAMS = ('x_p', 'y_p', 'z_p', 'a_p', 'n_p', 'k_p', 'alpha')
+
+
+class _GivenCoefficients(Particle):
+    """A particle at the origin whose Mie coefficients are given
+    (LorenzMie.lorenzmie evaluates them at arbitrary displacements)."""
+
+    def __init__(self, ab) -> None:
+        super().__init__(x_p=0., y_p=0., z_p=0.)
+        self._ab = np.asarray(ab, complex)
+
+    def ab(self, n_m=None, wavelength=None) -> np.ndarray:
+        return self._ab
 
 
 def dense_layout(c: np.ndarray):
@@ -293,6 +306,43 @@ class LorenzMie(LMObject):
             return fld
         self._field[...] = fld.cpu().numpy()
         return self._field
+
+    def scattered_field(self, particle, cartesian: bool = True,
+                        bohren: bool = True) -> np.ndarray:
+        """Field of ``particle`` alone at the coordinates, the propagation
+        phase exp(-i k z_p) included and no alpha, (3, npts) complex, a
+        new array (theory/LorenzMie.py:135-158)."""
+        if self._npts == 0:
+            return np.zeros((3, 0), complex)
+        need = self._particle_cap(particle)
+        cur = self._batch_key[4] if self._batch_key else 0
+        d = self._device_buffers(cur if need <= cur else need + 8)
+        alpha = self.alpha
+        try:
+            self.alpha = 1.
+            self._load_particle(particle, d)
+        finally:
+            self.alpha = alpha
+        _lmz.check(_lmz.load_library().lmz_field(
+            self._batch.handle, _lmz.ptr(d['params']), None,
+            _lmz.ptr(d['nmax']), _lmz.ptr(d['coef']), _lmz.ptr(d['coords']),
+            self._flags(cartesian, bohren), _lmz.ptr(d['field']),
+            _lmz.stream_handle()))
+        t = _lmz.torch()
+        fld = t.view_as_complex(d['field'].view(3, self._npts, 2))
+        return fld.cpu().numpy().astype(complex)
+
+    def lorenzmie(self, ab: np.ndarray, kdr: np.ndarray,
+                  cartesian: bool = True, bohren: bool = True) -> np.ndarray:
+        """Field of coefficients ``ab`` ((N + 1, 2), row 0 unused) at the
+        wavenumber-scaled displacements ``kdr`` (3, npts) from the particle,
+        without the propagation phase (theory/LorenzMie.py:183-342): the
+        per-order sum evaluated on the GPU for a particle at the origin."""
+        kdr = np.atleast_2d(np.asarray(kdr, float))
+        k = self.instrument.wavenumber()
+        m = LorenzMie(coordinates=kdr / k, particle=_GivenCoefficients(ab),
+                      instrument=self.instrument)
+        return m.scattered_field(m.particle, cartesian, bohren)
 
     def hologram(self, cartesian: bool = True, bohren: bool = True,
                  device: bool = False):

@@ -183,3 +183,38 @@ def test_single_precision_hologram(api):
     np.testing.assert_array_equal(m.hologram(cartesian=False),
                                   theory.LorenzMie(coordinates=m.coordinates,
                                                    particle=p).hologram(cartesian=False))
+
+
+def test_scattered_field_and_lorenzmie_hooks(api):
+    """LorenzMie.scattered_field (theory/LorenzMie.py:135-158: one particle,
+    r_0 included, the phase exp(-i k z_p), no alpha) and the per-order sum
+    lorenzmie(ab, kdr) (:183-342) against the oracle's, for every flag.
+    lorenzmie's displacements reach the kernel as float32 offsets from a
+    rounded origin, whence its 1e-5 bound."""
+    from oracle import lorenzmie as ref
+    theory, _ = api
+    c = theory.LorenzMie.meshgrid((48, 40))
+    m = theory.LorenzMie(coordinates=c, alpha=0.7)
+    own = m.particle
+    p = theory.Sphere(x_p=20.3, y_p=23.8, z_p=110., a_p=0.9, n_p=1.5, k_p=0.002)
+    p.r_0 = [1.5, -2., 10.]
+    k = ref.wavenumber()
+    ab = ref.mie_coefficients(0.9, 1.5, 0.002, 1.34, 0.447)
+    for cart in (True, False):
+        for boh in (True, False):
+            got = m.scattered_field(p, cartesian=cart, bohren=boh)
+            want = ref.scattered_field(c, 20.3, 23.8, 110., ab, k, (1.5, -2., 10.),
+                                       cart, boh)
+            assert np.abs(got - want).max() < 1e-6 * np.abs(want).max(), (cart, boh)
+    assert m.alpha == 0.7 and m.particle is own
+    rng = np.random.default_rng(8)
+    n = 600
+    kdr = k * np.vstack([rng.uniform(-30., 30., n), rng.uniform(-30., 30., n),
+                         np.concatenate([rng.uniform(-200., -20., n // 2),
+                                         rng.uniform(20., 150., n - n // 2)])])
+    for cart in (True, False):
+        for boh in (True, False):
+            got = m.lorenzmie(ab, kdr, cartesian=cart, bohren=boh)
+            want = ref.lorenzmie(ab, kdr, cart, boh)
+            assert got.shape == (3, n)
+            assert np.abs(got - want).max() < 1e-5 * np.abs(want).max(), (cart, boh)

@@ -54,7 +54,13 @@ class LMObject:
                 self.logger.debug(f'Ignoring unknown property {name}')
 
     def to_json(self, **kwargs) -> str:
-        return json.dumps(self.properties, **kwargs)
+        """JSON of the properties; numpy scalars become Python numbers
+        (lib/LMObject.py:110-127)."""
+        def encode(obj):
+            if isinstance(obj, np.generic):
+                return obj.item()
+            raise TypeError(f'{type(obj).__name__} is not JSON serializable')
+        return json.dumps(self.properties, default=encode, **kwargs)
 
     def from_json(self, s: str) -> None:
         self.properties = json.loads(s)

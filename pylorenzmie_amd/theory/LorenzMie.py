@@ -311,9 +311,18 @@ class LorenzMie(LMObject):
                         bohren: bool = True) -> np.ndarray:
         """Field of ``particle`` alone at the coordinates, the propagation
         phase exp(-i k z_p) included and no alpha, (3, npts) complex, a
-        new array (theory/LorenzMie.py:135-158)."""
+        new array (theory/LorenzMie.py:135-158); times the particle's
+        aberration mask when ``spherical`` is not 0 (theory/Aberrated.py:
+        78-84)."""
         if self._npts == 0:
             return np.zeros((3, 0), complex)
+        if getattr(self, 'spherical', 0.) != 0.:
+            own, alpha = self.particle, self.alpha
+            try:
+                self.particle, self.alpha = particle, 1.
+                return self.field(cartesian, bohren).copy()
+            finally:
+                self.particle, self.alpha = own, alpha
         need = self._particle_cap(particle)
         cur = self._batch_key[4] if self._batch_key else 0
         d = self._device_buffers(cur if need <= cur else need + 8)

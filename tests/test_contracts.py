@@ -569,3 +569,15 @@ def test_double_precision_switch():
     m.double_precision = True
     assert m.double_precision
     assert not LorenzMie(double_precision=False).double_precision
+
+
+def test_to_json_encodes_numpy_scalars():
+    """lib/LMObject.py:110-127: numpy scalars in the properties serialize
+    as numbers."""
+    s = Sphere(a_p=np.float32(0.5), n_p=np.float64(1.4))
+    s.x_p = np.int64(3)
+    b = Sphere()
+    b.from_json(s.to_json())
+    assert (b.a_p, b.n_p, b.x_p) == (0.5, 1.4, 3)
+    with pytest.raises(TypeError):
+        Sphere(a_p=object()).to_json()

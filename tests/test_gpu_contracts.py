@@ -218,3 +218,20 @@ def test_scattered_field_and_lorenzmie_hooks(api):
             want = ref.lorenzmie(ab, kdr, cart, boh)
             assert got.shape == (3, n)
             assert np.abs(got - want).max() < 1e-5 * np.abs(want).max(), (cart, boh)
+
+
+def test_aberrated_scattered_field(api):
+    """scattered_field of an aberrated model carries the particle's mask
+    (theory/Aberrated.py:78-84)."""
+    from oracle import lorenzmie as ref
+    theory, _ = api
+    c = theory.LorenzMie.meshgrid((40, 44))
+    m = theory.AberratedLorenzMie(coordinates=c, spherical=0.7, alpha=0.8)
+    p = theory.Sphere(x_p=21.3, y_p=19.6, z_p=95., a_p=0.7, n_p=1.55)
+    k = ref.wavenumber()
+    ab = ref.mie_coefficients(0.7, 1.55, 0., 1.34, 0.447)
+    want = ref.scattered_field(c, 21.3, 19.6, 95., ab, k) * \
+        ref.aberration(c, np.array([21.3, 19.6, 95.]), k, 1.34, 0.7)
+    got = m.scattered_field(p)
+    assert np.abs(got - want).max() < 1e-6 * np.abs(want).max()
+    assert m.alpha == 0.8 and m.particle is not p

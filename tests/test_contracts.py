@@ -581,3 +581,11 @@ def test_to_json_encodes_numpy_scalars():
     assert (b.a_p, b.n_p, b.x_p) == (0.5, 1.4, 3)
     with pytest.raises(TypeError):
         Sphere(a_p=object()).to_json()
+
+
+def test_lmobject_equality():
+    """tests/test_lib.py:51-64: test_equal_same_class,
+    test_not_equal_same_class, test_not_equal_different_class."""
+    assert Sphere(a_p=1.0, n_p=1.5) == Sphere(a_p=1.0, n_p=1.5)
+    assert Sphere(a_p=1.0, n_p=1.5) != Sphere(a_p=2.0, n_p=1.5)
+    assert Sphere() != Instrument()

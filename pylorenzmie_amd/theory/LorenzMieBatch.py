@@ -57,6 +57,19 @@ class LorenzMieBatch(LorenzMie):
         return fld.permute(1, 0, 2).contiguous()
 
     # ------------------------------------------------------------ device
+    def _allocate(self) -> None:
+        super()._allocate()
+        self._cache = {}          # batches kept between calls
+
+    def _cached(self, name, key, make):
+        """The object ``make()`` builds for ``key``, kept until the key or
+        the coordinates change."""
+        hit = self._cache.get(name)
+        if hit is None or hit[0] != key:
+            hit = (key, make())
+            self._cache[name] = hit
+        return hit[1]
+
     def _run_batch(self, particle_lists, cartesian, bohren, hologram):
         """The superposition launch over B features; None when there is
         nothing to scatter (no lists, no pixels or no particles)."""
@@ -70,10 +83,13 @@ class LorenzMieBatch(LorenzMie):
         lib = _lmz.load_library()
         stream = _lmz.stream_handle()
         inst = self.instrument
+        ikey = (float(np.real(inst.n_m)), float(inst.wavelength),
+                float(inst.magnification), float(inst.noise))
         n = len(flat)
         pcap = max(self._particle_cap(q) for q in flat)
-        pb = _lmz.Batch([_lmz.make_feature(1, 1, (0., 0., 0.), inst, pcap, i)
-                         for i in range(n)], 1)
+        pb = self._cached('particles', (n, pcap, ikey), lambda: _lmz.Batch(
+            [_lmz.make_feature(1, 1, (0., 0., 0.), inst, pcap, i)
+             for i in range(n)], 1))
         pp = np.zeros((n, _lmz.NPARAM))
         pp[:, 6] = 1.
         for i, q in enumerate(flat):
@@ -104,13 +120,14 @@ class LorenzMieBatch(LorenzMie):
                                             _lmz.ptr(nin_d), _lmz.ptr(pnmax),
                                             _lmz.ptr(coef), stream))
         # the B features: the model's pixels, once per particle list
-        fb = _lmz.Batch([_lmz.make_feature(P, self._width, self._origin, inst,
-                                           1, b * P) for b in range(B)], 1)
+        fb = self._cached('pixels', (B, ikey), lambda: _lmz.Batch(
+            [_lmz.make_feature(P, self._width, self._origin, inst, 1, b * P)
+             for b in range(B)], 1))
         coords = None
         if self._rel is not None:
-            coords = t.from_numpy(np.ascontiguousarray(
-                np.tile(self._rel.reshape(3, 1, P), (1, B, 1)).reshape(3, B * P))
-            ).to(dev)
+            coords = self._cached('coords', B, lambda: t.from_numpy(
+                np.ascontiguousarray(np.tile(self._rel.reshape(3, 1, P),
+                                             (1, B, 1)).reshape(3, B * P))).to(dev))
         offset = t.from_numpy(np.concatenate(
             [[0], np.cumsum([len(m) for m in members])]).astype(np.int32)).to(dev)
         alpha = t.full((B,), float(self.alpha), dtype=t.float64, device=dev)
@@ -125,7 +142,4 @@ class LorenzMieBatch(LorenzMie):
             _lmz.ptr(coef), pcap, _lmz.ptr(coords), self._flags(cartesian, bohren),
             float(getattr(self, 'spherical', 0.)), _lmz.ptr(alpha),
             _lmz.ptr(holo), _lmz.ptr(fld), stream))
-        t.cuda.current_stream().synchronize()
-        pb.close()
-        fb.close()
         return out

@@ -41,7 +41,14 @@ class LorenzMieBatch(LorenzMie):
             return np.ones((B, P))
         if device:
             return out.view(B, P)
-        return out.view(B, P).double().cpu().numpy()
+        # a fresh float64 array in pinned host memory (torch recycles the
+        # blocks once the array is dropped): a pageable copy of this size
+        # runs several times slower
+        t = _lmz.torch()
+        host = t.empty((B, P), dtype=t.float64, pin_memory=True)
+        host.copy_(out.view(B, P).double(), non_blocking=True)
+        t.cuda.current_stream().synchronize()
+        return host.numpy()
 
     def batch_field(self, particle_lists: list, cartesian: bool = True,
                     bohren: bool = True):

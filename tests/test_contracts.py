@@ -2,8 +2,8 @@
 tests/test_hologram.py, test_mask.py, test_instrument.py,
 test_particle.py, test_sphere.py, test_lorenzmie.py, test_optimizer.py,
 test_feature.py, test_frame.py, test_cluster.py, test_dimer.py,
-test_deestimator.py, test_radialestimator.py and test_radialmask.py.
-Each test names the
+test_deestimator.py, test_radialestimator.py, test_radialmask.py,
+test_baseestimator.py and test_lib.py (LMObject).  Each test names the
 reference cases it covers.  Cases that compute on the GPU (Sphere.ab,
 field, hologram, optimize, estimate, predicted, residuals) are in
 tests/test_gpu_contracts.py and tests/test_gpu_crop.py; nothing here
@@ -589,3 +589,38 @@ def test_lmobject_equality():
     assert Sphere(a_p=1.0, n_p=1.5) == Sphere(a_p=1.0, n_p=1.5)
     assert Sphere(a_p=1.0, n_p=1.5) != Sphere(a_p=2.0, n_p=1.5)
     assert Sphere() != Instrument()
+
+
+def test_base_estimator_contract():
+    """tests/test_baseestimator.py: test_cannot_instantiate_abstract,
+    test_concrete_subclass_instantiates, test_estimate_returns_series,
+    test_missing_estimate_raises, test_missing_properties_raises."""
+    from pylorenzmie_amd.analysis import BaseEstimator
+
+    class Concrete(BaseEstimator):
+        @BaseEstimator.properties.getter
+        def properties(self):
+            return dict()
+
+        def estimate(self, hologram):
+            return pd.Series({'x_p': 0., 'y_p': 0.})
+
+    with pytest.raises(TypeError):
+        BaseEstimator()
+    est = Concrete()
+    assert isinstance(est, BaseEstimator)
+    assert isinstance(est.estimate(Hologram(np.random.rand(50, 50))), pd.Series)
+
+    class NoEstimate(BaseEstimator):
+        @BaseEstimator.properties.getter
+        def properties(self):
+            return dict()
+
+    class NoProperties(BaseEstimator):
+        def estimate(self, hologram):
+            return pd.Series(dtype=float)
+
+    with pytest.raises(TypeError):
+        NoEstimate()
+    with pytest.raises(TypeError):
+        NoProperties()

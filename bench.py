@@ -725,6 +725,26 @@ def run_extra(args):
         fe.estimate()
         torch.cuda.synchronize()
         est_ms = (time.perf_counter() - t0) * 1e3
+        fits = fe.optimize()
+        seq_ok = np.mean([all(abs(fits.iloc[i][v] - q[j]) <= 6 * fits.iloc[i]['d' + v]
+                              for j, v in enumerate(('x_p', 'y_p', 'z_p', 'a_p', 'n_p')))
+                          for i, q in enumerate(truth)])
+        line.update(fits_from_sequential_estimates_at_truth=float(seq_ok))
+        # the same searches advanced together, then the fits from them
+        fb = setup(dev)
+        for ft in fb.features:
+            ft.estimator.seed = 0
+        np.random.seed(2)
+        t0 = time.perf_counter()
+        fb.estimate(batched=True)
+        torch.cuda.synchronize()
+        estb_ms = (time.perf_counter() - t0) * 1e3
+        fitb = fb.optimize()
+        truth_ok = np.mean([all(abs(fitb.iloc[i][v] - q[j]) <= 6 * fitb.iloc[i]['d' + v]
+                                for j, v in enumerate(('x_p', 'y_p', 'z_p', 'a_p', 'n_p')))
+                            for i, q in enumerate(truth)])
+        line.update(frame_estimate_batched_ms=estb_ms,
+                    fits_from_batched_estimates_at_truth=float(truth_ok))
         line.update(workload=f'{k} features of 201x201 (10% masks) in a '
                              '1280x1024 frame in device memory',
                     frame_optimize_ms=tb * 1e3, fits_per_s=k / tb,

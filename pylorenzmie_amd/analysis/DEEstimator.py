@@ -184,3 +184,132 @@ def _objective(prob: dict, chi2):
         p[:, cols] = x
         return chi2(p)
     return objective
+
+
+class _FramePopulation:
+    """chi2 of every candidate of every feature in one device launch: the
+    batch holds NP copies of each feature's masked pixels, feature-major."""
+
+    def __init__(self, probs: list, npop: int):
+        t = _lmz.torch()
+        self._t = t
+        dev = _lmz.device()
+        inst = probs[0]['instrument']
+        cap = max(p['cap'] for p in probs)
+        feats, coords, data, owner = [], [], [], []
+        off = 0
+        for p in probs:
+            c = np.atleast_2d(np.asarray(p['coords'], float))
+            if c.shape[0] == 2:
+                c = np.vstack([c, np.zeros((1, c.shape[1]))])
+            origin = (float(np.round(c[0].mean())), float(np.round(c[1].mean())), 0.)
+            rel = (c - np.asarray(origin)[:, None]).astype(np.float32)
+            d = np.asarray(p['data'], np.float32).ravel()
+            for _ in range(npop):
+                feats.append(_lmz.make_feature(d.size, 0, origin, inst, cap,
+                                               pix_offset=off))
+                coords.append(rel)
+                data.append(d)
+                off += d.size
+        self.batch = _lmz.Batch(feats, 1)
+        b = self.batch
+        self.coords = t.from_numpy(np.ascontiguousarray(
+            np.concatenate(coords, axis=1))).to(dev)
+        self.data = t.from_numpy(np.concatenate(data)).to(dev)
+        self.params = t.zeros((len(feats), _lmz.NPARAM), dtype=t.float64, device=dev)
+        self.nmax = t.zeros(len(feats), dtype=t.int32, device=dev)
+        self.coef = t.zeros(b.coef_floats(), dtype=t.float32, device=dev)
+        self.work = t.zeros(b.work_doubles, dtype=t.float64, device=dev)
+        self.chi2 = t.zeros(b.ntiles, dtype=t.float64, device=dev)
+        ntiles = [f.ntiles for f in b.features]
+        self.tile_owner = t.from_numpy(np.repeat(np.arange(len(feats)),
+                                                 ntiles)).to(dev)
+        self.n = len(feats)
+
+    def __call__(self, rows: np.ndarray) -> np.ndarray:
+        t = self._t
+        host = np.zeros((self.n, _lmz.NPARAM))
+        host[:, :7] = rows
+        self.params.copy_(t.from_numpy(host))
+        lib = _lmz.load_library()
+        s = _lmz.stream_handle()
+        b = self.batch
+        _lmz.check(lib.lmz_mie_ab(b.handle, _lmz.ptr(self.params), None,
+                                  _lmz.ptr(self.nmax), _lmz.ptr(self.coef), None,
+                                  None, _lmz.ptr(self.work), s))
+        _lmz.check(lib.lmz_hologram(b.handle, _lmz.ptr(self.params), None,
+                                    _lmz.ptr(self.nmax), _lmz.ptr(self.coef),
+                                    _lmz.ptr(self.coords),
+                                    _lmz.CARTESIAN | _lmz.BOHREN | _lmz.FP32,
+                                    _lmz.ptr(self.data), None,
+                                    _lmz.ptr(self.chi2), None, s))
+        out = t.zeros(self.n, dtype=t.float64, device=self.chi2.device)
+        out.index_add_(0, self.tile_owner, self.chi2)
+        chi2 = out.cpu().numpy()
+        return np.where(np.isfinite(chi2), chi2, np.inf)
+
+
+def estimate_batch(estimators: list, holograms: list, maxiter: int = 1000
+                   ) -> list:
+    """One differential-evolution search per (estimator, hologram) pair,
+    all advanced together: every generation's candidates of every search in
+    one device launch, and scipy's best1bin step (dithered scale in
+    [0.5, 1) drawn per generation, binomial crossover at 0.7 with one forced
+    dimension, out-of-range values redrawn, deferred selection, stop when
+    std(energies) <= tol |mean(energies)|) applied to all of them at once
+    as array operations.  The estimators must share bounds keys, popsize
+    and tol.  Returns each estimator's _finish (its model updated)."""
+    if not estimators:
+        return []
+    e0 = estimators[0]
+    names = list(e0.bounds.keys())
+    D, F = len(names), len(estimators)
+    npop = e0.popsize * D
+    tol = float(e0.settings.get('tol', 0.01))
+    for e in estimators:
+        if list(e.bounds.keys()) != names or e.popsize != e0.popsize:
+            raise ValueError('estimators differ in bounds or popsize')
+    probs = [e._problem(h) for e, h in zip(estimators, holograms)]
+    lo = np.array([[e.bounds[k][0] for k in names] for e in estimators])
+    hi = np.array([[e.bounds[k][1] for k in names] for e in estimators])
+    base = np.stack([p['base'] for p in probs])
+    cols = probs[0]['cols']
+    pop_chi2 = _FramePopulation(probs, npop)
+    rng = np.random.default_rng(e0.seed)
+
+    def energies(unit):
+        rows = np.repeat(base, npop, axis=0)
+        rows[:, cols] = (lo[:, None, :] + unit * (hi - lo)[:, None, :]).reshape(-1, D)
+        return pop_chi2(rows).reshape(F, npop)
+
+    # Latin hypercube start, per search and dimension
+    unit = (np.argsort(rng.random((F, npop, D)), axis=1) + rng.random((F, npop, D))) / npop
+    e = energies(unit)
+    done = np.zeros(F, bool)
+    idx = np.arange(npop)
+    for _ in range(maxiter):
+        best = unit[np.arange(F), np.argmin(e, axis=1)]             # (F, D)
+        scale = rng.uniform(0.5, 1., (F, 1, 1))
+        k0 = rng.integers(0, npop - 1, (F, npop))
+        r0 = k0 + (k0 >= idx)
+        a, b = np.minimum(idx, r0), np.maximum(idx, r0)
+        r1 = rng.integers(0, npop - 2, (F, npop))
+        r1 = r1 + (r1 >= a)
+        r1 = r1 + (r1 >= b)
+        take = np.arange(F)[:, None]
+        mutant = best[:, None, :] + scale * (unit[take, r0] - unit[take, r1])
+        cross = rng.random((F, npop, D)) < 0.7
+        cross[take, idx[None, :], rng.integers(0, D, (F, npop))] = True
+        trial = np.where(cross, mutant, unit)
+        bad = (trial < 0.) | (trial > 1.)
+        trial[bad] = rng.random(int(bad.sum()))
+        et = energies(trial)
+        better = (et < e) & ~done[:, None]
+        unit = np.where(better[..., None], trial, unit)
+        e = np.where(better, et, e)
+        with np.errstate(invalid='ignore'):     # a search holding inf waits
+            done |= np.std(e, axis=1) <= tol * np.abs(np.mean(e, axis=1))
+        if done.all():
+            break
+    x = lo + unit[np.arange(F), np.argmin(e, axis=1)] * (hi - lo)
+    return [est._finish(xi) for est, xi in zip(estimators, x)]

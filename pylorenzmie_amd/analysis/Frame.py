@@ -145,10 +145,17 @@ class Frame(Hologram):
         raise NotImplementedError('feature detection (Localizer) is not part '
                                   'of this library: set bboxes')
 
-    def estimate(self) -> None:
-        """Starting values of every feature, one DE search after another
+    def estimate(self, batched: bool = False) -> None:
+        """Starting values of every feature: one DE search after another
         (analysis/Frame.py:158-161), each generation's population on the
-        GPU."""
+        GPU; or, with ``batched``, all searches advanced together with
+        every generation of every feature in one launch
+        (DEEstimator.estimate_batch; the features' estimators must share
+        bounds keys and popsize)."""
+        if batched and self.features:
+            from .DEEstimator import estimate_batch
+            estimate_batch([f.estimator for f in self.features], self.features)
+            return
         for feature in self.features:
             feature.estimate()
 

@@ -51,3 +51,45 @@ def test_detection_is_outside_estimation_is_de():
     est = fr.features[0].estimator
     assert est.exclude is fr.features[0].mask.exclude
     assert set(est.bounds) == {'z_p', 'a_p', 'n_p'} and est.fraction == 0.02
+
+
+def test_batched_de_finds_each_minimum(monkeypatch):
+    """DEEstimator.estimate_batch, the search itself, on the CPU: with the
+    device objective replaced by a bowl whose minimum differs per feature,
+    every search lands at its own minimum (the best1bin step, the index
+    draws and the per-search stopping rule), and each estimator's model
+    receives it."""
+    import importlib
+    from pylorenzmie_amd.analysis import DEEstimator
+    from pylorenzmie_amd.theory import LorenzMie
+    de_mod = importlib.import_module('pylorenzmie_amd.analysis.DEEstimator')
+    targets = np.array([[120., 0.8, 1.45], [300., 1.6, 1.60], [75., 0.5, 1.38]])
+    scale = np.array([50., 0.2, 0.05])
+
+    class Bowl:
+        def __init__(self, probs, npop):
+            self.npop = npop
+
+        def __call__(self, rows):
+            x = rows[:, [2, 3, 4]].reshape(len(targets), self.npop, 3)
+            return (((x - targets[:, None, :]) / scale) ** 2).sum(-1).ravel()
+
+    monkeypatch.setattr(de_mod, '_FramePopulation', Bowl)
+    ests, holos = [], []
+    for i in range(3):
+        m = LorenzMie()
+        ests.append(DEEstimator(model=m, seed=5,
+                                bounds={'z_p': (50., 400.), 'a_p': (0.3, 2.0),
+                                        'n_p': (1.35, 1.8)}))
+        holos.append(Hologram(np.random.default_rng(i).random((20, 20))))
+    ests[0].settings['tol'] = 1e-6
+    out = de_mod.estimate_batch(ests, holos)
+    for i, (res, est) in enumerate(zip(out, ests)):
+        got = np.array([res['z_p'], res['a_p'], res['n_p']])
+        err = np.abs(got - targets[i]) / scale
+        print(i, got, err)
+        assert np.all(err < 0.05), i
+        assert est.model.particle.z_p == res['z_p']
+    with pytest.raises(ValueError):
+        de_mod.estimate_batch([ests[0], DEEstimator(model=LorenzMie(), popsize=5)],
+                              holos[:2])

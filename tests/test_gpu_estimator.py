@@ -68,7 +68,11 @@ def test_de_estimate_then_fit(api):
         assert abs(res[v] - t) <= 5 * res['d' + v], v
 
 
-def test_frame_estimate_and_optimize(api):
+@pytest.mark.parametrize('batched', [False, True])
+def test_frame_estimate_and_optimize(api, batched):
+    """Frame.estimate then Frame.optimize recovers every feature's truth,
+    with one scipy DE search per feature (the reference's sequence) and
+    with all searches advanced together (estimate(batched=True))."""
     theory, analysis = api
     rng = np.random.default_rng(9)
     data = np.ones((300, 460))
@@ -94,7 +98,7 @@ def test_frame_estimate_and_optimize(api):
         f.estimator.bounds = {'z_p': (50., 400.), 'a_p': (0.3, 1.5),
                               'n_p': (1.35, 1.8)}
         f.mask.fraction = 0.5
-    fr.estimate()
+    fr.estimate(batched=batched)
     df = fr.optimize()
     print(df[['x_p', 'y_p', 'z_p', 'a_p', 'n_p', 'redchi']])
     for i, q in enumerate(truth):

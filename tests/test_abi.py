@@ -110,3 +110,25 @@ def test_frame_calls_reject_bad_arguments(lib):
     assert lib.lmz_gather_frame(None, fake, 5, 5, None, None, fake, None, fake,
                                 None, None) == 1
     assert b'batch' in lib.lmz_last_error()
+
+
+def test_last_error_is_per_thread(lib):
+    """lmz_last_error holds the calling thread's last failure: a failure on
+    another thread does not replace it (the ABI promises thread-safe calls
+    on distinct streams)."""
+    import threading
+    assert lib.lmz_mie_ab(None, None, None, None, None, None, None, None, None) == 1
+    mine = lib.lmz_last_error()
+    assert b'batch is NULL' in mine
+    seen = []
+
+    def other():
+        seen.append(lib.lmz_last_error())
+        assert lib.lmz_median3_u8(None, 5, None, None) == 1
+        seen.append(lib.lmz_last_error())
+    t = threading.Thread(target=other)
+    t.start()
+    t.join()
+    assert seen[0] in (b'', None)
+    assert seen[1] != mine
+    assert lib.lmz_last_error() == mine

@@ -253,6 +253,23 @@ int lmz_hologram_vjp(const lmz_batch *batch, const double *params,
                      const float *grad_out, double *tile_work,
                      double *grad_params, hipStream_t stream);
 
+/* lmz_hologram_vjp without tile_work: the per-tile partial gradients go to
+ * a fixed device array of LMZ_VJP_STATIC_MAX_TILES rows (3.5 MiB, one copy
+ * per GPU, held until the process ends), so the call allocates nothing and
+ * calls no allocating routine.  Same kernels and results as
+ * lmz_hologram_vjp; batches with more tiles are rejected (LMZ_EINVAL).
+ * Every call on a GPU shares that GPU's array: calls that overlap on one
+ * GPU (two streams, two threads) race and must be serialized by the caller.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc). */
+#define LMZ_VJP_STATIC_MAX_TILES 65536
+int lmz_hologram_vjp_static(const lmz_batch *batch, const double *params,
+                            const int32_t *nmax, const float *coef,
+                            const float *dcoef, const float *coords,
+                            int32_t flags, const float *grad_out,
+                            double *grad_params, hipStream_t stream);
+
 /* Launch shape of a batch's pixel kernels (see lmz_schedule). */
 int lmz_batch_set_schedule(lmz_batch *batch, const lmz_schedule *schedule);
 int lmz_batch_get_schedule(const lmz_batch *batch, lmz_schedule *schedule);

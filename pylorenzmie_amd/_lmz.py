@@ -21,6 +21,7 @@ NJAC = 7
 NRED = 36
 NRES = 40
 TILE = 1024
+VJP_STATIC_MAX_TILES = 65536
 TUNE_FORCE = 1
 CARTESIAN = 1
 BOHREN = 2
@@ -113,6 +114,7 @@ SIGNATURES = {
     'lmz_jacobian': (_I, [_P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P,
                           _P, _P]),
     'lmz_hologram_vjp': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
+    'lmz_hologram_vjp_static': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P]),
     'lmz_batch_set_schedule': (_I, [_P, ctypes.POINTER(Schedule)]),
     'lmz_batch_get_schedule': (_I, [_P, ctypes.POINTER(Schedule)]),
     'lmz_autotune_scratch_bytes': (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),

@@ -874,6 +874,56 @@ using namespace lmz;
 // ====================================================================
 // C ABI
 // ====================================================================
+// Per-tile partial gradients of lmz_hologram_vjp_static: a fixed device
+// array, so that the call allocates nothing.  Every GPU holds its own copy
+// until the process ends; its address is looked up once per device.  Each
+// call writes rows 0 .. ntiles - 1 before it reads them, so nothing is
+// cleared; calls that overlap on one GPU race for it.
+static __device__ __align__(256) double
+    scratch[(size_t)LMZ_VJP_STATIC_MAX_TILES * LMZ_NJAC];
+
+#define LMZ_MAX_DEVICES 64
+static std::mutex g_scratch_mutex;
+static double *g_scratch_ptr[LMZ_MAX_DEVICES];
+
+// Device address of the current device's scratch (hipGetSymbolAddress on
+// the first call on that device, with it current).
+static int vjp_scratch(double **out) {
+    int dev = 0;
+    LMZ_HIP(hipGetDevice(&dev));
+    LMZ_REQUIRE(dev >= 0 && dev < LMZ_MAX_DEVICES, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        LMZ_REQUIRE(p != nullptr, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return LMZ_OK;
+}
+
+// k_vjp over every tile into tile_work, then per-feature sums in order.
+static int launch_vjp(const lmz_batch *b, const double *params,
+                      const int32_t *nmax, const float *coef,
+                      const float *dcoef, const float *coords, int32_t flags,
+                      const float *grad_out, double *tile_work,
+                      double *grad_params, hipStream_t stream) {
+    hipLaunchKernelGGL(k_vjp, dim3(b->ntiles), dim3(LMZ_THREADS), 0, stream,
+                       b->d_feat, b->d_tile_feature, params, b->coef_buffers,
+                       nmax, coef, dcoef, coords, b->npix_total, flags,
+                       grad_out, tile_work);
+    LMZ_HIP(hipGetLastError());
+    const long long n = (long long)b->nfeat * LMZ_NJAC;
+    hipLaunchKernelGGL(k_tile_sum<LMZ_NJAC>,
+                       dim3((unsigned)((n + LMZ_THREADS - 1) / LMZ_THREADS)),
+                       dim3(LMZ_THREADS), 0, stream, b->d_feat, b->nfeat,
+                       tile_work, grad_params);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
 extern "C" {
 
 const char *lmz_last_error(void) { return g_error.c_str(); }
@@ -1021,6 +1071,16 @@ static int forward_args(const lmz_batch *b, const double *params,
     return LMZ_OK;
 }
 
+static int vjp_args(const lmz_batch *b, const double *params,
+                    const int32_t *nmax, const float *coef, const float *dcoef,
+                    const float *coords, int32_t flags) {
+    if (int rc = forward_args(b, params, nmax, coef, coords, nullptr)) return rc;
+    LMZ_REQUIRE(dcoef != nullptr, "dcoef (coefficient tangents) is required");
+    LMZ_REQUIRE(flags & LMZ_CARTESIAN,
+                "the Jacobian is defined for the Cartesian hologram");
+    return LMZ_OK;
+}
+
 int lmz_field(const lmz_batch *b, const double *params, const int32_t *sel,
               const int32_t *nmax, const float *coef, const float *coords,
               int32_t flags, float *field, hipStream_t stream) {
@@ -1137,24 +1197,30 @@ int lmz_hologram_vjp(const lmz_batch *b, const double *params,
                      const float *dcoef, const float *coords, int32_t flags,
                      const float *grad_out, double *tile_work,
                      double *grad_params, hipStream_t stream) {
-    if (int rc = forward_args(b, params, nmax, coef, coords, nullptr)) return rc;
-    LMZ_REQUIRE(dcoef != nullptr, "dcoef (coefficient tangents) is required");
-    LMZ_REQUIRE(flags & LMZ_CARTESIAN,
-                "the Jacobian is defined for the Cartesian hologram");
+    if (int rc = vjp_args(b, params, nmax, coef, dcoef, coords, flags))
+        return rc;
     LMZ_REQUIRE(grad_out && tile_work && grad_params,
                 "grad_out, tile_work and grad_params are required");
-    hipLaunchKernelGGL(k_vjp, dim3(b->ntiles), dim3(LMZ_THREADS), 0, stream,
-                       b->d_feat, b->d_tile_feature, params, b->coef_buffers,
-                       nmax, coef, dcoef, coords, b->npix_total, flags,
-                       grad_out, tile_work);
-    LMZ_HIP(hipGetLastError());
-    const long long n = (long long)b->nfeat * LMZ_NJAC;
-    hipLaunchKernelGGL(k_tile_sum<LMZ_NJAC>,
-                       dim3((unsigned)((n + LMZ_THREADS - 1) / LMZ_THREADS)),
-                       dim3(LMZ_THREADS), 0, stream, b->d_feat, b->nfeat,
-                       tile_work, grad_params);
-    LMZ_HIP(hipGetLastError());
-    return LMZ_OK;
+    return launch_vjp(b, params, nmax, coef, dcoef, coords, flags, grad_out,
+                      tile_work, grad_params, stream);
+}
+
+int lmz_hologram_vjp_static(const lmz_batch *b, const double *params,
+                            const int32_t *nmax, const float *coef,
+                            const float *dcoef, const float *coords,
+                            int32_t flags, const float *grad_out,
+                            double *grad_params, hipStream_t stream) {
+    if (int rc = vjp_args(b, params, nmax, coef, dcoef, coords, flags))
+        return rc;
+    LMZ_REQUIRE(grad_out && grad_params, "grad_out and grad_params are required");
+    LMZ_REQUIRE(b->ntiles <= LMZ_VJP_STATIC_MAX_TILES,
+                "the batch has more tiles than lmz_hologram_vjp_static's "
+                "scratch holds (LMZ_VJP_STATIC_MAX_TILES); use "
+                "lmz_hologram_vjp with a tile_work buffer");
+    double *work = nullptr;
+    if (int rc = vjp_scratch(&work)) return rc;
+    return launch_vjp(b, params, nmax, coef, dcoef, coords, flags, grad_out,
+                      work, grad_params, stream);
 }
 
 int lmz_batch_set_schedule(lmz_batch *b, const lmz_schedule *sc) {

@@ -23,12 +23,20 @@ Fits (all from the same starts, tolerances 1e-12)
                 /root/reference as tests/golden/make_golden.py does; nothing
                 else pins its end-to-end result on this workload).  Only for
                 the 32 headline features.
+  * exact J (--exact-jacobian, added to an existing headline.npz): the same
+                MINPACK (scipy least_squares, method 'lm', x_scale 'jac')
+                with the oracle's 5-point central-difference Jacobian
+                (oracle.lorenzmie.jacobian, errors ~1e-10 of scale) in place
+                of lmdif's forward differences, 7 parameters, for the
+                headline features and the small crops: the minimum MINPACK
+                reaches when its Jacobian is exact, as the device's
+                analytic Jacobian is.
 
 Runs ONLY in the development container (the reference is read there); the
 fixtures are data and travel with the repository.  About 3 minutes on 8
 CPUs.
 
-Usage:  python tests/golden/make_golden_headline.py
+Usage:  python tests/golden/make_golden_headline.py [--exact-jacobian]
 """
 import json
 import multiprocessing as mp
@@ -134,6 +142,56 @@ def _reference_fit(job):
     return r
 
 
+def _exact_j_fit(job):
+    from scipy.optimize import least_squares
+    from oracle import fit, lorenzmie as lm
+    data, side, start, sigma = job
+    c = lm.meshgrid((side, side))
+    data = data.ravel().astype(float)
+
+    def fun(x):
+        return fit.residuals(x, PARAMS, {}, data, c, INSTRUMENT, sigma)
+
+    def jac(x):
+        return lm.jacobian(c, dict(zip(PARAMS, x)), INSTRUMENT, order=4).T / sigma
+    t0 = time.perf_counter()
+    r = least_squares(fun, np.asarray(start, float), jac=jac, method='lm',
+                      x_scale='jac', ftol=TIGHT['ftol'], xtol=TIGHT['xtol'],
+                      gtol=TIGHT['gtol'], max_nfev=TIGHT['max_nfev'])
+    redchi, unc = fit.statistics(r.jac, r.cost, data.size, len(PARAMS))
+    return dict(values=r.x, uncertainties=unc, chi2=2. * r.cost,
+                nfev=r.nfev, success=bool(r.success),
+                seconds=time.perf_counter() - t0)
+
+
+def exact_jacobian():
+    path = os.path.join(HERE, 'headline.npz')
+    out = dict(np.load(path))
+    with open(os.path.join(HERE, 'headline.json')) as f:
+        meta = json.load(f)
+    jobs = [(out['head_data'][i], SIDE, out['head_start'][i], SIGMA)
+            for i in range(len(out['head_start']))]
+    off = out['small_offsets']
+    jobs_s = [(out['small_data'][off[k]:off[k + 1]], int(out['small_side'][k]),
+               out['small_start'][k], float(out['small_sigma'][k]))
+              for k in range(len(out['small_side']))]
+    os.environ.update(OMP_NUM_THREADS='1', MKL_NUM_THREADS='1',
+                      OPENBLAS_NUM_THREADS='1')
+    t0 = time.perf_counter()
+    with mp.get_context('spawn').Pool(7) as pool:
+        ah = pool.map_async(_exact_j_fit, jobs)
+        asm = pool.map_async(_exact_j_fit, jobs_s)
+        fits = dict(oracle7j=ah.get(), small7j=asm.get())
+    print(f'exact-Jacobian fits done in {time.perf_counter() - t0:.0f} s')
+    for key, rows in fits.items():
+        for f in ('values', 'uncertainties', 'chi2', 'nfev', 'success'):
+            out[f'{key}_{f}'] = np.array([r[f] for r in rows])
+        meta[f'{key}_seconds'] = [r['seconds'] for r in rows]
+    np.savez_compressed(path, **out)
+    with open(os.path.join(HERE, 'headline.json'), 'w') as f:
+        json.dump(meta, f, indent=1)
+
+
 def main():
     from oracle import lorenzmie as lm
     out, meta = {}, dict(instrument=INSTRUMENT, tolerances=TIGHT,
@@ -206,4 +264,7 @@ def main():
 
 
 if __name__ == '__main__':
-    main()
+    if '--exact-jacobian' in sys.argv[1:]:
+        exact_jacobian()
+    else:
+        main()

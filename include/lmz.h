@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define LMZ_VERSION 103
+#define LMZ_VERSION 104
 #define LMZ_NPARAM 8       /* stride of params rows */
 #define LMZ_NJAC 7         /* x_p y_p z_p a_p n_p k_p alpha */
 #define LMZ_NRED 36        /* 28 + 7 + 1 */
@@ -322,7 +322,27 @@ typedef struct lmz_lm_settings {
                               with steps fd_step |p_j| (fd_step if p_j = 0),
                               MINPACK lmdif's rule for scipy's diff_step
                               (analysis/Optimizer.py:111) */
+    int32_t damping;       /* LMZ_DAMPING_MINPACK or LMZ_DAMPING_NIELSEN */
+    int32_t reserved;      /* 0 */
 } lmz_lm_settings;
+
+/* Damping rules of lmz_lm_settings.damping.
+ * LMZ_DAMPING_MINPACK: the step rule of MINPACK lmder/lmdif, which scipy
+ *   least_squares(method='lm') runs for Optimizer.optimize
+ *   (analysis/Optimizer.py:221): a trust region of radius delta in the
+ *   scaled norm ||D dp|| (D from the running maximum of the Jacobian's
+ *   column norms, x_scale='jac'), delta = 100 ||D p0|| at the start, the
+ *   Levenberg parameter found by lmpar's Newton iteration, radius updates
+ *   from the ratio of actual to predicted reduction, steps accepted when
+ *   that ratio is >= 1e-4; ftol, xtol and gtol as MINPACK tests them
+ *   (relative chi2 reductions <= ftol, delta <= xtol ||D p||).  lambda0 is
+ *   not used.
+ * LMZ_DAMPING_NIELSEN: Marquardt damping lambda diag(D^2) from lambda0,
+ *   Nielsen's update on accepted steps and doubling growth on rejections;
+ *   ftol on the chi2 reductions of accepted steps, xtol on ||D dp|| <=
+ *   xtol (||D p|| + xtol). */
+#define LMZ_DAMPING_MINPACK 0
+#define LMZ_DAMPING_NIELSEN 1
 
 /* per-feature LM result, double [nfeat][LMZ_NRES] */
 #define LMZ_NRES 40

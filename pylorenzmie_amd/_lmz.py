@@ -36,6 +36,7 @@ NP_DTYPE_CODE = {np.dtype(np.uint8): U8, np.dtype(np.uint16): U16,
 
 LM_RUNNING, LM_FTOL, LM_XTOL, LM_GTOL = 0, 1, 2, 4
 LM_MAXITER, LM_STALLED, LM_NONFINITE = 8, 16, 32
+DAMPING_MINPACK, DAMPING_NIELSEN = 0, 1
 
 
 class LMZError(RuntimeError):
@@ -92,7 +93,9 @@ class LMSettings(ctypes.Structure):
                 ('lambda0', ctypes.c_double),
                 ('vary', ctypes.c_uint8 * NJAC),
                 ('use_graph', ctypes.c_uint8),
-                ('fd_step', ctypes.c_double)]
+                ('fd_step', ctypes.c_double),
+                ('damping', ctypes.c_int32),
+                ('reserved', ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p

@@ -27,6 +27,7 @@ class LMSettings:
     check_every: int = 4
     use_graph: bool = True
     fd_step: float = 0.0      # > 0: forward-difference Jacobian (transparency)
+    damping: str = 'minpack'  # MINPACK's trust region, or 'nielsen'
 
 
 def stats_from_jtj(jtj_upper: np.ndarray, chi2: float, npix: int,
@@ -195,6 +196,10 @@ class LMBatch:
             s.vary[i] = 1 if (PARAMS[i] in vary) else 0
         s.use_graph = 1 if settings.use_graph else 0
         s.fd_step = float(settings.fd_step)
+        rules = {'minpack': _lmz.DAMPING_MINPACK, 'nielsen': _lmz.DAMPING_NIELSEN}
+        if settings.damping not in rules:
+            raise ValueError(f"damping must be one of {sorted(rules)}")
+        s.damping = rules[settings.damping]
         return s
 
     def load(self, params0: np.ndarray) -> None:

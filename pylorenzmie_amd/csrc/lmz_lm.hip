@@ -7,33 +7,20 @@
 // Jacobian, trial forward pass and normal-equation reductions stay on the
 // device.  Per iteration and feature:
 //   J^T J, J^T r   from the forward-mode Jacobian (skipped after a rejection)
-//   solve          (J^T J + lambda D^2) dp = -J^T r, FP64 Cholesky
+//   solve          (J^T J + par D^2) dp = -J^T r, FP64 Cholesky, par from
+//                  MINPACK's trust-region rule (or Marquardt-Nielsen)
 //   trial          a_n, b_n and chi2 at p + dp
-//   accept/reject  Nielsen's damping update; ftol / xtol / gtol tests
+//   accept/reject  radius / damping update; ftol / xtol / gtol tests
+// The per-feature step logic is in lmz_lmstep.h.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
 #include <vector>
 
 #include "lmz_device.h"
+#include "lmz_lmstep.h"
 
 namespace lmz {
-
-struct LMState {
-    double jtj[28];
-    double g[7];
-    double dscale[7];
-    double chi2;         // at the current parameters
-    double lambda;
-    double nu;
-    double pred;         // predicted chi2 reduction of the trial step
-    double dnorm;        // ||D dp||
-    double pnorm;        // ||D p||
-    int32_t iter;
-    int32_t status;
-    int32_t need_j;
-    int32_t njev;
-};
 
 struct LMLayout {
     size_t coef, dcoef, nmax, sel, work, partials, chi2_tiles, state, ptrial,
@@ -72,10 +59,6 @@ static LMLayout lm_layout(const lmz_batch *b) {
     return L;
 }
 
-__device__ __forceinline__ int tri(int i, int j) {  // i <= j, row-major upper
-    return i * LMZ_NJAC - (i * (i - 1)) / 2 + (j - i);
-}
-
 __global__ __launch_bounds__(64) void k_lm_init(
     int nfeat, LMState *__restrict__ st, const double *__restrict__ params,
     double *__restrict__ ptrial, int32_t *__restrict__ sel,
@@ -84,16 +67,7 @@ __global__ __launch_bounds__(64) void k_lm_init(
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nfeat) return;
     LMState s;
-    for (int i = 0; i < 28; ++i) s.jtj[i] = 0.0;
-    for (int i = 0; i < 7; ++i) { s.g[i] = 0.0; s.dscale[i] = 0.0; }
-    s.chi2 = 0.0;
-    s.lambda = lambda0;
-    s.nu = 2.0;
-    s.pred = s.dnorm = s.pnorm = 0.0;
-    s.iter = 0;
-    s.status = LMZ_LM_RUNNING;
-    s.need_j = 1;
-    s.njev = 0;
+    lm_init(s, lambda0);
     st[f] = s;
     for (int i = 0; i < LMZ_NPARAM; ++i)
         ptrial[(long long)LMZ_NPARAM * f + i] = params[(long long)LMZ_NPARAM * f + i];
@@ -152,105 +126,26 @@ __global__ __launch_bounds__(64) void k_lm_solve(
     if (!mask_f[f]) return;
     LMState &S = st[f];
     const lmz_feature &F = feat[f];
-    if (mask_j[f]) {
+    const double *p = params + (long long)LMZ_NPARAM * f;
+    const bool fresh = mask_j[f];
+    if (fresh) {
         double red[LMZ_NRED];
         feature_tile_sum<LMZ_NRED>(F, partials, red);
-        if (threadIdx.x == 0) {
-            for (int i = 0; i < 28; ++i) S.jtj[i] = red[i];
-            for (int i = 0; i < 7; ++i) S.g[i] = red[28 + i];
-            S.njev += 1;
-            for (int i = 0; i < 7; ++i)
-                S.dscale[i] = fmax(S.dscale[i], sqrt(fmax(S.jtj[tri(i, i)], 0.0)));
-        }
+        if (threadIdx.x == 0) lm_take_jacobian(S, red, set, p);
     }
     if (threadIdx.x != 0) return;
-    // free parameters
-    int idx[7], nv = 0;
-    for (int i = 0; i < 7; ++i)
-        if (set.vary[i]) idx[nv++] = i;
-    const double *p = params + (long long)LMZ_NPARAM * f;
-    double *pt = ptrial + (long long)LMZ_NPARAM * f;
     // gradient test (analysis/Optimizer.py:108 gtol), on a fresh Jacobian
-    if (mask_j[f] && set.gtol > 0.0) {
-        double gmax = 0.0;
-        const double rn = sqrt(fmax(S.chi2, 0.0));
-        for (int a = 0; a < nv; ++a) {
-            const int i = idx[a];
-            const double jn = sqrt(fmax(S.jtj[tri(i, i)], 0.0));
-            if (jn > 0.0 && rn > 0.0) gmax = fmax(gmax, fabs(S.g[i]) / (jn * rn));
-        }
-        if (gmax <= set.gtol) {
-            S.status |= LMZ_LM_GTOL;
-            mask_f[f] = 0;
-            mask_j[f] = 0;
-            return;
-        }
-    }
-    double delta[7] = {0, 0, 0, 0, 0, 0, 0};
-    bool ok = false;
-    for (int attempt = 0; attempt < 12 && !ok; ++attempt) {
-        double A[7][7], bvec[7];
-        for (int a = 0; a < nv; ++a) {
-            for (int c = 0; c < nv; ++c) {
-                const int i = idx[a], j = idx[c];
-                A[a][c] = (i <= j) ? S.jtj[tri(i, j)] : S.jtj[tri(j, i)];
-            }
-            const double d = S.dscale[idx[a]];
-            A[a][a] += S.lambda * (d > 0.0 ? d * d : 1.0);
-            bvec[a] = -S.g[idx[a]];
-        }
-        // Cholesky A = L L^T in place (lower)
-        ok = true;
-        for (int j = 0; j < nv && ok; ++j) {
-            double s = A[j][j];
-            for (int k = 0; k < j; ++k) s -= A[j][k] * A[j][k];
-            if (!(s > 0.0) || !isfinite(s)) { ok = false; break; }
-            const double ljj = sqrt(s);
-            A[j][j] = ljj;
-            for (int i = j + 1; i < nv; ++i) {
-                double t = A[i][j];
-                for (int k = 0; k < j; ++k) t -= A[i][k] * A[j][k];
-                A[i][j] = t / ljj;
-            }
-        }
-        if (!ok) { S.lambda *= 10.0; continue; }
-        double y[7];
-        for (int i = 0; i < nv; ++i) {
-            double t = bvec[i];
-            for (int k = 0; k < i; ++k) t -= A[i][k] * y[k];
-            y[i] = t / A[i][i];
-        }
-        for (int i = nv - 1; i >= 0; --i) {
-            double t = y[i];
-            for (int k = i + 1; k < nv; ++k) t -= A[k][i] * delta[idx[k]];
-            delta[idx[i]] = t / A[i][i];
-        }
-    }
-    if (!ok) {
-        S.status |= LMZ_LM_STALLED;
+    if (fresh && set.gtol > 0.0 && lm_gtol_met(S, set)) {
+        S.status |= LMZ_LM_GTOL;
         mask_f[f] = 0;
         mask_j[f] = 0;
         return;
     }
-    // predicted reduction of chi2 = r^T r under the linear model
-    double gd = 0.0, dhd = 0.0, dn = 0.0, pn = 0.0;
-    for (int a = 0; a < nv; ++a) {
-        const int i = idx[a];
-        gd += S.g[i] * delta[i];
-        double hd = 0.0;
-        for (int c = 0; c < nv; ++c) {
-            const int j = idx[c];
-            hd += ((i <= j) ? S.jtj[tri(i, j)] : S.jtj[tri(j, i)]) * delta[j];
-        }
-        dhd += delta[i] * hd;
-        const double d = S.dscale[i];
-        dn += d * d * delta[i] * delta[i];
-        pn += d * d * p[i] * p[i];
+    if (!lm_solve(S, set, p, ptrial + (long long)LMZ_NPARAM * f)) {
+        S.status |= LMZ_LM_STALLED;
+        mask_f[f] = 0;
+        mask_j[f] = 0;
     }
-    S.pred = -2.0 * gd - dhd;
-    S.dnorm = sqrt(dn);
-    S.pnorm = sqrt(pn);
-    for (int i = 0; i < LMZ_NPARAM; ++i) pt[i] = p[i] + (i < 7 ? delta[i] : 0.0);
 }
 
 // Compare the trial chi2 with the current one; update damping and status.
@@ -266,34 +161,12 @@ __global__ __launch_bounds__(64) void k_lm_accept(
     feature_tile_sum<1>(feat[f], chi2_tiles, c);
     if (threadIdx.x != 0) return;
     LMState &S = st[f];
-    const double chi2t = c[0];
-    const double chi2 = S.chi2;
-    S.iter += 1;
-    const double act = chi2 - chi2t;
-    if (isfinite(chi2t) && act > 0.0) {
-        const double rho = (S.pred > 0.0) ? act / S.pred : 0.0;
+    const double *pt = ptrial + (long long)LMZ_NPARAM * f;
+    if (lm_accept(S, set, c[0], pt)) {
         for (int i = 0; i < LMZ_NPARAM; ++i)
-            params[(long long)LMZ_NPARAM * f + i] = ptrial[(long long)LMZ_NPARAM * f + i];
+            params[(long long)LMZ_NPARAM * f + i] = pt[i];
         sel[f] ^= 1;
-        S.chi2 = chi2t;
-        const double t = 2.0 * rho - 1.0;
-        S.lambda *= fmax(1.0 / 3.0, 1.0 - t * t * t);
-        S.nu = 2.0;
-        S.need_j = 1;
-        if (act <= set.ftol * chi2 && S.pred <= set.ftol * chi2)
-            S.status |= LMZ_LM_FTOL;
-        if (S.dnorm <= set.xtol * (S.pnorm + set.xtol)) S.status |= LMZ_LM_XTOL;
-    } else {
-        S.lambda *= S.nu;
-        S.nu *= 2.0;
-        S.need_j = 0;
-        // nothing left to gain at this resolution of chi2
-        if (isfinite(chi2t) && S.pred <= set.ftol * chi2) S.status |= LMZ_LM_FTOL;
-        if (S.dnorm <= set.xtol * (S.pnorm + set.xtol)) S.status |= LMZ_LM_XTOL;
-        if (S.lambda > 1e16) S.status |= LMZ_LM_STALLED;
     }
-    if (S.status == LMZ_LM_RUNNING && S.iter >= set.max_iter)
-        S.status |= LMZ_LM_MAXITER;
     const bool running = (S.status == LMZ_LM_RUNNING);
     mask_f[f] = running ? 1 : 0;
     mask_j[f] = (running && S.need_j) ? 1 : 0;
@@ -502,7 +375,11 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
     LMZ_REQUIRE(set.check_every >= 1, "check_every must be >= 1");
     LMZ_REQUIRE(set.ftol >= 0.0 && set.xtol >= 0.0 && set.gtol >= 0.0,
                 "tolerances must be >= 0");
-    LMZ_REQUIRE(set.lambda0 > 0.0, "lambda0 must be > 0");
+    LMZ_REQUIRE(set.damping == LMZ_DAMPING_MINPACK ||
+                    set.damping == LMZ_DAMPING_NIELSEN,
+                "damping must be LMZ_DAMPING_MINPACK or LMZ_DAMPING_NIELSEN");
+    LMZ_REQUIRE(set.damping != LMZ_DAMPING_NIELSEN || set.lambda0 > 0.0,
+                "lambda0 must be > 0");
     LMZ_REQUIRE(set.fd_step >= 0.0 && set.fd_step < 1.0,
                 "fd_step must be in [0, 1): 0 for the analytic Jacobian");
     LMZ_REQUIRE(!(flags & LMZ_FP32),

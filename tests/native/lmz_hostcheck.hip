@@ -1,5 +1,6 @@
 // Host build of the per-pixel and per-feature arithmetic in
-// pylorenzmie_amd/csrc/lmz_pixel.h and the frame algorithms in
+// pylorenzmie_amd/csrc/lmz_pixel.h, the LM step logic in
+// pylorenzmie_amd/csrc/lmz_lmstep.h and the frame algorithms in
 // pylorenzmie_amd/csrc/lmz_frame.h, for CPU tests of the kernel formulas
 // against the FP64 oracle, numpy and scipy.  Test instrument only: the library's product
 // path is the gfx950 kernels in liblmz.so.
@@ -8,6 +9,7 @@
 #include <vector>
 
 #include "../../pylorenzmie_amd/csrc/lmz_frame.h"
+#include "../../pylorenzmie_amd/csrc/lmz_lmstep.h"
 #include "../../pylorenzmie_amd/csrc/lmz_pixel.h"
 
 using namespace lmz;
@@ -75,6 +77,42 @@ void chk_holo32(const lmz_feature *feat, const double *params,
                        bohren, g);
         holo[j] = intensity_f(g, forward_sum_f(g, coef, nmax), fu.alpha);
     }
+}
+
+// The LM's per-feature step logic (lmz_lmstep.h) as k_lm_solve and
+// k_lm_accept run it, on an opaque LMState owned by the caller.
+int chk_lm_state_bytes(void) { return (int)sizeof(LMState); }
+void chk_lm_init(void *st, double lambda0) {
+    lm_init(*static_cast<LMState *>(st), lambda0);
+}
+void chk_lm_set_chi2(void *st, double chi2) {
+    static_cast<LMState *>(st)->chi2 = chi2;
+}
+// A fresh Jacobian's reduction (J^T J upper triangle, J^T r, r^T r) at p;
+// returns 1 when the gradient test stops the fit (status GTOL).
+int chk_lm_fresh(void *st, const double *red, const lmz_lm_settings *set,
+                 const double *p) {
+    LMState &S = *static_cast<LMState *>(st);
+    lm_take_jacobian(S, red, *set, p);
+    if (set->gtol > 0.0 && lm_gtol_met(S, *set)) {
+        S.status |= LMZ_LM_GTOL;
+        return 1;
+    }
+    return 0;
+}
+int chk_lm_solve(void *st, const lmz_lm_settings *set, const double *p,
+                 double *pt) {
+    return lm_solve(*static_cast<LMState *>(st), *set, p, pt) ? 1 : 0;
+}
+int chk_lm_accept(void *st, const lmz_lm_settings *set, double chi2t,
+                  const double *pt) {
+    return lm_accept(*static_cast<LMState *>(st), *set, chi2t, pt) ? 1 : 0;
+}
+// chi2, lambda (par), delta, status, iter, njev, need_j
+void chk_lm_read(const void *st, double *out) {
+    const LMState &S = *static_cast<const LMState *>(st);
+    out[0] = S.chi2; out[1] = S.lambda; out[2] = S.delta;
+    out[3] = S.status; out[4] = S.iter; out[5] = S.njev; out[6] = S.need_j;
 }
 
 // Dense-grid pixel coordinates as the kernels derive them.

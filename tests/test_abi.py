@@ -33,14 +33,14 @@ def test_exports_every_declared_symbol(lib):
 
 
 def test_version(lib):
-    assert lib.lmz_version() == 103
+    assert lib.lmz_version() == 104
 
 
 def test_struct_layouts_match_header():
     from pylorenzmie_amd import _lmz
     assert ctypes.sizeof(_lmz.Feature) == 88
     assert ctypes.sizeof(_lmz.BatchSizes) == 48
-    assert ctypes.sizeof(_lmz.LMSettings) == 56
+    assert ctypes.sizeof(_lmz.LMSettings) == 64
     assert ctypes.sizeof(_lmz.Schedule) == 16
 
 

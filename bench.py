@@ -21,15 +21,22 @@ Also measured:
           timed with HIP events on the stream they run on, against the FP32
           VALU peak, with algorithmic FLOP counts per pixel of 296 N + 480
           (Jacobian) and 64 N + 70 (forward), N the order count;
-  cpu_baseline  the FP64 oracle (numpy model + scipy MINPACK LM) on one
-          feature on the host (--cpu-workers N: also on N host cores).
+  cpu_baseline  the FP64 oracle (numpy model + scipy MINPACK LM) on the
+          host's cores (one process per core, two features each, at most
+          16 cores), started as a child process that never touches the
+          GPU; cpu_baseline_1core: the same fits one after another on one
+          core.
+  fit_agreement  the timed fits of the first 32 features against the
+          FP64 oracle's fits of the same data (tests/golden/headline.npz,
+          whose data replace those 32 synthetic crops).
 
---extra superposed | frame | de | batch measures a path around the fitter
-instead:
+--extra superposed | frame | de | batch | normalize measures a path around
+the fitter instead:
   the five-sphere hologram of the reference's MultiSphere example, a
   frame's features fitted as one batch from device memory, the DE
-  estimator's GPU population, and LorenzMieBatch.batch_hologram of 256
-  particle lists (one JSON line each).
+  estimator's GPU population, LorenzMieBatch.batch_hologram of 256
+  particle lists, and the Normalizer's three methods on 1280 x 1024 frames
+  of four pixel types (one JSON line each).
 
 --config 1..4 measures the other configurations instead (one JSON line):
   1  batched forward holograms (1024 crops, side U{64..400}): holograms/s,
@@ -224,7 +231,62 @@ def kernel_times(lm, settings, args):
     return tj, tf, tn, flop_j, flop_f, float(np.sum(P)), float(np.mean(nm))
 
 
-def fit_agreement(lm, params, res, p0):
+HEADLINE_FIXTURE = os.path.join(ROOT, 'tests', 'golden', 'headline.npz')
+
+
+def load_parity_sample(args, truth, start, data_dev):
+    """The oracle's fits of the first 32 headline features
+    (tests/golden/headline.npz: the same draws, FP64 holograms plus seeded
+    noise, fitted by the FP64 MINPACK oracle to 1e-12): their data replace
+    the first 32 synthetic crops, so that the timed run fits them.  None
+    for other workloads."""
+    import torch
+    if not (args.side == 201 and args.seed == 3 and args.sigma == 0.05 and
+            args.nfeat >= 32 and os.path.exists(HEADLINE_FIXTURE)):
+        return None
+    fx = np.load(HEADLINE_FIXTURE)
+    n = len(fx['head_truth'])
+    if not (np.array_equal(fx['head_truth'], truth[:n]) and
+            np.array_equal(fx['head_start'], start[:n])):
+        return None
+    px = args.side * args.side
+    data_dev[: n * px] = torch.from_numpy(
+        fx['head_data'].reshape(-1)).to(data_dev.device)
+    return fx
+
+
+def oracle_parity(fx, params, res):
+    """The timed fits of the fixture's features against the oracle's: the
+    same minimum where chi2 agree to 1e-7 relative (parameters then within
+    1e-4 relative, k_p 1e-5 absolute), and whether the device's chi2 is
+    higher than the oracle's by more than 1e-9 relative."""
+    n = len(fx['oracle7_chi2'])
+    ref_p, ref_c = fx['oracle7_values'], fx['oracle7_chi2']
+    chi2 = res[:n, 35]
+    rel_c = (chi2 - ref_c) / ref_c
+    d = np.abs(params[:n, :7] - ref_p)
+    rel = d / np.abs(ref_p)
+    rel[:, 5] = d[:, 5]
+    same = np.abs(rel_c) <= 1e-7
+    ok = (np.delete(rel, 5, axis=1).max(1) <= 1e-4) & (rel[:, 5] <= 1e-5)
+    capped = (res[:n, 38].astype(int) & 8) != 0
+    return {'reference': 'FP64 oracle (numpy model + scipy MINPACK, 1e-12 '
+                         'tolerances) fits of the first 32 features, '
+                         'tests/golden/headline.npz',
+            'features': int(n),
+            'same_minimum': int(same.sum()),
+            'same_minimum_within_1e-4': int((same & ok).sum()),
+            'higher_chi2': int((rel_c > 1e-9).sum()),
+            'lower_chi2': int((rel_c < -1e-7).sum()),
+            'capped_at_50': int(capped.sum()),
+            'max_param_rel_diff_same_minimum': (float(np.delete(rel[same], 5, axis=1).max())
+                                                if same.any() else None),
+            'max_kp_abs_diff_same_minimum': (float(rel[same, 5].max())
+                                             if same.any() else None),
+            'chi2_rel_diff': [float(v) for v in rel_c]}
+
+
+def convergence_check(lm, params, res, p0):
     """Distance of the timed fits from the minimum each one is heading for:
     the same LM rerun from the same starts to 1e-13 tolerances and 200
     iterations (not timed).  Features capped in either run are left out.
@@ -240,8 +302,8 @@ def fit_agreement(lm, params, res, p0):
     rel[:, 5] = np.abs(params[:, 5] - pref[:, 5])
     rel = rel[keep]
     ok = (np.delete(rel, 5, axis=1).max(1) <= 1e-4) & (rel[:, 5] <= 1e-5)
-    return {'reference': 'same LM from the same starts to 1e-13 tolerances, '
-                         '<=200 iterations',
+    return {'reference': 'the same LM from the same starts to 1e-13 '
+                         'tolerances, <=200 iterations (self-consistency)',
             'features_compared': int(keep.sum()),
             'frac_within_1e-4_rel_and_kp_1e-5_abs': float(ok.mean()),
             'p99': {k: float(np.percentile(rel[:, i], 99))
@@ -250,16 +312,37 @@ def fit_agreement(lm, params, res, p0):
                        for i, k in enumerate(PARAMS)}}
 
 
+def kernel_source_hash() -> str:
+    """First 16 hex digits of the SHA-256 of the library's sources (the
+    HIP kernels, their headers and include/lmz.h), in name order."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(ROOT, 'pylorenzmie_amd', 'csrc', 'lmz_*')))
+    files.append(os.path.join(ROOT, 'include', 'lmz.h'))
+    for path in files:
+        h.update(os.path.basename(path).encode())
+        with open(path, 'rb') as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def committed_figure(name, key, nfeat, side):
     """A per-launch figure of a kernel from a PMC summary committed under
     profiles/ for this workload, or None for other workloads: HBM bytes
     (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel) or counted
-    FP32 FLOP (SQ_INSTS_VALU_FLOPS_FP32)."""
+    FP32 FLOP (SQ_INSTS_VALU_FLOPS_FP32).  The summary names the source
+    hash (kernel_source_hash) of the build it measured; a figure measured
+    on other sources is None, never a stale number."""
     import re
     path = os.path.join(ROOT, 'profiles', f'{name}_{nfeat}x{side}x{side}.md')
     if not os.path.exists(path):
         return None
-    m = re.search(key + r'\s*=\s*([0-9.eE+]+)', open(path).read())
+    text = open(path).read()
+    stamp = re.search(r'source_sha256_16\s*=\s*([0-9a-f]{16})', text)
+    if not stamp or stamp.group(1) != kernel_source_hash():
+        return None
+    m = re.search(key + r'\s*=\s*([0-9.eE+]+)', text)
     return float(m.group(1)) if m else None
 
 
@@ -855,9 +938,81 @@ def run_extra(args):
                     max_abs_err_vs_oracle=err,
                     cpu_baseline={'value': 1. / tc, 'unit': 'holograms/s',
                                   'cores': 1, 'kind': 'port'})
+    elif args.extra == 'normalize':
+        # utilities/normalizer.py:11-76: Normalizer('median'), ('filter',
+        # size 51) and ('reference') of a 1280 x 1024 camera frame already
+        # in device memory, as 8-bit, 16-bit, float32 and float64 pixels
+        from pylorenzmie_amd.utilities import Normalizer
+        ny, nx, size, reps = 1024, 1280, 51, 20
+        raw = _synthetic_frame(ny, nx)
+        frames = {'u8': raw, 'u16': raw.astype(np.uint16),
+                  'f32': raw.astype(np.float32) / np.float32(100.),
+                  'f64': raw.astype(np.float64) / 100.}
+        background = torch.from_numpy(_synthetic_frame(ny, nx, seed=1)).cuda()
+        cases = []
+        for name, arr in frames.items():
+            d = torch.from_numpy(arr).cuda()
+            for method, norm in (('median', Normalizer('median')),
+                                 ('filter', Normalizer('filter', size=size)),
+                                 ('reference', Normalizer('reference',
+                                                          reference=background))):
+                ms = _timed(lambda: norm(d), reps) * 1e3
+                cases.append(dict(dtype=name, method=method,
+                                  ms_per_frame=round(ms, 4),
+                                  mpix_per_s=round(ny * nx / ms / 1e3, 1)))
+        line.update(workload=f'frame normalization, {nx} x {ny} frames in '
+                             f'device memory (median filter window {size})',
+                    cases=cases)
+    elif args.extra == 'valu_peak':
+        # the attainable FP32 and FP64 FMA rates of code built with the
+        # library's flags: independent FMA chains in registers
+        import ctypes
+        lib = ctypes.CDLL(os.path.join(ROOT, 'tests', 'native',
+                                       'liblmz_valu_peak.so'))
+        lib.valu_peak.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_void_p,
+                                  ctypes.POINTER(ctypes.c_float)]
+        props = torch.cuda.get_device_properties(0)
+        blocks, iters = props.multi_processor_count * 8, 4096
+        chains, nt = lib.valu_peak_chains(), lib.valu_peak_threads()
+        rates = {}
+        ops = (('fma_uniform', 2.), ('fma_lane', 2.), ('mul', 1.), ('add', 1.))
+        for dtype, name, tdt in ((0, 'fp32', torch.float32), (1, 'fp64', torch.float64)):
+            out = torch.empty(blocks * nt, dtype=tdt, device='cuda')
+            for op, (opname, fl) in enumerate(ops):
+                best = None
+                for _ in range(5):
+                    ms = ctypes.c_float()
+                    if lib.valu_peak(dtype, op, blocks, iters,
+                                     ctypes.c_void_p(out.data_ptr()), ctypes.byref(ms)):
+                        raise RuntimeError('valu_peak launch failed')
+                    best = ms.value if best is None else min(best, ms.value)
+                inst = float(blocks * nt) * iters * chains
+                rates[f'{name}_{opname}'] = {
+                    'lane_instructions_per_s': inst / (best * 1e-3),
+                    'tflops': fl * inst / (best * 1e-3) / 1e12, 'ms': best}
+        line.update(workload=f'{blocks} workgroups of {nt} threads, {chains} '
+                             f'independent FMA chains x {iters} per thread',
+                    cus=props.multi_processor_count, rates=rates,
+                    peak_published={'fp32': PEAK_FP32, 'fp32_unpacked': PEAK_FP32_UNPACKED,
+                                    'fp64': 78.6})
     else:
-        raise SystemExit('--extra must be superposed, frame, de or batch')
+        raise SystemExit('--extra must be superposed, frame, de, batch, '
+                         'normalize or valu_peak')
     print(json.dumps(line), flush=True)
+
+
+def _synthetic_frame(ny: int, nx: int, seed: int = 0) -> np.ndarray:
+    """An 8-bit camera frame: fringes of two features on a sloped
+    background, with shot noise."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[:ny, :nx].astype(float)
+    v = 90. + 20. * x / nx + 10. * y / ny
+    for cx, cy, k in ((0.3 * nx, 0.5 * ny, 0.35), (0.7 * nx, 0.3 * ny, 0.25)):
+        r = np.hypot(x - cx, y - cy)
+        v += 40. * np.cos(k * r) * np.exp(-r / 150.)
+    v += rng.normal(0., 3., v.shape)
+    return np.clip(v, 0, 255).astype(np.uint8)
 
 
 def _lmz_cap(a_p, n_p):
@@ -876,12 +1031,13 @@ def main():
     ap.add_argument('--ftol', type=float, default=1e-9)
     ap.add_argument('--xtol', type=float, default=1e-9)
     ap.add_argument('--gtol', type=float, default=1e-10)
-    ap.add_argument('--fd-sample', type=int, default=8)
+    ap.add_argument('--fd-sample', type=int, default=32)
     ap.add_argument('--cpu-seconds', type=float, default=15.)
     ap.add_argument('--cpu-max-fits', type=int, default=4)
-    ap.add_argument('--cpu-workers', type=int, default=0,
-                    help='also time the oracle on this many host cores '
-                         '(one process each, 2 features each; 0: skip)')
+    ap.add_argument('--cpu-workers', type=int, default=-1,
+                    help='time the oracle on this many host cores (one '
+                         'process each, 2 features each; -1: the CPUs this '
+                         'process may use, at most 16; 0: skip)')
     ap.add_argument('--kernel-reps', type=int, default=3)
     ap.add_argument('--calibrate-copy', type=int, default=0, metavar='MIB',
                     help='also run one device copy of this many MiB (PMC '
@@ -894,7 +1050,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--seed', type=int, default=3)
     ap.add_argument('--extra', default='',
-                    help='superposed | frame | de | batch: measure that path instead '
+                    help='superposed | frame | de | batch | normalize | valu_peak: measure that path instead '
                          'of the headline (one JSON line)')
     ap.add_argument('--config', type=int, default=0,
                     help='1-4: measure that configuration instead of the '
@@ -925,6 +1081,7 @@ def main():
     truth, start = draw_features(args.nfeat, args.side, args.seed)
     log(f'bench: generating {args.nfeat} synthetic {args.side}^2 holograms')
     data_dev = make_data(truth, args.side, args.sigma, args.seed)
+    parity_fx = load_parity_sample(args, truth, start, data_dev)
     torch.cuda.synchronize()
     log('bench: batched LM warmup + timed steps')
     lm, dt, params, res, settings, p0 = run_batched(args, truth, start, data_dev)
@@ -945,8 +1102,12 @@ def main():
     if args.dump:
         np.savez_compressed(args.dump, truth=truth, start=start, params=params,
                             result=res)
+    parity = oracle_parity(parity_fx, params, res) if parity_fx is not None else None
+    if parity:
+        log(f"bench: oracle parity: same minimum {parity['same_minimum']}/"
+            f"{parity['features']}, higher chi2 {parity['higher_chi2']}")
     log('bench: agreement with fully converged fits')
-    agree = fit_agreement(lm, params, res, p0)
+    agree = convergence_check(lm, params, res, p0)
     log(f"bench: {agree['frac_within_1e-4_rel_and_kp_1e-5_abs']:.4f} of "
         f"{agree['features_compared']} fits within 1e-4 of the converged minimum")
     log('bench: kernel timing')
@@ -1006,7 +1167,11 @@ def main():
                           f'differences), {t:.1f} s')
         log(f'bench: CPU oracle {rate:.4f} fits/s')
     cpu_pool = None
-    if args.cpu_workers > 0:
+    workers = args.cpu_workers
+    if workers < 0:
+        workers = min(16, len(os.sched_getaffinity(0)))
+    if workers > 0 and not args.no_cpu:
+        args.cpu_workers = workers
         cpu_pool = run_cpu_pool(args, start, data_dev)
         log(f"bench: CPU oracle on {cpu_pool['cores']} cores "
             f"{cpu_pool['value']:.4f} fits/s")
@@ -1022,7 +1187,8 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f32 pixel terms, f64 sums/Mie/solve',
+        'dtype': 'f64 forward/residual/chi2, f32 Jacobian with f64 tile sums, '
+                 'f64 Mie and 7x7 solve',
         'data': 'synthetic (seeded config-3 parameter draws, sigma=0.05)',
         'config': {'workload': f'batched 7-parameter LM, {args.nfeat} features, '
                                f'{args.side}x{args.side} crops, <=50 iterations',
@@ -1059,13 +1225,14 @@ def main():
                         'converged_redchi_median': float(np.median(redchi[ok])),
                         'median_abs_err': {k: float(np.median(err[:, i]))
                                            for i, k in enumerate(PARAMS)}},
-        'fit_agreement': agree,
+        'fit_agreement': parity,
+        'convergence_check': agree,
         'fd_plain_gpu': fd,
         'speedup_vs_fd': (fits_per_s / fd['value']) if fd else None,
         'fd_batched': fdb,
         'speedup_vs_fd_batched': (fits_per_s / fdb['value']) if fdb else None,
-        'cpu_baseline': cpu,
-        'cpu_baseline_cores': cpu_pool,
+        'cpu_baseline': cpu_pool or cpu,
+        'cpu_baseline_1core': cpu,
     }
     print(json.dumps(line), flush=True)
 

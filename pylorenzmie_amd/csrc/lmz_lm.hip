@@ -124,28 +124,25 @@ __global__ __launch_bounds__(64) void k_lm_solve(
     uint8_t *__restrict__ mask_f, lmz_lm_settings set) {
     const int f = blockIdx.x;
     if (!mask_f[f]) return;
-    LMState &S = st[f];
     const lmz_feature &F = feat[f];
     const double *p = params + (long long)LMZ_NPARAM * f;
     const bool fresh = mask_j[f];
-    if (fresh) {
-        double red[LMZ_NRED];
-        feature_tile_sum<LMZ_NRED>(F, partials, red);
-        if (threadIdx.x == 0) lm_take_jacobian(S, red, set, p);
-    }
+    double red[LMZ_NRED];
+    if (fresh) feature_tile_sum<LMZ_NRED>(F, partials, red);
     if (threadIdx.x != 0) return;
+    LMState S = st[f];                  // registers; written back once
+    if (fresh) lm_take_jacobian(S, red, set, p);
     // gradient test (analysis/Optimizer.py:108 gtol), on a fresh Jacobian
     if (fresh && set.gtol > 0.0 && lm_gtol_met(S, set)) {
         S.status |= LMZ_LM_GTOL;
         mask_f[f] = 0;
         mask_j[f] = 0;
-        return;
-    }
-    if (!lm_solve(S, set, p, ptrial + (long long)LMZ_NPARAM * f)) {
+    } else if (!lm_solve(S, set, p, ptrial + (long long)LMZ_NPARAM * f)) {
         S.status |= LMZ_LM_STALLED;
         mask_f[f] = 0;
         mask_j[f] = 0;
     }
+    st[f] = S;
 }
 
 // Compare the trial chi2 with the current one; update damping and status.
@@ -160,13 +157,14 @@ __global__ __launch_bounds__(64) void k_lm_accept(
     double c[1];
     feature_tile_sum<1>(feat[f], chi2_tiles, c);
     if (threadIdx.x != 0) return;
-    LMState &S = st[f];
+    LMState S = st[f];
     const double *pt = ptrial + (long long)LMZ_NPARAM * f;
     if (lm_accept(S, set, c[0], pt)) {
         for (int i = 0; i < LMZ_NPARAM; ++i)
             params[(long long)LMZ_NPARAM * f + i] = pt[i];
         sel[f] ^= 1;
     }
+    st[f] = S;
     const bool running = (S.status == LMZ_LM_RUNNING);
     mask_f[f] = running ? 1 : 0;
     mask_j[f] = (running && S.need_j) ? 1 : 0;

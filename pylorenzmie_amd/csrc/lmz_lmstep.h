@@ -60,14 +60,6 @@ LMZ_HD double lm_jtj(const LMState &S, int i, int j) {
     return (i <= j) ? S.jtj[lm_tri(i, j)] : S.jtj[lm_tri(j, i)];
 }
 
-// Free parameters of the fit: their indices, and how many.
-LMZ_HD int lm_free(const lmz_lm_settings &set, int (&idx)[7]) {
-    int nv = 0;
-    for (int i = 0; i < 7; ++i)
-        if (set.vary[i]) idx[nv++] = i;
-    return nv;
-}
-
 LMZ_HD void lm_init(LMState &S, double lambda0) {
     for (int i = 0; i < 28; ++i) S.jtj[i] = 0.0;
     for (int i = 0; i < 7; ++i) { S.g[i] = 0.0; S.dscale[i] = 0.0; }
@@ -81,77 +73,108 @@ LMZ_HD void lm_init(LMState &S, double lambda0) {
     S.njev = 0;
 }
 
-// Cholesky factor L (lower) of A + par diag(d^2) over the free parameters
-// (d = 1 in place of zero columns' scales); false if it is not positive
-// definite in floating point.
-LMZ_HD bool lm_cholesky(const LMState &S, const int (&idx)[7], int nv,
-                        double par, double (&L)[7][7]) {
-    for (int a = 0; a < nv; ++a)
-        for (int c = 0; c <= a; ++c) L[a][c] = lm_jtj(S, idx[a], idx[c]);
-    for (int a = 0; a < nv; ++a) {
-        const double d = S.dscale[idx[a]];
-        L[a][a] += par * (d > 0.0 ? d * d : 1.0);
+// The normal equations over all seven parameters with the held ones
+// decoupled: their rows and columns of J^T J become the identity and their
+// J^T r entries 0, so every solve gives them a zero step and the free
+// parameters exactly the solution of the reduced system.  Fixed sizes keep
+// every array in registers once the loops are unrolled.
+struct LMSys {
+    double A[7][7];   // J^T J (free block), identity elsewhere
+    double g[7];      // J^T r, 0 for held parameters
+    double d[7];      // D (1 for held parameters and zero columns)
+};
+
+LMZ_HD void lm_system(const LMState &S, const lmz_lm_settings &set,
+                      LMSys &Y) {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        const bool fi = set.vary[i] != 0;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const bool fj = set.vary[j] != 0;
+            Y.A[i][j] = (fi && fj) ? lm_jtj(S, i, j) : (i == j ? 1.0 : 0.0);
+        }
+        Y.g[i] = fi ? S.g[i] : 0.0;
+        Y.d[i] = (fi && S.dscale[i] > 0.0) ? S.dscale[i] : 1.0;
     }
-    for (int j = 0; j < nv; ++j) {
+}
+
+// Cholesky factor L (lower) of A + par diag(d^2); false if it is not
+// positive definite in floating point.
+LMZ_HD bool lm_cholesky(const LMSys &Y, double par, double (&L)[7][7]) {
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) L[i][j] = Y.A[i][j];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) L[i][i] += par * (Y.d[i] * Y.d[i]);
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
         double s = L[j][j];
+#pragma unroll
         for (int k = 0; k < j; ++k) s -= L[j][k] * L[j][k];
-        if (!(s > 0.0) || !lm_finite(s)) return false;
-        const double ljj = sqrt(s);
+        ok = ok && (s > 0.0) && lm_finite(s);
+        const double ljj = sqrt(fmax(s, 0.0));
         L[j][j] = ljj;
-        for (int i = j + 1; i < nv; ++i) {
+#pragma unroll
+        for (int i = j + 1; i < 7; ++i) {
             double t = L[i][j];
+#pragma unroll
             for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
             L[i][j] = t / ljj;
         }
     }
-    return true;
+    return ok;
 }
 
 // y = L^-1 b
-LMZ_HD void lm_lower_solve(const double (&L)[7][7], int nv, const double *b,
-                           double *y) {
-    for (int i = 0; i < nv; ++i) {
+LMZ_HD void lm_lower_solve(const double (&L)[7][7], const double (&b)[7],
+                           double (&y)[7]) {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
         double t = b[i];
+#pragma unroll
         for (int k = 0; k < i; ++k) t -= L[i][k] * y[k];
         y[i] = t / L[i][i];
     }
 }
 
-// x = -(L L^T)^-1 g over the free parameters (x indexed like idx)
-LMZ_HD void lm_step(const double (&L)[7][7], int nv, const LMState &S,
-                    const int (&idx)[7], double *x) {
+// x = -(L L^T)^-1 g
+LMZ_HD void lm_step(const double (&L)[7][7], const LMSys &Y, double (&x)[7]) {
     double b[7], y[7];
-    for (int a = 0; a < nv; ++a) b[a] = -S.g[idx[a]];
-    lm_lower_solve(L, nv, b, y);
-    for (int i = nv - 1; i >= 0; --i) {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) b[i] = -Y.g[i];
+    lm_lower_solve(L, b, y);
+#pragma unroll
+    for (int i = 6; i >= 0; --i) {
         double t = y[i];
-        for (int k = i + 1; k < nv; ++k) t -= L[k][i] * x[k];
+#pragma unroll
+        for (int k = i + 1; k < 7; ++k) t -= L[k][i] * x[k];
         x[i] = t / L[i][i];
     }
 }
 
-LMZ_HD double lm_dnorm(const LMState &S, const int (&idx)[7], int nv,
-                       const double *x) {
+LMZ_HD double lm_dnorm(const LMSys &Y, const double (&x)[7]) {
     double s = 0.0;
-    for (int a = 0; a < nv; ++a) {
-        const double t = S.dscale[idx[a]] * x[a];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        const double t = Y.d[i] * x[i];
         s += t * t;
     }
     return sqrt(s);
 }
 
 // ||L^-1 D^2 x / ||D x|| ||: the lmpar Newton correction's denominator
-LMZ_HD double lm_newton_norm(const double (&L)[7][7], int nv,
-                             const LMState &S, const int (&idx)[7],
-                             const double *x, double dxnorm) {
+LMZ_HD double lm_newton_norm(const double (&L)[7][7], const LMSys &Y,
+                             const double (&x)[7], double dxnorm) {
     double w[7], y[7];
-    for (int a = 0; a < nv; ++a) {
-        const double d = S.dscale[idx[a]];
-        w[a] = d * d * x[a] / dxnorm;
-    }
-    lm_lower_solve(L, nv, w, y);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) w[i] = Y.d[i] * Y.d[i] * x[i] / dxnorm;
+    lm_lower_solve(L, w, y);
     double s = 0.0;
-    for (int a = 0; a < nv; ++a) s += y[a] * y[a];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) s += y[i] * y[i];
     return sqrt(s);
 }
 
@@ -161,16 +184,16 @@ LMZ_HD double lm_newton_norm(const double (&L)[7][7], int nv,
 // Newton iterations on ||D x(par)|| = delta between the bounds parl, paru).
 // par holds the previous parameter on entry (the starting guess).  Returns
 // false if no factorization succeeded.
-LMZ_HD bool lm_lmpar(const LMState &S, const int (&idx)[7], int nv,
-                     double delta, double &par, double *x) {
+LMZ_HD bool lm_lmpar(const LMSys &Y, double delta, double &par,
+                     double (&x)[7]) {
     const double dwarf = 2.2250738585072014e-308;
     double L[7][7];
     // Gauss-Newton direction
-    const bool nonsing = lm_cholesky(S, idx, nv, 0.0, L);
+    const bool nonsing = lm_cholesky(Y, 0.0, L);
     double dxnorm = INFINITY, fp = INFINITY;
     if (nonsing) {
-        lm_step(L, nv, S, idx, x);
-        dxnorm = lm_dnorm(S, idx, nv, x);
+        lm_step(L, Y, x);
+        dxnorm = lm_dnorm(Y, x);
         fp = dxnorm - delta;
         if (fp <= 0.1 * delta) {
             par = 0.0;
@@ -180,14 +203,14 @@ LMZ_HD bool lm_lmpar(const LMState &S, const int (&idx)[7], int nv,
     // lower bound from the Newton step at par = 0 (full rank only)
     double parl = 0.0;
     if (nonsing) {
-        const double t = lm_newton_norm(L, nv, S, idx, x, dxnorm);
+        const double t = lm_newton_norm(L, Y, x, dxnorm);
         parl = ((fp / delta) / t) / t;
     }
     // upper bound: ||D^-1 J^T r|| / delta
     double gn = 0.0;
-    for (int a = 0; a < nv; ++a) {
-        const double d = S.dscale[idx[a]];
-        const double t = S.g[idx[a]] / d;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        const double t = Y.g[i] / Y.d[i];
         gn += t * t;
     }
     gn = sqrt(gn);
@@ -200,21 +223,21 @@ LMZ_HD bool lm_lmpar(const LMState &S, const int (&idx)[7], int nv,
     for (int iter = 1; iter <= 10; ++iter) {
         if (par == 0.0) par = fmax(dwarf, 0.001 * paru);
         int grow = 0;
-        while (!lm_cholesky(S, idx, nv, par, L)) {
+        while (!lm_cholesky(Y, par, L)) {
             // J^T J + par D^2 is positive definite for par > 0; only
             // rounding at a tiny par can defeat the factorization
             if (++grow > 30) return ok;
             par = fmax(par * 10.0, 1e-300);
         }
-        lm_step(L, nv, S, idx, x);
+        lm_step(L, Y, x);
         ok = true;
-        dxnorm = lm_dnorm(S, idx, nv, x);
+        dxnorm = lm_dnorm(Y, x);
         const double temp = fp;
         fp = dxnorm - delta;
         if (fabs(fp) <= 0.1 * delta || (parl == 0.0 && fp <= temp && temp < 0.0) ||
             iter == 10)
             break;
-        const double t = lm_newton_norm(L, nv, S, idx, x, dxnorm);
+        const double t = lm_newton_norm(L, Y, x, dxnorm);
         const double parc = ((fp / delta) / t) / t;
         if (fp > 0.0) parl = fmax(parl, par);
         if (fp < 0.0) paru = fmin(paru, par);
@@ -240,11 +263,10 @@ LMZ_HD void lm_take_jacobian(LMState &S, const double *red,
             S.dscale[i] = fmax(S.dscale[i], cn);
     }
     if (set.damping == LMZ_DAMPING_MINPACK && first) {
-        int idx[7];
-        const int nv = lm_free(set, idx);
         double xs = 0.0;
-        for (int a = 0; a < nv; ++a) {
-            const double t = S.dscale[idx[a]] * p[idx[a]];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+            const double t = set.vary[i] ? S.dscale[i] * p[i] : 0.0;
             xs += t * t;
         }
         S.xnorm = sqrt(xs);
@@ -256,14 +278,13 @@ LMZ_HD void lm_take_jacobian(LMState &S, const double *red,
 // MINPACK's gradient test: max_j |(J^T r)_j| / (||J_j|| ||r||) <= gtol
 // over the free parameters with nonzero columns.
 LMZ_HD bool lm_gtol_met(const LMState &S, const lmz_lm_settings &set) {
-    int idx[7];
-    const int nv = lm_free(set, idx);
     double gmax = 0.0;
     const double rn = sqrt(fmax(S.chi2, 0.0));
-    for (int a = 0; a < nv; ++a) {
-        const int i = idx[a];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
         const double jn = sqrt(fmax(S.jtj[lm_tri(i, i)], 0.0));
-        if (jn > 0.0 && rn > 0.0) gmax = fmax(gmax, fabs(S.g[i]) / (jn * rn));
+        if (set.vary[i] && jn > 0.0 && rn > 0.0)
+            gmax = fmax(gmax, fabs(S.g[i]) / (jn * rn));
     }
     return gmax <= set.gtol;
 }
@@ -274,34 +295,35 @@ LMZ_HD bool lm_gtol_met(const LMState &S, const lmz_lm_settings &set) {
 // can be formed (the feature stalls).
 LMZ_HD bool lm_solve(LMState &S, const lmz_lm_settings &set, const double *p,
                      double *pt) {
-    int idx[7];
-    const int nv = lm_free(set, idx);
+    LMSys Y;
+    lm_system(S, set, Y);
     double x[7] = {0, 0, 0, 0, 0, 0, 0};
     if (set.damping == LMZ_DAMPING_MINPACK) {
         double par = S.lambda;
-        if (!lm_lmpar(S, idx, nv, S.delta, par, x)) return false;
+        if (!lm_lmpar(Y, S.delta, par, x)) return false;
         S.lambda = par;
     } else {
         bool ok = false;
         for (int attempt = 0; attempt < 12 && !ok; ++attempt) {
             double L[7][7];
-            ok = lm_cholesky(S, idx, nv, S.lambda, L);
+            ok = lm_cholesky(Y, S.lambda, L);
             if (!ok) { S.lambda *= 10.0; continue; }
-            lm_step(L, nv, S, idx, x);
+            lm_step(L, Y, x);
         }
         if (!ok) return false;
     }
     // predicted reduction of chi2 under the linear model:
     // -2 g.dp - dp.H.dp = dp.H.dp + 2 par ||D dp||^2 for the damped step
     double gd = 0.0, dhd = 0.0, dn = 0.0, pn = 0.0;
-    for (int a = 0; a < nv; ++a) {
-        const int i = idx[a];
-        gd += S.g[i] * x[a];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        gd += Y.g[i] * x[i];
         double hd = 0.0;
-        for (int c = 0; c < nv; ++c) hd += lm_jtj(S, i, idx[c]) * x[c];
-        dhd += x[a] * hd;
-        const double d = S.dscale[i];
-        dn += d * d * x[a] * x[a];
+#pragma unroll
+        for (int j = 0; j < 7; ++j) hd += Y.A[i][j] * x[j];
+        dhd += x[i] * hd;
+        const double d = set.vary[i] ? S.dscale[i] : 0.0;
+        dn += d * d * x[i] * x[i];
         pn += d * d * p[i] * p[i];
     }
     S.jpn = dhd;
@@ -313,8 +335,8 @@ LMZ_HD bool lm_solve(LMState &S, const lmz_lm_settings &set, const double *p,
     // MINPACK: on the first outer iteration the radius shrinks to the step
     if (set.damping == LMZ_DAMPING_MINPACK && S.njev == 1)
         S.delta = fmin(S.delta, S.dnorm);
-    for (int i = 0; i < LMZ_NPARAM; ++i) pt[i] = p[i];
-    for (int a = 0; a < nv; ++a) pt[idx[a]] = p[idx[a]] + x[a];
+#pragma unroll
+    for (int i = 0; i < LMZ_NPARAM; ++i) pt[i] = p[i] + (i < 7 ? x[i] : 0.0);
     return true;
 }
 
@@ -348,11 +370,10 @@ LMZ_HD bool lm_accept(LMState &S, const lmz_lm_settings &set, double chi2t,
         accept = ratio >= 1e-4;
         if (accept) {
             S.chi2 = chi2t;
-            int idx[7];
-            const int nv = lm_free(set, idx);
             double xs = 0.0;
-            for (int a = 0; a < nv; ++a) {
-                const double t = S.dscale[idx[a]] * pt[idx[a]];
+#pragma unroll
+            for (int i = 0; i < 7; ++i) {
+                const double t = set.vary[i] ? S.dscale[i] * pt[i] : 0.0;
                 xs += t * t;
             }
             S.xnorm = sqrt(xs);

@@ -33,6 +33,11 @@ LMZ_HD void cmac(cf &acc, float s, cf a) {
 }
 // Re(conj(a) b)
 LMZ_HD float credot(cf a, cf b) { return fmaf(a.re, b.re, a.im * b.im); }
+// c z for a table coefficient given as (Re c, Im c, -Im c): each part is a
+// product and a multiply-add into it
+LMZ_HD cf cmul3(float cre, float cim, float cnim, cf z) {
+    return {fmaf(cre, z.re, cnim * z.im), fmaf(cre, z.im, cim * z.re)};
+}
 
 LMZ_HD cd operator+(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
 LMZ_HD cd operator-(cd a, cd b) { return {a.re - b.re, a.im - b.im}; }
@@ -225,14 +230,24 @@ LMZ_HD void pixel_geometry(const FeatUniform &fu, float ax, float ay, float az,
 // ====================================================================
 // per-order table row (LMZ_COEF_STRIDE floats, scalar loads per order)
 // ====================================================================
-//   [0..5]   PR = i^(n+1)(2n+1) a_n,  P = i E_n a_n,  Q = E_n b_n  (FP32)
+//   [0] -Im P   [1] Im Q    (P = i E_n a_n, Q = E_n b_n)
+//   [2..3]   P              [4..5]  -Q                           (FP32)
 //   [6]      n          [7]  n(n+1)
 //   [8..9]   1 + 1/n    [10..11]  n(n+1)/2    [12..13]  (n+1)(n+2)/2
 //   [14..15] n                                                   (FP64)
-//   [16..27] PR, P, Q                                            (FP64)
-//   [28]     1 + 1/n (FP32, the Jacobian's angular recurrence)
-// E_n = i^n (2n+1)/(n(n+1)) as theory/LorenzMie.py:297-303.
-#define LMZ_COEF_STRIDE 32
+//   [16..27] PR = i^(n+1)(2n+1) a_n, P, Q                        (FP64)
+//   [28] 1 + 1/n   [29] -n(n+1)/2   [30] (n+1)(n+2)/2   [31] 2n - 1
+//   [32] -n                                                       (FP32)
+// E_n = i^n (2n+1)/(n(n+1)) as theory/LorenzMie.py:297-303.  The FP32
+// entries carry the signs the per-pixel loops need, so that every complex
+// product is a multiply and a multiply-add into its own result and every
+// sum accumulates a product of positive factors: on gfx950 those are the
+// two-operand (VOP2) instructions that issue two at a time, where forms
+// with a negated operand (VOP3) issue one at a time.
+#define LMZ_COEF_STRIDE 40
+// dcoef row: for each of a_p and n_p (9 floats each, at 0 and 9): the
+// tangents of PR (re, im, -im), of P (re, im, -im) and of -Q (re, im, -im);
+// then the k_p tangents of PR, P, Q (re, im) at 18..23.
 #define LMZ_DCOEF_STRIDE 24
 
 LMZ_HD double row_d(const float *row, int i) {
@@ -282,11 +297,11 @@ struct Angular32 {
     LMZ_HD void init() { dm = 0.f; dn = 0.f; pn = 1.f; }
     LMZ_HD void step(float w, const float *row, float &pi, float &tau) {
         const float fn = row[6], c1 = row[28];
-        const float an = 0.5f * row[7], an1 = an + fn + 1.f;
+        const float man = row[29], an1 = row[30];    // -a_n, a_{n+1}
         const float wp = w * pn;
         const float Td = (dn - dm) + wp;
         pi = pn;
-        tau = fmaf(fn, Td, -(an + dm));
+        tau = fmaf(fn, Td, man - dm);                // n Td - (a_n + d_{n-1})
         const float dnext = fmaf(c1, Td, dn + wp);
         dm = dn;
         dn = dnext;
@@ -411,20 +426,21 @@ LMZ_HD SumsF forward_sum_f(const PixGeom &g, const float *coef, int nmax) {
 #pragma unroll 2
     for (int n = 1; n <= nmax; ++n) {
         const float *cp = coef + LMZ_COEF_STRIDE * (n - 1);
-        const cf P{cp[2], cp[3]}, Q{cp[4], cp[5]};
-        const float fn = cp[6], nn1 = cp[7];
+        const float nn1 = cp[7];
         float pn, tau;
         ang.step(w, cp, pn, tau);
-        const float ninv = fn * invR;
-        const float cx = fmaf(2.f, ninv, -invR);
+        const float ninv = cp[6] * invR;
+        const float cx = cp[31] * invR;                  // (2n - 1) / kr
         const cf xi{fmaf(cx, x1.re, -x2.re), fmaf(cx, x1.im, -x2.im)};
         const cf D{fmaf(ninv, xi.re, -x1.re), fmaf(ninv, xi.im, -x1.im)};
-        const cf PD = cmul(P, D), QX = cmul(Q, xi), PX = cmul(P, xi);
+        const cf PD = cmul3(cp[2], cp[3], cp[0], D);
+        const cf PX = cmul3(cp[2], cp[3], cp[0], xi);
+        const cf mQX = cmul3(cp[4], cp[5], cp[1], xi);   // -Q xi
         cmac(Sr, pn * nn1, PX);          // PR = n(n+1) P
         cmac(St, tau, PD);
-        cmac(St, -pn, QX);
+        cmac(St, pn, mQX);
         cmac(Sp, pn, PD);
-        cmac(Sp, -tau, QX);
+        cmac(Sp, tau, mQX);
         x2 = x1;
         x1 = xi;
     }
@@ -500,8 +516,10 @@ struct JacPixel {
 LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
                            const float *coef, const float *dcoef, int nmax,
                            JacPixel &out) {
+    // Sums of the loop, each accumulating products of positive factors (see
+    // the table layout): mSrR = -S_r,R, mStR = -S_t,R, mSpR = -S_p,R.
     cf Sr{0.f, 0.f}, St{0.f, 0.f}, Sp{0.f, 0.f};
-    cf SrR{0.f, 0.f}, StR{0.f, 0.f}, SpR{0.f, 0.f};
+    cf mSrR{0.f, 0.f}, mStR{0.f, 0.f}, mSpR{0.f, 0.f};
     cf Src{0.f, 0.f}, Stc{0.f, 0.f}, Spc{0.f, 0.f};
     cf Tr[2], Tt[2], Tp[2];    // coefficient tangents: a_p, n_p
 #pragma unroll
@@ -516,55 +534,56 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
     for (int n = 1; n <= nmax; ++n) {
         const float *cp = coef + LMZ_COEF_STRIDE * (n - 1);
         const float *dp = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
-        const cf P{cp[2], cp[3]}, Q{cp[4], cp[5]};
-        const float fn = cp[6], nn1 = cp[7];
+        const float nn1 = cp[7];
         const float c1 = cp[28];
         float pn, tau;
         ang.step(w, cp, pn, tau);
         // tangent of Wiscombe's recurrence with respect to c
         const float dSv = fmaf(dpn, cl, pn);
         const float dT = dSv - dpm;
-        const float dtau = fmaf(-fn, dT, dpm);
-        const float ninv = fn * invRl;
-        const float cx = fmaf(2.f, ninv, -invRl);
+        const float dtau = fmaf(cp[32], dT, dpm);     // dpm - n dT
+        const float ninv = cp[6] * invRl;
+        const float cx = cp[31] * invRl;              // (2n - 1) / kr
         const cf xi{fmaf(cx, x1.re, -x2.re), fmaf(cx, x1.im, -x2.im)};
         const cf D{fmaf(ninv, xi.re, -x1.re), fmaf(ninv, xi.im, -x1.im)};
-        const float gn = fmaf(-nn1, invR2, 1.f);
+        const float mgn = fmaf(nn1, invR2, -1.f);      // -(1 - n(n+1)/R^2)
         // PR = n(n+1) P: the S_r terms use P xi and P D with pi_n n(n+1)
-        const cf PD = cmul(P, D), QX = cmul(Q, xi);
-        const cf PX = cmul(P, xi), QD = cmul(Q, D);
+        const cf PD = cmul3(cp[2], cp[3], cp[0], D);
+        const cf PX = cmul3(cp[2], cp[3], cp[0], xi);
+        const cf mQX = cmul3(cp[4], cp[5], cp[1], xi); // -Q xi
+        const cf mQD = cmul3(cp[4], cp[5], cp[1], D);  // -Q D
         const float pnn = pn * nn1;
         // value
         cmac(Sr, pnn, PX);
         cmac(St, tau, PD);
-        cmac(St, -pn, QX);
+        cmac(St, pn, mQX);
         cmac(Sp, pn, PD);
-        cmac(Sp, -tau, QX);
-        // d/dR
-        cmac(SrR, -pnn, PD);
-        const float tg = tau * gn, pg = pn * gn;
-        cmac(StR, tg, PX);
-        cmac(StR, pn, QD);
-        cmac(SpR, pg, PX);
-        cmac(SpR, tau, QD);
+        cmac(Sp, tau, mQX);
+        // d/dR (negated)
+        cmac(mSrR, pnn, PD);
+        const float mtg = tau * mgn, mpg = pn * mgn;
+        cmac(mStR, mtg, PX);
+        cmac(mStR, pn, mQD);
+        cmac(mSpR, mpg, PX);
+        cmac(mSpR, tau, mQD);
         // d/dc
         cmac(Src, dpn * nn1, PX);
         cmac(Stc, dtau, PD);
-        cmac(Stc, -dpn, QX);
+        cmac(Stc, dpn, mQX);
         cmac(Spc, dpn, PD);
-        cmac(Spc, -dtau, QX);
+        cmac(Spc, dtau, mQX);
         // coefficient tangents
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const float *d = dp + 8 * q;
-            const cf a1 = cmul(cf{d[0], d[1]}, xi);
-            const cf a2 = cmul(cf{d[2], d[3]}, D);
-            const cf a3 = cmul(cf{d[4], d[5]}, xi);
+            const float *d = dp + 9 * q;
+            const cf a1 = cmul3(d[0], d[1], d[2], xi);
+            const cf a2 = cmul3(d[3], d[4], d[5], D);
+            const cf b3 = cmul3(d[6], d[7], d[8], xi);   // -dQ xi
             cmac(Tr[q], pn, a1);
             cmac(Tt[q], tau, a2);
-            cmac(Tt[q], -pn, a3);
+            cmac(Tt[q], pn, b3);
             cmac(Tp[q], pn, a2);
-            cmac(Tp[q], -tau, a3);
+            cmac(Tp[q], tau, b3);
         }
         dpm = dpn;
         dpn = fmaf(c1, dT, dSv);
@@ -573,10 +592,13 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
     }
 
     // ---- undo the mirror (S_p, and d/dR, d/dc of S_r and S_t, change sign)
+    // and the signs of the negated d/dR sums
+    cf SrR, StR, SpR;
     {
         const float m = g.mir;
         Sp = m * Sp;
-        SrR = m * SrR; StR = m * StR;
+        SrR = (-m) * mSrR; StR = (-m) * mStR;
+        SpR = -1.f * mSpR;
         Src = m * Src; Stc = m * Stc;
 #pragma unroll
         for (int q = 0; q < 2; ++q) Tp[q] = m * Tp[q];
@@ -682,9 +704,9 @@ LMZ_HD void write_order(int n, cd a, cd b, float *crow) {
     const cd P = mul_ipow(n + 1, en * a);           // i E_n a_n
     const cd Q = mul_ipow(n, en * b);               // E_n b_n
     const cd PR = nn1 * P;                          // i^(n+1) (2n+1) a_n
-    crow[0] = (float)PR.re; crow[1] = (float)PR.im;
+    crow[0] = -(float)P.im; crow[1] = (float)Q.im;
     crow[2] = (float)P.re;  crow[3] = (float)P.im;
-    crow[4] = (float)Q.re;  crow[5] = (float)Q.im;
+    crow[4] = -(float)Q.re; crow[5] = -(float)Q.im;
     crow[6] = (float)n;
     crow[7] = (float)nn1;
     put_d(crow, 8, 1.0 + 1.0 / n);
@@ -695,9 +717,25 @@ LMZ_HD void write_order(int n, cd a, cd b, float *crow) {
     put_d(crow, 20, P.re);  put_d(crow, 22, P.im);
     put_d(crow, 24, Q.re);  put_d(crow, 26, Q.im);
     crow[28] = (float)(1.0 + 1.0 / n);
-    for (int i = 29; i < LMZ_COEF_STRIDE; ++i) crow[i] = 0.f;
+    crow[29] = (float)(-0.5 * nn1);
+    crow[30] = (float)(0.5 * (n + 1.0) * (n + 2.0));
+    crow[31] = (float)(2.0 * n - 1.0);
+    crow[32] = (float)(-n);
+    for (int i = 33; i < LMZ_COEF_STRIDE; ++i) crow[i] = 0.f;
 }
+
+// Tangent rows: q = 0 (a_p), 1 (n_p) as (Re, Im, -Im) triples of PR, P,
+// -Q at 9 q; write_dorder_k: the k_p tangents (Re, Im of PR, P, Q) at 18.
 LMZ_HD void write_dorder(int n, cd da, cd db, float *drow) {
+    const double en = (2.0 * n + 1.0) / ((double)n * (n + 1.0));
+    const cd P = mul_ipow(n + 1, en * da);
+    const cd Q = mul_ipow(n, en * db);
+    const cd PR = ((double)n * (n + 1.0)) * P;
+    drow[0] = (float)PR.re; drow[1] = (float)PR.im; drow[2] = -(float)PR.im;
+    drow[3] = (float)P.re;  drow[4] = (float)P.im;  drow[5] = -(float)P.im;
+    drow[6] = -(float)Q.re; drow[7] = -(float)Q.im; drow[8] = (float)Q.im;
+}
+LMZ_HD void write_dorder_k(int n, cd da, cd db, float *drow) {
     const double en = (2.0 * n + 1.0) / ((double)n * (n + 1.0));
     const cd P = mul_ipow(n + 1, en * da);
     const cd Q = mul_ipow(n, en * db);
@@ -705,23 +743,7 @@ LMZ_HD void write_dorder(int n, cd da, cd db, float *drow) {
     drow[0] = (float)PR.re; drow[1] = (float)PR.im;
     drow[2] = (float)P.re;  drow[3] = (float)P.im;
     drow[4] = (float)Q.re;  drow[5] = (float)Q.im;
-    drow[6] = 0.f;          drow[7] = 0.f;
 }
-
-// Homogeneous sphere: the one-layer recursion of theory/Sphere.py:147-226
-// (D1 downward from D1[nmax] = 0 for z = m x and z = x; Psi, zeta, Psi*zeta,
-// D3 upward; Eqs. 5 and 6), carried with tangents d/dx (real) and d/dm
-// (holomorphic), then mapped to a_p (x = k a_p), n_p and k_p
-// (m = (n_p + i k_p) / n_m).  It runs in three phases over a scratch table
-// W(n, j) = work[(n * LMZ_WORK_PER_ORDER + j) * wstride], n = 0..N:
-//   1. two independent downward chains, D1(m x) with dD1/dz into columns
-//      0-3 and D1(x) with dD1/dx into columns 4-7 (mie_d1_down);
-//   2. the upward chain Psi_n, zeta_n, Psi_n zeta_n, D3_n with their x
-//      derivatives into columns 8-13 (mie_up_chain);
-//   3. per order, independently, a_n, b_n and their tangents from the
-//      table (mie_order).
-// The kernel runs phase 1 on two lanes and phase 3 on all lanes of a
-// wavefront; mie_feature runs the phases in sequence.
 
 struct MieSetup {
     double kum, x, n_m;
@@ -849,8 +871,8 @@ LMZ_HD void mie_order(const MieSetup &ms, int n, const double *work,
         const double inm = 1.0 / ms.n_m;
         float *drow = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
         write_dorder(n, ms.kum * a_x, ms.kum * b_x, drow);                 // a_p
-        write_dorder(n, inm * a_m, inm * b_m, drow + 8);                  // n_p
-        write_dorder(n, mul_i(inm * a_m), mul_i(inm * b_m), drow + 16);   // k_p
+        write_dorder(n, inm * a_m, inm * b_m, drow + 9);                  // n_p
+        write_dorder_k(n, mul_i(inm * a_m), mul_i(inm * b_m), drow + 18); // k_p
     }
     if (ab) {
         double *r = ab + 4 * (n - 1);

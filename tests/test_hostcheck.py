@@ -12,7 +12,7 @@ from conftest import INSTRUMENT, ROOT
 from oracle import lorenzmie as ref
 
 LIB = os.path.join(ROOT, 'tests', 'native', 'liblmz_hostcheck.so')
-COEF_STRIDE = 32
+COEF_STRIDE = 40
 DCOEF_STRIDE = 24
 
 
@@ -101,8 +101,15 @@ def test_mie_matches_golden_and_tangents(chk, forward_golden):
             fd = fd + w * (ref.mie_coefficients(*qp, 1.34, 0.447, nmax=N) -
                            ref.mie_coefficients(*qm, 1.34, 0.447, nmax=N))
         fd = fd[1:] / h
-        dP = dcoef[:N, 8 * j + 2] + 1j * dcoef[:N, 8 * j + 3]
-        dQ = dcoef[:N, 8 * j + 4] + 1j * dcoef[:N, 8 * j + 5]
+        if j < 2:   # (Re, Im, -Im) triples of dPR, dP, -dQ at 9 j
+            b = 9 * j
+            dP = dcoef[:N, b + 3] + 1j * dcoef[:N, b + 4]
+            dQ = -(dcoef[:N, b + 6] + 1j * dcoef[:N, b + 7])
+            assert np.array_equal(dcoef[:N, b + 5], -dcoef[:N, b + 4])
+            assert np.array_equal(dcoef[:N, b + 8], -dcoef[:N, b + 7])
+        else:       # k_p: (Re, Im) of dPR, dP, dQ at 18
+            dP = dcoef[:N, 20] + 1j * dcoef[:N, 21]
+            dQ = dcoef[:N, 22] + 1j * dcoef[:N, 23]
         da = dP / (1j ** (nn + 1) * en)
         db = dQ / (1j ** nn * en)
         scale = np.abs(fd).max()

@@ -249,26 +249,63 @@ class _FramePopulation:
         return np.where(np.isfinite(chi2), chi2, np.inf)
 
 
-def estimate_batch(estimators: list, holograms: list, maxiter: int = 1000
+_BATCH_SETTINGS = {'tol', 'atol', 'maxiter', 'mutation', 'recombination',
+                   'strategy', 'polish', 'updating', 'workers', 'init'}
+
+
+def estimate_batch(estimators: list, holograms: list, maxiter: int | None = None
                    ) -> list:
     """One differential-evolution search per (estimator, hologram) pair,
     all advanced together: every generation's candidates of every search in
-    one device launch, and scipy's best1bin step (dithered scale in
-    [0.5, 1) drawn per generation, binomial crossover at 0.7 with one forced
-    dimension, out-of-range values redrawn, deferred selection, stop when
-    std(energies) <= tol |mean(energies)|) applied to all of them at once
-    as array operations.  The estimators must share bounds keys, popsize
-    and tol.  Returns each estimator's _finish (its model updated)."""
+    one device launch, and scipy's best1bin step (scale in the ``mutation``
+    range drawn per generation, binomial crossover at ``recombination`` with
+    one forced dimension, out-of-range values redrawn, deferred selection,
+    stop when std(energies) <= atol + tol |mean(energies)|, at most
+    ``maxiter`` generations) applied to all of them at once as array
+    operations; scipy's population of max(5, popsize * D) members.  The
+    estimators must be DEEstimators with the same bounds keys, popsize,
+    settings and instrument; the searches draw from one random stream
+    seeded by the first estimator's seed.  Returns each estimator's
+    _finish (its model updated)."""
     if not estimators:
         return []
     e0 = estimators[0]
+    for e in estimators:
+        if not isinstance(e, DEEstimator):
+            raise TypeError('estimate_batch needs DEEstimators, got '
+                            f'{type(e).__name__}')
     names = list(e0.bounds.keys())
     D, F = len(names), len(estimators)
-    npop = e0.popsize * D
-    tol = float(e0.settings.get('tol', 0.01))
+    npop = max(5, e0.popsize * D)
+    settings = dict(e0.settings)
+    unknown = set(settings) - _BATCH_SETTINGS
+    if unknown:
+        raise ValueError(f'estimate_batch does not support {sorted(unknown)}')
+    if settings.get('strategy', 'best1bin') != 'best1bin':
+        raise ValueError('estimate_batch runs the best1bin strategy only')
+    if settings.get('polish', False):
+        raise ValueError('estimate_batch does not polish (polish=False)')
+    if settings.get('init', 'latinhypercube') != 'latinhypercube':
+        raise ValueError('estimate_batch starts from a Latin hypercube')
+    tol = float(settings.get('tol', 0.01))
+    atol = float(settings.get('atol', 0.))
+    mutation = settings.get('mutation', (0.5, 1.))
+    mlo, mhi = ((float(mutation), float(mutation)) if np.isscalar(mutation)
+                else (float(mutation[0]), float(mutation[1])))
+    recombination = float(settings.get('recombination', 0.7))
+    if maxiter is None:
+        maxiter = int(settings.get('maxiter', 1000))
+    inst0 = e0.model.instrument
     for e in estimators:
         if list(e.bounds.keys()) != names or e.popsize != e0.popsize:
             raise ValueError('estimators differ in bounds or popsize')
+        if {k: v for k, v in e.settings.items() if k != 'workers'} != \
+                {k: v for k, v in settings.items() if k != 'workers'}:
+            raise ValueError('estimators differ in settings')
+        inst = e.model.instrument
+        if (inst.n_m, inst.wavelength, inst.magnification) != \
+                (inst0.n_m, inst0.wavelength, inst0.magnification):
+            raise ValueError('estimators differ in instrument')
     probs = [e._problem(h) for e, h in zip(estimators, holograms)]
     lo = np.array([[e.bounds[k][0] for k in names] for e in estimators])
     hi = np.array([[e.bounds[k][1] for k in names] for e in estimators])
@@ -289,7 +326,7 @@ def estimate_batch(estimators: list, holograms: list, maxiter: int = 1000
     idx = np.arange(npop)
     for _ in range(maxiter):
         best = unit[np.arange(F), np.argmin(e, axis=1)]             # (F, D)
-        scale = rng.uniform(0.5, 1., (F, 1, 1))
+        scale = rng.uniform(mlo, mhi, (F, 1, 1))
         k0 = rng.integers(0, npop - 1, (F, npop))
         r0 = k0 + (k0 >= idx)
         a, b = np.minimum(idx, r0), np.maximum(idx, r0)
@@ -298,7 +335,7 @@ def estimate_batch(estimators: list, holograms: list, maxiter: int = 1000
         r1 = r1 + (r1 >= b)
         take = np.arange(F)[:, None]
         mutant = best[:, None, :] + scale * (unit[take, r0] - unit[take, r1])
-        cross = rng.random((F, npop, D)) < 0.7
+        cross = rng.random((F, npop, D)) < recombination
         cross[take, idx[None, :], rng.integers(0, D, (F, npop))] = True
         trial = np.where(cross, mutant, unit)
         bad = (trial < 0.) | (trial > 1.)
@@ -308,7 +345,7 @@ def estimate_batch(estimators: list, holograms: list, maxiter: int = 1000
         unit = np.where(better[..., None], trial, unit)
         e = np.where(better, et, e)
         with np.errstate(invalid='ignore'):     # a search holding inf waits
-            done |= np.std(e, axis=1) <= tol * np.abs(np.mean(e, axis=1))
+            done |= np.std(e, axis=1) <= atol + tol * np.abs(np.mean(e, axis=1))
         if done.all():
             break
     x = lo + unit[np.arange(F), np.argmin(e, axis=1)] * (hi - lo)

@@ -35,11 +35,13 @@ class VMedian:
         return self.get()
 
     def get(self, reshape: bool = True):
-        """The current median image (vmedian.py:35-48)."""
+        """The current median image (vmedian.py:35-48): a new array or
+        tensor each call, which later frames do not change (the running
+        median is updated in place)."""
         d = self._data
         if reshape:
             d = d.reshape(self.shape)
-        return d.cpu().numpy() if self._host else d
+        return d.cpu().numpy() if self._host else d.clone()
 
     def add(self, data) -> None:
         """Include a new frame (vmedian.py:50-77)."""

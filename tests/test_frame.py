@@ -82,7 +82,8 @@ def test_batched_de_finds_each_minimum(monkeypatch):
                                 bounds={'z_p': (50., 400.), 'a_p': (0.3, 2.0),
                                         'n_p': (1.35, 1.8)}))
         holos.append(Hologram(np.random.default_rng(i).random((20, 20))))
-    ests[0].settings['tol'] = 1e-6
+    for e in ests:
+        e.settings['tol'] = 1e-6
     out = de_mod.estimate_batch(ests, holos)
     for i, (res, est) in enumerate(zip(out, ests)):
         got = np.array([res['z_p'], res['a_p'], res['n_p']])
@@ -93,3 +94,33 @@ def test_batched_de_finds_each_minimum(monkeypatch):
     with pytest.raises(ValueError):
         de_mod.estimate_batch([ests[0], DEEstimator(model=LorenzMie(), popsize=5)],
                               holos[:2])
+    # scipy's population floor max(5, popsize * D): one bound, popsize 2
+    small = [DEEstimator(model=LorenzMie(), seed=1, popsize=2,
+                         bounds={'z_p': (50., 400.)}) for _ in range(3)]
+    seen = []
+
+    class Line:
+        def __init__(self, probs, npop):
+            seen.append(npop)
+            self.npop = npop
+
+        def __call__(self, rows):
+            return ((rows[:, 2] - 200.) / 50.) ** 2
+    monkeypatch.setattr(de_mod, '_FramePopulation', Line)
+    out = de_mod.estimate_batch(small, holos)
+    assert seen == [5]
+    assert all(abs(r['z_p'] - 200.) < 5. for r in out)
+    # settings the batched step cannot honour are refused, others differ
+    odd = DEEstimator(model=LorenzMie(), seed=1, popsize=2,
+                      bounds={'z_p': (50., 400.)})
+    odd.settings['strategy'] = 'rand1bin'
+    with pytest.raises(ValueError):
+        de_mod.estimate_batch([odd], holos[:1])
+    other = DEEstimator(model=LorenzMie(), seed=1, popsize=2,
+                        bounds={'z_p': (50., 400.)})
+    other.settings['tol'] = 0.5
+    with pytest.raises(ValueError):
+        de_mod.estimate_batch([small[0], other], holos[:2])
+    from pylorenzmie_amd.analysis import RadialEstimator
+    with pytest.raises(TypeError):
+        de_mod.estimate_batch([RadialEstimator(model=LorenzMie())], holos[:1])

@@ -219,6 +219,25 @@ def test_vmedian_matches_oracle(torch, order):
         np.testing.assert_array_equal(dev.get().cpu().numpy(), theirs.get())
 
 
+def test_vmedian_results_are_not_changed_by_later_frames(torch):
+    """Each get() is a new array, as the reference's np.median result is
+    (utilities/vmedian.py:73): the in-place running median does not reach a
+    result returned earlier, for host or device frames."""
+    from pylorenzmie_amd.utilities import VMedian
+    rng = np.random.default_rng(9)
+    frames = rng.integers(0, 256, (12, 8, 9), dtype=np.uint8)
+    for device in (False, True):
+        vm = VMedian(order=1)
+        kept = []
+        for f in frames:
+            vm.add(torch.from_numpy(f).cuda() if device else f)
+            got = vm.get()
+            kept.append((got, (got.cpu().numpy() if device else got).copy()))
+        for got, snap in kept:
+            now = got.cpu().numpy() if device else got
+            np.testing.assert_array_equal(now, snap)
+
+
 # ----------------------------------------------------------- gather / Frame
 def _frame_pair(torch, corner, mask_fraction):
     from pylorenzmie_amd.analysis import Frame

@@ -261,6 +261,11 @@ __global__ __launch_bounds__(NT) void k_jacobian(
     const uint8_t *__restrict__ active, const int32_t *__restrict__ list,
     int32_t *count) {
   __shared__ int next_w;
+  // Each thread's normal-equation partials live in LDS (its own column of
+  // acc_s), not in registers: the 36 registers they would hold keep the
+  // kernel below the register budget of four waves per SIMD.
+  __shared__ float acc_s[LMZ_NRED][NT];
+  float *mine = &acc_s[0][threadIdx.x];
   int chunks_done = 0;
   const int nwork = count ? count[0] : (int)gridDim.x;
   for (int w = list ? next_tile(count, next_w, chunks_done)
@@ -280,11 +285,12 @@ __global__ __launch_bounds__(NT) void k_jacobian(
     const float *cf0 = coef + (long long)LMZ_COEF_STRIDE * fu.slot0;
     const float *df0 = dcoef + (long long)LMZ_DCOEF_STRIDE * fu.slot0;
     const float *rinf = rin ? rin + b * rin_stride : nullptr;
-    float acc[LMZ_NRED];
 #pragma unroll
-    for (int i = 0; i < LMZ_NRED; ++i) acc[i] = 0.f;
+    for (int i = 0; i < LMZ_NRED; ++i) mine[i * NT] = 0.f;
 #pragma unroll 1
     for (int q = 0; q < LMZ_TILE / NT; ++q) {
+        // the partials stay in LDS across pixels (no register copies)
+        asm volatile("" ::: "memory");
         const int j = (t - fu.tile0) * LMZ_TILE + q * NT + threadIdx.x;
         if (j >= fu.npix) break;
         float ax, ay, az;
@@ -310,15 +316,18 @@ __global__ __launch_bounds__(NT) void k_jacobian(
                 for (int a = 0; a < LMZ_NJAC; ++a)
 #pragma unroll
                     for (int c = a; c < LMZ_NJAC; ++c, ++idx)
-                        acc[idx] = fmaf(Jn[a], Jn[c], acc[idx]);
+                        mine[idx * NT] = fmaf(Jn[a], Jn[c], mine[idx * NT]);
 #pragma unroll
                 for (int a = 0; a < LMZ_NJAC; ++a)
-                    acc[28 + a] = fmaf(Jn[a], r, acc[28 + a]);
-                acc[35] = fmaf(r, r, acc[35]);
+                    mine[(28 + a) * NT] = fmaf(Jn[a], r, mine[(28 + a) * NT]);
+                mine[35 * NT] = fmaf(r, r, mine[35 * NT]);
             }
         }
     }
     if (partials && data) {
+        float acc[LMZ_NRED];
+#pragma unroll
+        for (int i = 0; i < LMZ_NRED; ++i) acc[i] = mine[i * NT];
         block_reduce_store<LMZ_NRED, NT>(acc, partials + (long long)LMZ_NRED * t);
         __syncthreads();   // the reduction buffer is reused by the next tile
     }

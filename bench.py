@@ -46,7 +46,7 @@ the fitter instead:
   3  batched LM on uneven crops (side log-U[48, 512]): fits/s, the
      Jacobian's FP32 fraction, the J^T J reduction's HBM rate;
   4  hard regime (512 crops of 512^2, layered and large spheres): hologram
-     time and error, fits/s of both halves.
+     time and error, 7-parameter fits/s of both halves.
 """
 from __future__ import annotations
 
@@ -662,13 +662,17 @@ def run_config(args):
             hb.npix_total, generator=g, device='cuda', dtype=torch.float32)
         inst_n = Instrument(noise=0.05, **INSTRUMENT)
         out = {}
-        for name, idx, vary in (('layered', lay_idx, ['x_p', 'y_p', 'z_p', 'alpha']),
-                                ('homogeneous', [i for i in range(n) if len(layers[i]) == 1],
-                                 list(PARAMS))):
+        # both halves fit all seven parameters from the uneven-crop
+        # workload's starts; a layered sphere's a_p scales every radius and
+        # n_p + i k_p is its outer layer's index (lmz_lm_run_layered)
+        for name, idx in (('layered', lay_idx),
+                          ('homogeneous', [i for i in range(n) if len(layers[i]) == 1])):
+            vary = list(PARAMS)
             sub = data.view(n, side * side)[idx].reshape(-1)
             crops = [dict(data=np.zeros((side, side), np.float32), width=side,
                           origin=(0., 0., 0.),
-                          nmax_cap=(caps[i] if name == 'layered' else
+                          nmax_cap=(layered_cap(layers[i], 1.34, 0.447, headroom=0.25)
+                                    if name == 'layered' else
                                     _lmz.nmax_cap(1.05 * params[i, 3], params[i, 4],
                                                   0., 1.34, 0.447)))
                      for i in idx]
@@ -676,12 +680,10 @@ def run_config(args):
             lm.data.copy_(sub)
             start = params[idx].copy()
             start[:, :3] += (2., -2., 2.)
+            start[:, 3] *= 1.05
+            start[:, 4] -= 0.02
             if name == 'layered':
-                start[:, 6] *= 1.1
-                lm.load_layers([layers[i] for i in idx])
-            else:
-                start[:, 3] *= 1.05
-                start[:, 4] -= 0.02
+                lm.fit_layers([layers[i] for i in idx])
             settings = LMSettings(max_iter=50)
             lm.load(start)
             lm.run(settings, vary)
@@ -699,12 +701,14 @@ def run_config(args):
                              varied=vary, converged_frac=float(ok.mean()),
                              at_noise_level_frac=float(good.mean()),
                              median_abs_err_xyz=float(np.median(
-                                 np.abs(p[good, :3] - params[idx][good, :3]))))
+                                 np.abs(p[good, :3] - params[idx][good, :3]))),
+                             median_rel_err_a_n=float(np.median(
+                                 np.abs(p[good, 3:5] / params[idx][good, 3:5] - 1.))))
             lm.close()
         line.update(workload='hard regime: 512 crops of 512x512, half layered '
-                             '(2-5 layers, fixed coefficients, x y z alpha '
-                             'fitted), half homogeneous a_p U[0.5, 10] um '
-                             '(7 parameters from config-3 starts)',
+                             '(2-5 layers; a_p scales the radii, outer n_p, k_p), '
+                             'half homogeneous a_p U[0.5, 10] um; 7 parameters '
+                             'from config-3 starts',
                     nmax_max=int(nm.max()), holograms_ms=t * 1e3,
                     max_abs_err_vs_oracle=err,
                     max_err_beyond_float32_rounding=excess, **out)

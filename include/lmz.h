@@ -24,15 +24,15 @@
  *                                         when every feature is a dense crop
  *   field    float2 [3][npix_total]       complex field components
  *   jac      float  [LMZ_NJAC][npix_total] dI/d(param), same parameter order
- *   coef     float  [slots][coef_stride]  per order n: i^(n+1)(2n+1) a_n,
- *                                         i E_n a_n, E_n b_n (complex), n,
- *                                         n(n+1), then as doubles 1 + 1/n,
- *                                         n(n+1)/2, (n+1)(n+2)/2, n and the
- *                                         three coefficients again
- *                                         (lmz_batch_sizes gives the strides)
- *   dcoef    float  [slots][dcoef_stride] the three complex entries above
- *                                         differentiated by a_p, n_p, k_p
- *                                         (8 floats per parameter)
+ *   coef     float  [slots][coef_stride]  per order n: P = i E_n a_n and
+ *                                         -Q = -E_n b_n (complex, with -Im P
+ *                                         and Im Q beside them), n, n(n+1),
+ *                                         the recurrence constants, and as
+ *                                         doubles i^(n+1)(2n+1) a_n, P, Q
+ *                                         (lmz_batch_sizes gives the strides;
+ *                                         the layout is the library's own)
+ *   dcoef    float  [slots][dcoef_stride] the tangents of those entries by
+ *                                         a_p, n_p, k_p
  *   partials double [ntiles][36]          per pixel tile: upper triangle of
  *                                         J^T J (28, row major), J^T r (7),
  *                                         r^T r (1), with J and r scaled by
@@ -156,6 +156,24 @@ int lmz_mie_ab(const lmz_batch *batch, const double *params,
 int lmz_mie_ab_layered(const lmz_batch *batch, const double *layers,
                        const int32_t *nlayers, int32_t *nmax_out, float *coef,
                        double *ab_out, double *work, hipStream_t stream);
+
+/* Layered spheres in the fitting parametrization: each feature's layer
+ * table (as lmz_mie_ab_layered) with its radii scaled by params a_p / (the
+ * table's outer radius) -- fixed radius fractions -- and the outer layer's
+ * index replaced by params n_p + i k_p; inner indices held.  Writes the
+ * coefficient tables of buffer 0 and, unless dcoef is NULL, the tangents
+ * by a_p, n_p, k_p (forward-mode through Yang's recursion; the k_p tangent
+ * is i times the n_p one, the coefficients being holomorphic in the outer
+ * index), which lmz_jacobian and lmz_hologram_vjp use as for homogeneous
+ * spheres.  nmax_out as lmz_mie_ab; 0 for a parameter row outside the
+ * domain (a_p <= 0, n_p <= 0).  The layered extension of the coefficient
+ * tangents Optimizer needs to fit a layered particle's size and index;
+ * the reference has no working layered recursion
+ * (lib/obsolete/sphere_coefficients.py:126-215 fails for layered input). */
+int lmz_mie_ab_layered_fit(const lmz_batch *batch, const double *params,
+                           const double *layers, const int32_t *nlayers,
+                           int32_t *nmax_out, float *coef, float *dcoef,
+                           double *work, hipStream_t stream);
 
 /* Per-order tables from coefficients computed elsewhere (any particle
  * whose ab() is not a homogeneous sphere): ab_in double [slots][4] rows
@@ -387,6 +405,21 @@ int lmz_lm_run_fixed_ab(lmz_lm_plan *plan, const float *coords, int32_t flags,
                         const double *ab, const int32_t *nmax,
                         double *params_io, double *result, void *workspace,
                         hipStream_t stream);
+
+/* As lmz_lm_run for layered spheres whose outer radius a_p and outer index
+ * n_p + i k_p may be fitted, in lmz_mie_ab_layered_fit's parametrization
+ * (radius fractions and inner indices from layers / nlayers, device
+ * arrays as lmz_mie_ab_layered takes them): the coefficients and their
+ * tangents are recomputed at every trial; with settings->fd_step > 0 the
+ * a_p, n_p, k_p columns difference the layered coefficients instead.
+ * Replaces Optimizer.optimize (analysis/Optimizer.py:196-222) for layered
+ * particles, whose ab() the reference re-evaluates on every residual
+ * (theory/LorenzMie.py:156). */
+int lmz_lm_run_layered(lmz_lm_plan *plan, const float *coords, int32_t flags,
+                       const float *data, const lmz_lm_settings *settings,
+                       const double *layers, const int32_t *nlayers,
+                       double *params_io, double *result, void *workspace,
+                       hipStream_t stream);
 
 /* ------------------------------------------------------------------ *
  * Frames resident in device memory: background normalization and the

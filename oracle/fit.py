@@ -31,12 +31,34 @@ def default_settings() -> dict:
             'x_scale': 'jac'}
 
 
+def layered_properties(p, layers, instrument):
+    """Properties of a layered sphere in the fitting parametrization:
+    ``layers`` (L, 3) rows of outer radius, n, k fix the radius fractions
+    r_l / r_L and the inner indices; a_p is the outer radius (all radii
+    scale with it) and n_p + i k_p the outer layer's index.  The
+    coefficients come from Yang's recursion with its own n_max.  The
+    reference fits only homogeneous spheres; this model extends its
+    residuals (analysis/Optimizer.py:257-260) to the layered particle."""
+    lay = np.asarray(layers, float).reshape(-1, 3)
+    r = lay[:, 0] * (p['a_p'] / lay[-1, 0])
+    n = lay[:, 1].copy()
+    k = lay[:, 2].copy()
+    n[-1], k[-1] = p['n_p'], p['k_p']
+    q = dict(p)
+    q['ab'] = lm.mie_coefficients_layered(r, n, k, instrument.get('n_m', 1.340),
+                                          instrument.get('wavelength', 0.447))
+    return q
+
+
 def residuals(values, variables, fixed, data, coordinates, instrument,
-              noise):
+              noise, layers=None):
     """(hologram - data) / noise at the properties ``fixed`` updated with
-    ``values`` of ``variables``; analysis/Optimizer.py:257-260."""
+    ``values`` of ``variables``; analysis/Optimizer.py:257-260.  With
+    ``layers`` the particle is layered (:func:`layered_properties`)."""
     p = dict(fixed)
     p.update(zip(variables, values))
+    if layers is not None:
+        p = layered_properties(p, layers, instrument)
     holo = lm.hologram(coordinates, p, alpha=p.get('alpha', 1.),
                        **instrument)
     return (holo - data) / noise
@@ -56,7 +78,7 @@ def statistics(jac, cost, npix, nvar):
 
 
 def fit(data, coordinates, start, variables=VARS6, instrument=None,
-        noise=0.05, settings=None):
+        noise=0.05, settings=None, layers=None):
     """Fit the hologram model to ``data`` (npts,) at ``coordinates``.
 
     ``start`` maps property names to starting values (x_p .. k_p, and
@@ -64,7 +86,8 @@ def fit(data, coordinates, start, variables=VARS6, instrument=None,
     rest stay at their start.  ``settings`` update the reference's
     defaults.  Returns the reference's result record
     (analysis/Optimizer.py:166-186): value and 'd'-prefixed uncertainty
-    per variable, success, npix, redchi; plus nfev.
+    per variable, success, npix, redchi; plus nfev.  ``layers`` fits a
+    layered sphere (:func:`layered_properties`).
     """
     instrument = dict(instrument or {})
     s = default_settings()
@@ -75,7 +98,8 @@ def fit(data, coordinates, start, variables=VARS6, instrument=None,
     p0 = np.array([fixed.get(v, 1. if v == 'alpha' else 0.)
                    for v in variables], float)
     res = least_squares(residuals, p0, args=(variables, fixed, data,
-                                             coordinates, instrument, noise),
+                                             coordinates, instrument, noise,
+                                             layers),
                         **s)
     redchi, unc = statistics(res.jac, res.cost, data.size, len(variables))
     out = {}

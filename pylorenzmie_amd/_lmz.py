@@ -108,6 +108,7 @@ SIGNATURES = {
     'lmz_batch_query': (_I, [_P, ctypes.POINTER(BatchSizes)]),
     'lmz_mie_ab': (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'lmz_mie_ab_layered': (_I, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    'lmz_mie_ab_layered_fit': (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P]),
     'lmz_coef_from_ab': (_I, [_P, _P, _P, _P, _P, _P]),
     'lmz_field': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P]),
     'lmz_hologram': (_I, [_P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
@@ -131,6 +132,8 @@ SIGNATURES = {
                         _P, _P]),
     'lmz_lm_run_fixed_ab': (_I, [_P, _P, _I, _P, ctypes.POINTER(LMSettings),
                                  _P, _P, _P, _P, _P, _P]),
+    'lmz_lm_run_layered': (_I, [_P, _P, _I, _P, ctypes.POINTER(LMSettings),
+                                _P, _P, _P, _P, _P, _P]),
     'lmz_median_scratch_bytes': (ctypes.c_size_t, []),
     'lmz_median': (_I, [_P, _I, ctypes.c_int64, _P, _P, _P]),
     'lmz_median_filter': (_I, [_P, _I, _I, _I, _I, _P, _P, _I, _P]),

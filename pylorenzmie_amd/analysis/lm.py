@@ -230,6 +230,27 @@ class LMBatch:
             None, _lmz.ptr(self.fixed_ab), _lmz.ptr(work), _lmz.stream_handle()))
         if int(self.fixed_nmax[0::2].min()) < 1:
             raise ValueError('invalid layer table')
+        self.layer_tables = None
+
+    def fit_layers(self, layers_list) -> None:
+        """Layered particles whose outer radius a_p and outer index n_p +
+        i k_p may be fitted (lmz_lm_run_layered): per feature an (L, 3)
+        table of outer radius [um], n, k, innermost first.  At every trial
+        the radii are the table's scaled by a_p / (its outer radius) --
+        fixed radius fractions -- and the outer layer's index is n_p + i
+        k_p; the inner indices are held.  Runs then vary any of the seven
+        parameters, with the coefficient tangents from the device."""
+        from ..theory.LayeredSphere import layer_table
+        t = _lmz.torch()
+        dev = self.params.device
+        table, counts = layer_table(layers_list)
+        if len(counts) != self.nfeat:
+            raise ValueError(f'need {self.nfeat} layer tables')
+        if np.any(counts < 1) or np.any(counts > _lmz.MAX_LAYERS):
+            raise ValueError(f'1..{_lmz.MAX_LAYERS} layers per particle')
+        self.layer_tables = (t.from_numpy(table).to(dev),
+                             t.from_numpy(counts).to(dev))
+        self.fixed_ab = None
 
     def autotune(self, force: bool = False):
         """Time the schedules of the pixel kernels on this batch at the
@@ -253,6 +274,14 @@ class LMBatch:
         """Fit on the device, stream-ordered; results stay in HBM."""
         s = self.lm_settings(settings, vary)
         lib = _lmz.load_library()
+        layers = getattr(self, 'layer_tables', None)
+        if layers is not None:
+            _lmz.check(lib.lmz_lm_run_layered(
+                self.plan, _lmz.ptr(self.coords), self.flags, _lmz.ptr(self.data),
+                ctypes.byref(s), _lmz.ptr(layers[0]), _lmz.ptr(layers[1]),
+                _lmz.ptr(self.params), _lmz.ptr(self.result), self._ws_ptr,
+                _lmz.stream_handle(stream)))
+            return
         if getattr(self, 'fixed_ab', None) is not None:
             _lmz.check(lib.lmz_lm_run_fixed_ab(
                 self.plan, _lmz.ptr(self.coords), self.flags, _lmz.ptr(self.data),

@@ -103,6 +103,58 @@ __global__ __launch_bounds__(64) void k_mie_layered(
         ab_out ? ab_out + 4 * slot0 : nullptr);
 }
 
+// Layered spheres in the fitting parametrization, one lane per feature:
+// the table's radii scaled by params a_p / (the table's outer radius), the
+// outer layer's index n_p + i k_p from params, inner indices from the
+// table; coefficient tables of buffer (sel ^ flip) with the a_p, n_p, k_p
+// tangents (dcoef, may be NULL).  A trial point outside the domain
+// (a_p <= 0, n_p <= 0, non-finite) gets no orders (n_max 0: no scattered
+// field), which the LM rejects by its chi2.
+__global__ __launch_bounds__(64) void k_mie_layered_fit(
+    const lmz_feature *__restrict__ feat, int nfeat,
+    const double *__restrict__ params, const int32_t *__restrict__ sel,
+    int flip, int nbuf, const double *__restrict__ layers,
+    const int32_t *__restrict__ nlayers, int32_t *__restrict__ nmax_out,
+    float *__restrict__ coef, float *__restrict__ dcoef,
+    double *__restrict__ work, const uint8_t *__restrict__ active) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfeat) return;
+    if (active && !active[f]) return;
+    const lmz_feature F = feat[f];
+    const int b = sel ? (sel[f] ^ flip) : 0;
+    const double *p = params + (long long)LMZ_NPARAM * f;
+    const int L = nlayers[f];
+    double r[LMZ_MAX_LAYERS], nl[LMZ_MAX_LAYERS], kl[LMZ_MAX_LAYERS];
+    bool ok = L >= 1 && L <= LMZ_MAX_LAYERS && p[3] > 0.0 && p[4] > 0.0 &&
+              isfinite(p[3]) && isfinite(p[4]) && isfinite(p[5]);
+    for (int l = 0; ok && l < L; ++l) {
+        const double *q = layers + ((long long)f * LMZ_MAX_LAYERS + l) * 3;
+        r[l] = q[0];
+        nl[l] = q[1];
+        kl[l] = q[2];
+        ok = r[l] > (l ? r[l - 1] : 0.0) && isfinite(r[l]);
+    }
+    if (!ok) {
+        nmax_out[(long long)f * nbuf + b] = 0;
+        return;
+    }
+    const double scale = p[3] / r[L - 1];
+    for (int l = 0; l < L - 1; ++l) r[l] *= scale;
+    r[L - 1] = p[3];
+    nl[L - 1] = p[4];
+    kl[L - 1] = p[5];
+    const long long slot0 = F.coef_offset + (long long)b * F.nmax_cap;
+    const int N = mie_layered_feature(r, nl, kl, L, F.n_m, F.wavelength,
+                                      F.nmax_cap, work + f, nfeat,
+                                      coef + LMZ_COEF_STRIDE * slot0, nullptr);
+    if (dcoef)
+        for (int dir = 0; dir < 2; ++dir)
+            mie_layered_tangent(r, nl, kl, L, F.n_m, F.wavelength, dir, N,
+                                work + f, nfeat,
+                                dcoef + LMZ_DCOEF_STRIDE * slot0);
+    nmax_out[(long long)f * nbuf + b] = N;
+}
+
 // Per-order tables from FP64 a_n, b_n supplied by the caller.
 __global__ __launch_bounds__(64) void k_coef_from_ab(
     const lmz_feature *__restrict__ feat, int nfeat,
@@ -660,6 +712,17 @@ void launch_mie(const lmz_batch *b, const double *params, const int32_t *sel,
                        nmax, coef, dcoef, ab_out, work, active);
 }
 
+void launch_mie_layered(const lmz_batch *b, const double *params,
+                        const int32_t *sel, int flip, const double *layers,
+                        const int32_t *nlayers, int32_t *nmax, float *coef,
+                        float *dcoef, double *work, const uint8_t *active,
+                        hipStream_t stream) {
+    hipLaunchKernelGGL(k_mie_layered_fit, dim3((b->nfeat + 63) / 64), dim3(64),
+                       0, stream, b->d_feat, b->nfeat, params, sel, flip,
+                       b->coef_buffers, layers, nlayers, nmax, coef, dcoef,
+                       work, active);
+}
+
 // Pixel-kernel launches at the batch's threads per tile (64, 128, 256).
 template <int MODE, typename... Args>
 static void launch_forward_nt(int nt, int grid, hipStream_t stream,
@@ -745,10 +808,15 @@ void launch_fd_jacobian(const lmz_batch *b, const double *params,
     // params array)
     for (int q = 0; q < 3; ++q) {
         if (!(vary & (1 << (3 + q)))) continue;
-        launch_mie(b, fd.params + (long long)q * b->nfeat * LMZ_NPARAM, nullptr,
-                   0, fd.nmax + (long long)q * b->nfeat * b->coef_buffers,
-                   fd.coef + q * fd.coef_stride, nullptr, nullptr, work, active,
-                   stream);
+        const double *pq = fd.params + (long long)q * b->nfeat * LMZ_NPARAM;
+        int32_t *nq = fd.nmax + (long long)q * b->nfeat * b->coef_buffers;
+        if (fd.layers)
+            launch_mie_layered(b, pq, nullptr, 0, fd.layers, fd.nlayers, nq,
+                               fd.coef + q * fd.coef_stride, nullptr, work,
+                               active, stream);
+        else
+            launch_mie(b, pq, nullptr, 0, nq, fd.coef + q * fd.coef_stride,
+                       nullptr, nullptr, work, active, stream);
     }
     const int grid = tl ? tl->grid : b->ntiles;
     const int nt = b->sched.jac_threads;
@@ -1050,6 +1118,19 @@ int lmz_mie_ab_layered(const lmz_batch *b, const double *layers,
     hipLaunchKernelGGL(k_mie_layered, dim3((b->nfeat + 63) / 64), dim3(64), 0,
                        stream, b->d_feat, b->nfeat, layers, nlayers,
                        b->coef_buffers, nmax_out, coef, ab_out, work);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+int lmz_mie_ab_layered_fit(const lmz_batch *b, const double *params,
+                           const double *layers, const int32_t *nlayers,
+                           int32_t *nmax_out, float *coef, float *dcoef,
+                           double *work, hipStream_t stream) {
+    if (int rc = check_batch(b)) return rc;
+    LMZ_REQUIRE(params && layers && nlayers && nmax_out && coef && work,
+                "params, layers, nlayers, nmax_out, coef and work are required");
+    launch_mie_layered(b, params, nullptr, 0, layers, nlayers, nmax_out, coef,
+                       dcoef, work, nullptr, stream);
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
 }

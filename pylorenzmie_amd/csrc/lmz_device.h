@@ -41,6 +41,12 @@ void launch_mie(const lmz_batch *b, const double *params, const int32_t *sel,
                 int flip, int32_t *nmax, float *coef, float *dcoef,
                 double *ab_out, double *work, const uint8_t *active,
                 hipStream_t stream);
+// layered spheres in the fitting parametrization (k_mie_layered_fit)
+void launch_mie_layered(const lmz_batch *b, const double *params,
+                        const int32_t *sel, int flip, const double *layers,
+                        const int32_t *nlayers, int32_t *nmax, float *coef,
+                        float *dcoef, double *work, const uint8_t *active,
+                        hipStream_t stream);
 // residual (I - d) / noise written to out + (sel[f] ^ flip) * out_stride
 void launch_residual(const lmz_batch *b, const double *params,
                      const int32_t *sel, int flip, const int32_t *nmax,
@@ -66,6 +72,9 @@ struct FdTables {
     int32_t *nmax;         // [3][nfeat * coef_buffers]
     float *coef;           // 3 tables, coef_stride floats apart
     long long coef_stride;
+    // layered spheres in their fitting parametrization (null: homogeneous)
+    const double *layers = nullptr;
+    const int32_t *nlayers = nullptr;
 };
 // Normal-equation partials of a forward-difference Jacobian of the FP64
 // hologram, for the features in tl (or all); rin as in launch_jacobian.

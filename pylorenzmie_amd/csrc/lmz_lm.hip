@@ -215,7 +215,8 @@ struct lmz_lm_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // arguments the captured graph was built for
-    const void *key_ptrs[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    const void *key_ptrs[7] = {nullptr, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, nullptr};
     int32_t key_flags = -1;
     int key_fixed = -1;
     lmz_lm_settings key_set{};
@@ -256,10 +257,20 @@ __global__ __launch_bounds__(64) void k_lm_load_ab(
     }
 }
 
+// Where a trial's Mie coefficients come from: recomputed for homogeneous
+// spheres (k_mie), held (fixed_ab), or recomputed for layered spheres in
+// their fitting parametrization (layers, nlayers).
+struct MieSource {
+    bool fixed_ab;
+    const double *layers;
+    const int32_t *nlayers;
+};
+
 static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
                               char *ws, const float *coords, int32_t flags,
                               const float *data, const lmz_lm_settings &set,
-                              bool fixed_ab, double *params, hipStream_t s) {
+                              const MieSource &src, double *params,
+                              hipStream_t s) {
     const lmz_batch *b = plan->batch;
     float *coef = reinterpret_cast<float *>(ws + L.coef);
     float *dcoef = reinterpret_cast<float *>(ws + L.dcoef);
@@ -290,6 +301,8 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
         fd.nmax = reinterpret_cast<int32_t *>(ws + L.fd_nmax);
         fd.coef = reinterpret_cast<float *>(ws + L.fd_coef);
         fd.coef_stride = (long long)LMZ_COEF_STRIDE * b->coef_slots;
+        fd.layers = src.layers;
+        fd.nlayers = src.nlayers;
         int vary = 0;
         for (int i = 0; i < LMZ_NJAC; ++i) vary |= set.vary[i] ? (1 << i) : 0;
         launch_fd_jacobian(b, params, sel, 0, nmax, coef, fd, vary, coords,
@@ -301,7 +314,10 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
     }
     hipLaunchKernelGGL(k_lm_solve, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
                        partials, st, params, ptrial, mask_j, mask_f, set);
-    if (!fixed_ab)
+    if (src.layers)
+        launch_mie_layered(b, ptrial, sel, 1, src.layers, src.nlayers, nmax,
+                           coef, dcoef, work, mask_f, s);
+    else if (!src.fixed_ab)
         launch_mie(b, ptrial, sel, 1, nmax, coef, dcoef, nullptr, work, mask_f, s);
     launch_tile_list(b, mask_f, tl_f, s);
     launch_residual(b, ptrial, sel, 1, nmax, coef, coords, flags, data,
@@ -358,6 +374,7 @@ int lmz_lm_plan_destroy(lmz_lm_plan *p) {
 static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
                   const float *data, const lmz_lm_settings *settings,
                   const double *ab_fixed, const int32_t *nmax_fixed,
+                  const double *layers, const int32_t *nlayers,
                   double *params_io, double *result, void *workspace,
                   hipStream_t stream) {
     LMZ_REQUIRE(plan != nullptr && settings != nullptr, "NULL plan or settings");
@@ -389,6 +406,9 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
     LMZ_REQUIRE(!fixed_ab || (!set.vary[3] && !set.vary[4] && !set.vary[5]),
                 "a_p, n_p and k_p must be held fixed with fixed coefficients");
     LMZ_REQUIRE(!fixed_ab || nmax_fixed, "nmax is required with ab");
+    LMZ_REQUIRE(!layers || (nlayers && !fixed_ab),
+                "nlayers is required with layers, and excludes ab");
+    const MieSource src{fixed_ab, layers, nlayers};
     LMZ_REQUIRE((reinterpret_cast<uintptr_t>(workspace) & 255) == 0,
                 "workspace must be 256-byte aligned");
 
@@ -421,6 +441,9 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
         hipLaunchKernelGGL(k_lm_load_ab, dim3((b->nfeat + 63) / 64), dim3(64), 0, s,
                            b->d_feat, b->nfeat, ab_fixed, nmax_fixed, nmax, coef,
                            dcoef);
+    else if (layers)
+        launch_mie_layered(b, params_io, sel, 0, layers, nlayers, nmax, coef,
+                           dcoef, work, nullptr, s);
     else
         launch_mie(b, params_io, sel, 0, nmax, coef, dcoef, nullptr, work, nullptr, s);
     launch_residual(b, params_io, sel, 0, nmax, coef, coords, flags, data,
@@ -432,7 +455,8 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
     LMZ_HIP(hipGetLastError());
 
     if (set.use_graph) {
-        const void *key[5] = {coords, data, params_io, workspace, result};
+        const void *key[7] = {coords, data, params_io, workspace, result,
+                              layers, nlayers};
         const bool same = plan->exec && std::memcmp(key, plan->key_ptrs, sizeof key) == 0 &&
                           plan->key_flags == flags &&
                           plan->key_fixed == (int)fixed_ab &&
@@ -442,7 +466,7 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
         if (!same) {
             plan_release_graph(plan);
             LMZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            enqueue_iteration(plan, L, ws, coords, flags, data, set, fixed_ab,
+            enqueue_iteration(plan, L, ws, coords, flags, data, set, src,
                               params_io, s);
             hipError_t e = hipStreamEndCapture(s, &plan->graph);
             if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
@@ -458,7 +482,7 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
         if (set.use_graph) {
             LMZ_HIP(hipGraphLaunch(plan->exec, s));
         } else {
-            enqueue_iteration(plan, L, ws, coords, flags, data, set, fixed_ab,
+            enqueue_iteration(plan, L, ws, coords, flags, data, set, src,
                               params_io, s);
             LMZ_HIP(hipGetLastError());
         }
@@ -482,7 +506,7 @@ int lmz_lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
                double *params_io, double *result, void *workspace,
                hipStream_t stream) {
     return lm_run(plan, coords, flags, data, settings, nullptr, nullptr,
-                  params_io, result, workspace, stream);
+                  nullptr, nullptr, params_io, result, workspace, stream);
 }
 
 int lmz_lm_run_fixed_ab(lmz_lm_plan *plan, const float *coords, int32_t flags,
@@ -491,8 +515,18 @@ int lmz_lm_run_fixed_ab(lmz_lm_plan *plan, const float *coords, int32_t flags,
                         double *params_io, double *result, void *workspace,
                         hipStream_t stream) {
     LMZ_REQUIRE(ab != nullptr, "ab is NULL");
-    return lm_run(plan, coords, flags, data, settings, ab, nmax, params_io,
-                  result, workspace, stream);
+    return lm_run(plan, coords, flags, data, settings, ab, nmax, nullptr,
+                  nullptr, params_io, result, workspace, stream);
+}
+
+int lmz_lm_run_layered(lmz_lm_plan *plan, const float *coords, int32_t flags,
+                       const float *data, const lmz_lm_settings *settings,
+                       const double *layers, const int32_t *nlayers,
+                       double *params_io, double *result, void *workspace,
+                       hipStream_t stream) {
+    LMZ_REQUIRE(layers != nullptr, "layers is NULL");
+    return lm_run(plan, coords, flags, data, settings, nullptr, nullptr,
+                  layers, nlayers, params_io, result, workspace, stream);
 }
 
 }  // extern "C"

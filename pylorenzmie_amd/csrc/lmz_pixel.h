@@ -682,7 +682,7 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
 // Mie coefficients in FP64 with a_p, n_p, k_p tangents
 // ====================================================================
 // doubles of scratch per order and feature (lmz_batch_sizes.work_doubles)
-#define LMZ_WORK_PER_ORDER 14
+#define LMZ_WORK_PER_ORDER 16
 // Sphere.wiscombe_yang, theory/Sphere.py:91-123 (one layer: the rolled
 // term equals |x m|)
 LMZ_HD int wiscombe_yang(double x, cd m) {
@@ -1042,6 +1042,168 @@ LMZ_HD int mie_layered_feature(const double *radius, const double *nl,
     }
 #undef W
     return N;
+}
+
+// ====================================================================
+// Layered spheres: tangents of a_n, b_n for fitting
+// ====================================================================
+// Parametrization: the outer radius a_p scales every radius (fixed radius
+// fractions r_l / a_p), and n_p + i k_p is the outer layer's index; inner
+// indices are held.  a_n, b_n are holomorphic in the outer m_L, so the k_p
+// tangent is i times the n_p one.  The tangents are forward-mode
+// derivatives carried through the same recursion (value and tangent of
+// every complex quantity in one direction per pass).
+struct cdl { cd v, t; };                  // complex value and tangent
+struct rdl { double v, t; };              // real value and tangent
+LMZ_HD cdl operator+(cdl a, cdl b) { return {a.v + b.v, a.t + b.t}; }
+LMZ_HD cdl operator-(cdl a, cdl b) { return {a.v - b.v, a.t - b.t}; }
+LMZ_HD cdl operator*(cdl a, cdl b) { return {a.v * b.v, a.t * b.v + a.v * b.t}; }
+LMZ_HD cdl operator*(double s, cdl a) { return {s * a.v, s * a.t}; }
+LMZ_HD cdl operator*(rdl s, cdl a) {
+    return {s.v * a.v, s.t * a.v + s.v * a.t};
+}
+LMZ_HD cdl operator/(cdl a, cdl b) {
+    const cd q = a.v / b.v;
+    return {q, (a.t - q * b.t) / b.v};
+}
+LMZ_HD cdl cinv(cdl b) {
+    const cd q = cinv(b.v);
+    return {q, -((q * q) * b.t)};
+}
+LMZ_HD cdl mul_i(cdl a) { return {mul_i(a.v), mul_i(a.t)}; }
+LMZ_HD cdl cexp_i2(cdl z) {               // exp(2 i z), tangent 2 i exp(2 i z) z'
+    const cd e = cexp_i2(z.v);
+    return {e, mul_i(2.0 * (e * z.t))};
+}
+LMZ_HD cdl ndiv(int n, cdl z) {           // n / z, as the value recursion divides
+    const cd q = cd{(double)n, 0.0} / z.v;
+    return {q, -((q / z.v) * z.t)};
+}
+
+// One tangent direction (dir 0: a_p, 1: n_p) of the layered coefficients
+// at n_max N (frozen at the value pass's), into the dcoef rows of orders
+// 1..N: direction 0 at offset 0, direction 1 at 9 and the k_p rows at 18.
+// Work: 16 doubles per order (value and tangent of H_a, H_b, and the two
+// D1 chains of a layer).
+LMZ_HD void mie_layered_tangent(const double *radius, const double *nl,
+                                const double *kl, int L, double n_m,
+                                double wavelength, int dir, int N,
+                                double *work, long long wstride,
+                                float *dcoef) {
+    const double kum = 2.0 * M_PI / wavelength * n_m;
+    const double aL = radius[L - 1];
+    rdl x[LMZ_MAX_LAYERS];
+    cdl m[LMZ_MAX_LAYERS];
+    for (int l = 0; l < L; ++l) {
+        x[l] = rdl{kum * radius[l], dir == 0 ? kum * radius[l] / aL : 0.0};
+        m[l] = cdl{cd{nl[l] / n_m, kl[l] / n_m},
+                   cd{(dir == 1 && l == L - 1) ? 1.0 / n_m : 0.0, 0.0}};
+    }
+    auto xm = [](rdl a, cdl b) { return a * b; };
+#define W(n, j) work[((long long)(n) * LMZ_WORK_PER_ORDER + (j)) * wstride]
+    auto put = [&](int n, int c, cdl z) {
+        W(n, c) = z.v.re; W(n, c + 1) = z.v.im;
+        W(n, c + 2) = z.t.re; W(n, c + 3) = z.t.im;
+    };
+    auto get = [&](int n, int c) {
+        return cdl{cd{W(n, c), W(n, c + 1)}, cd{W(n, c + 2), W(n, c + 3)}};
+    };
+    auto d1_down = [&](cdl z, int c) {
+        cdl d1{cd{0.0, 0.0}, cd{0.0, 0.0}};
+        put(N, c, d1);
+        for (int n = N; n >= 1; --n) {
+            const cdl nz = ndiv(n, z);
+            d1 = nz - cinv(d1 + nz);
+            put(n - 1, c, d1);
+        }
+    };
+    // core: H_a = H_b = D1(m_1 x_1) in columns 0..3 and 4..7
+    d1_down(xm(x[0], m[0]), 0);
+    for (int n = 0; n <= N; ++n) put(n, 4, get(n, 0));
+    const cdl one{cd{1.0, 0.0}, cd{0.0, 0.0}};
+    for (int l = 1; l < L; ++l) {
+        const cdl za = xm(x[l - 1], m[l]), zb = xm(x[l], m[l]);
+        d1_down(za, 8);
+        d1_down(zb, 12);
+        const cdl iza = cinv(za), izb = cinv(zb);
+        const cdl ea = cexp_i2(za), eb = cexp_i2(zb);
+        cdl pza = 0.5 * (one - ea), pzb = 0.5 * (one - eb);
+        cdl d3a{cd{0.0, 1.0}, cd{0.0, 0.0}}, d3b = d3a;
+        cdl q = cexp_i2(zb - za) * (ea - one) / (eb - one);
+        const double rr = x[l - 1].v / x[l].v;
+        const rdl r2{rr * rr, 2.0 * rr * (x[l - 1].t * x[l].v - x[l - 1].v * x[l].t) /
+                                   (x[l].v * x[l].v)};
+        for (int n = 1; n <= N; ++n) {
+            const cdl d1a_m = get(n - 1, 8), d1b_m = get(n - 1, 12);
+            const cdl d1a = get(n, 8), d1b = get(n, 12);
+            const cdl nza = (double)n * iza, nzb = (double)n * izb;
+            const cdl nn{cd{(double)n, 0.0}, cd{0.0, 0.0}};
+            const cdl num = (zb * d1b + nn) * (nn - zb * d3b);
+            const cdl den = (za * d1a + nn) * (nn - za * d3a);
+            q = (r2 * q) * num / den;
+            pza = pza * ((nza - d1a_m) * (nza - d3a));
+            pzb = pzb * ((nzb - d1b_m) * (nzb - d3b));
+            d3a = d1a + mul_i(cinv(pza));
+            d3b = d1b + mul_i(cinv(pzb));
+            const cdl ha = get(n, 0), hb = get(n, 4);
+            cdl g1 = m[l] * ha - m[l - 1] * d1a;
+            cdl g2 = m[l] * ha - m[l - 1] * d3a;
+            const cdl han = (g2 * d1b - q * g1 * d3b) / (g2 - q * g1);
+            g1 = m[l - 1] * hb - m[l] * d1a;
+            g2 = m[l - 1] * hb - m[l] * d3a;
+            const cdl hbn = (g2 * d1b - q * g1 * d3b) / (g2 - q * g1);
+            put(n, 0, han);
+            put(n, 4, hbn);
+        }
+    }
+    // outer boundary: D1(x_L) downward (real) in columns 8, 9
+    const rdl xL = x[L - 1];
+    {
+        rdl r1{0.0, 0.0};
+        W(N, 8) = 0.0; W(N, 9) = 0.0;
+        for (int n = N; n >= 1; --n) {
+            const rdl nx{n / xL.v, -n * xL.t / (xL.v * xL.v)};
+            const double den = r1.v + nx.v;
+            r1 = rdl{nx.v - 1.0 / den, nx.t + (r1.t + nx.t) / (den * den)};
+            W(n - 1, 8) = r1.v; W(n - 1, 9) = r1.t;
+        }
+    }
+    double sx, cx;
+    sincos(xL.v, &sx, &cx);
+    double s2x, c2x;
+    sincos(2.0 * xL.v, &s2x, &c2x);
+    rdl psi{sx, cx * xL.t};
+    cdl zeta{cd{sx, -cx}, cd{cx * xL.t, sx * xL.t}};           // -i e^{ix}
+    cdl pz{cd{0.5 * (1.0 - c2x), -0.5 * s2x}, cd{s2x * xL.t, -c2x * xL.t}};
+    cdl d3{cd{0.0, 1.0}, cd{0.0, 0.0}};
+    const cdl mL = m[L - 1], imL = cinv(mL);
+    for (int n = 1; n <= N; ++n) {
+        const rdl nx{n / xL.v, -n * xL.t / (xL.v * xL.v)};
+        const rdl w1{nx.v - W(n - 1, 8), nx.t - W(n - 1, 9)};
+        const cdl w3 = cdl{cd{nx.v, 0.0}, cd{nx.t, 0.0}} - d3;
+        const rdl psi_m = psi;
+        const cdl zeta_m = zeta;
+        psi = rdl{psi_m.v * w1.v, psi_m.t * w1.v + psi_m.v * w1.t};
+        zeta = zeta_m * w3;
+        pz = pz * (w1 * w3);
+        d3 = cdl{cd{W(n, 8), 0.0}, cd{W(n, 9), 0.0}} + mul_i(cinv(pz));
+        const cdl ha = get(n, 0), hb = get(n, 4);
+        const cdl cnx{cd{nx.v, 0.0}, cd{nx.t, 0.0}};
+        const cdl cpsi{cd{psi.v, 0.0}, cd{psi.t, 0.0}};
+        const cdl cpsi_m{cd{psi_m.v, 0.0}, cd{psi_m.t, 0.0}};
+        const cdl fa = ha * imL + cnx;
+        const cdl a = (fa * cpsi - cpsi_m) / (fa * zeta - zeta_m);
+        const cdl fb = mL * hb + cnx;
+        const cdl b = (fb * cpsi - cpsi_m) / (fb * zeta - zeta_m);
+        float *drow = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
+        if (dir == 0) {
+            write_dorder(n, a.t, b.t, drow);
+        } else {
+            write_dorder(n, a.t, b.t, drow + 9);
+            write_dorder_k(n, mul_i(a.t), mul_i(b.t), drow + 18);
+        }
+    }
+#undef W
 }
 
 }  // namespace lmz

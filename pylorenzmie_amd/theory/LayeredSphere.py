@@ -91,9 +91,15 @@ def layer_table(layers_list) -> tuple[np.ndarray, np.ndarray]:
     return table, counts
 
 
-def layered_cap(layers, n_m: float, wavelength: float) -> int:
-    """Orders to reserve for one layered particle ((L, 3) rows)."""
-    lay = np.asarray(layers, float).reshape(-1, 3)
+def layered_cap(layers, n_m: float, wavelength: float,
+                headroom: float = 0.) -> int:
+    """Orders to reserve for one layered particle ((L, 3) rows); with
+    ``headroom`` for a fit whose radii may grow by that fraction and whose
+    outer index may grow by 0.1."""
+    lay = np.asarray(layers, float).reshape(-1, 3).copy()
+    if headroom:
+        lay[:, 0] *= 1. + headroom
+        lay[-1, 1] += 0.1
     k = 2. * np.pi / wavelength * n_m
     return LayeredSphere.wiscombe_yang(k * lay[:, 0],
                                        (lay[:, 1] + 1j * lay[:, 2]) / n_m) + 2

@@ -32,6 +32,22 @@ int chk_mie_layered(const double *radius, const double *nl, const double *kl,
                                work.data(), 1, nullptr, ab);
 }
 
+// Tangent rows (dcoef, LMZ_DCOEF_STRIDE floats per order) of a layered
+// sphere's coefficients in the fitting parametrization (outer radius
+// scaling every radius, outer index n + i k); returns n_max.
+int chk_mie_layered_tangent(const double *radius, const double *nl,
+                            const double *kl, int L, double n_m,
+                            double wavelength, int cap, float *dcoef) {
+    std::vector<double> work(LMZ_WORK_PER_ORDER * (cap + 1));
+    std::vector<double> ab(4 * (cap + 1));
+    const int N = mie_layered_feature(radius, nl, kl, L, n_m, wavelength, cap,
+                                      work.data(), 1, nullptr, ab.data());
+    for (int dir = 0; dir < 2; ++dir)
+        mie_layered_tangent(radius, nl, kl, L, n_m, wavelength, dir, N,
+                            work.data(), 1, dcoef);
+    return N;
+}
+
 // Field (flags as in lmz.h), hologram and Jacobian of one feature at
 // explicit coordinates (x, y, z arrays relative to feat->origin).
 void chk_pixels(const lmz_feature *feat, const double *params,

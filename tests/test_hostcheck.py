@@ -28,6 +28,10 @@ def chk():
     lib.chk_mie_layered.restype = ctypes.c_int
     lib.chk_mie_layered.argtypes = [P, P, P, ctypes.c_int, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_int, P]
+    lib.chk_mie_layered_tangent.restype = ctypes.c_int
+    lib.chk_mie_layered_tangent.argtypes = [P, P, P, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_int, P]
     lib.chk_pixels.argtypes = [P, P, P, P, ctypes.c_int, P, ctypes.c_int,
                                ctypes.c_int, P, P, P]
     lib.chk_dense_coords.argtypes = [ctypes.c_int, ctypes.c_int, P, P]
@@ -226,3 +230,60 @@ def test_fp32_residual_arithmetic(chk, q, tol):
                    _p(h))
     truth = ref.hologram(c, dict(zip(ref.PARAMS, q)), **INSTRUMENT)
     assert np.abs(h - truth).max() < tol
+
+
+@pytest.mark.parametrize('seed', [0, 1, 2, 3])
+def test_layered_tangents_match_differences(chk, seed):
+    """Tangents of layered coefficients in the fitting parametrization
+    (a_p scales every radius, n_p + i k_p is the outer layer's index)
+    against central differences of the oracle's layered recursion with
+    n_max frozen; the k_p tangent against differences in k_p (it is i
+    times the n_p tangent by holomorphy in the outer index)."""
+    lib, _ = chk
+    rng = np.random.default_rng(seed)
+    L = 2 + seed
+    a_p = rng.uniform(0.5, 3.)
+    radii = np.sort(rng.uniform(0.1, 1., L)) * a_p
+    radii[-1] = a_p
+    nl = rng.uniform(1.33, 2.0, L)
+    kl = np.zeros(L)
+    kl[-1] = 0.003 * seed
+    cap = 400
+    dcoef = np.zeros((cap, DCOEF_STRIDE), np.float32)
+    N = lib.chk_mie_layered_tangent(_p(radii), _p(nl), _p(kl), L, 1.34, 0.447,
+                                    cap, _p(dcoef))
+    nn = np.arange(1, N + 1)
+    en = (2 * nn + 1) / (nn * (nn + 1))
+
+    def coeffs(r, n, k):
+        return ref.mie_coefficients_layered(r, n, k, 1.34, 0.447, nmax=N)[1:]
+    for name, base in (('a_p', 0), ('n_p', 9), ('k_p', 18)):
+        h = 1e-5
+        fd = 0
+        for sgn, w in ((1, 8 / 12), (2, -1 / 12)):
+            r_p, r_m = radii.copy(), radii.copy()
+            n_p, n_m = nl.copy(), nl.copy()
+            k_p, k_m = kl.copy(), kl.copy()
+            if name == 'a_p':
+                r_p *= (a_p + sgn * h) / a_p
+                r_m *= (a_p - sgn * h) / a_p
+            elif name == 'n_p':
+                n_p[-1] += sgn * h
+                n_m[-1] -= sgn * h
+            else:
+                k_p[-1] += sgn * h
+                k_m[-1] -= sgn * h
+            fd = fd + w * (coeffs(r_p, n_p, k_p) - coeffs(r_m, n_m, k_m))
+        fd = fd / h
+        if base < 18:
+            dP = dcoef[:N, base + 3] + 1j * dcoef[:N, base + 4]
+            dQ = -(dcoef[:N, base + 6] + 1j * dcoef[:N, base + 7])
+        else:
+            dP = dcoef[:N, 20] + 1j * dcoef[:N, 21]
+            dQ = dcoef[:N, 22] + 1j * dcoef[:N, 23]
+        da = dP / (1j ** (nn + 1) * en)
+        db = dQ / (1j ** nn * en)
+        scale = np.abs(fd).max()
+        err = max(np.abs(da - fd[:, 0]).max(), np.abs(db - fd[:, 1]).max())
+        print(name, 'layers', L, 'N', N, 'max err / scale', err / scale)
+        assert err <= 2e-6 * scale, name

@@ -283,6 +283,39 @@ def test_fit_planted_mistakes_are_caught(fits_golden, monkeypatch):
     assert r['redchi'] != pytest.approx(g['redchi'], rel=1e-9)
 
 
+def test_layered_fit_model_invariants(fits_golden):
+    """The oracle's layered fitting model (a_p scales every radius, n_p + i
+    k_p is the outer index): one layer reduces to the homogeneous sphere,
+    fit included; an inner layer of the outer index is invisible; and the
+    coefficients depend on radii only through k r, so scaling a_p and the
+    wavelength together leaves them unchanged (which fails if the inner
+    radii do not follow a_p)."""
+    from oracle import fit
+    inst = dict(INSTRUMENT)
+    p = dict(x_p=0., y_p=0., z_p=100., a_p=1.2, n_p=1.5, k_p=0.002)
+    hom = ref.mie_coefficients(1.2, 1.5, 0.002, inst['n_m'], inst['wavelength'])
+    one = fit.layered_properties(p, [[0.7, 1.9, 0.]], inst)['ab']
+    np.testing.assert_allclose(one, hom, rtol=0, atol=1e-13)
+    two = fit.layered_properties(p, [[0.3, 1.5, 0.002], [0.9, 1.7, 0.]], inst)['ab']
+    np.testing.assert_allclose(two, hom, rtol=0, atol=1e-10)
+    lay = np.array([[0.4, 1.8, 0.], [0.7, 1.35, 0.], [1.0, 1.6, 0.001]])
+    base = ref.mie_coefficients_layered(lay[:, 0], lay[:, 1], lay[:, 2],
+                                        inst['n_m'], inst['wavelength'])
+    q = dict(p, a_p=2.5, n_p=1.6, k_p=0.001)
+    scaled = fit.layered_properties(q, lay, dict(inst, wavelength=2.5 * inst['wavelength']))
+    np.testing.assert_allclose(scaled['ab'], base, rtol=0, atol=1e-12)
+    # a whole fit: the one-layer model lands where the homogeneous one does
+    z, _ = fits_golden
+    data = z['crop00_data']
+    c = ref.meshgrid(data.shape)
+    start = dict(zip(fit.VARS6, z['crop00_start']))
+    kw = dict(instrument=INSTRUMENT, noise=float(z['crop00_sigma']))
+    r0 = fit.fit(data, c, start, **kw)
+    r1 = fit.fit(data, c, start, layers=[[1., 1.5, 0.]], **kw)
+    for v in fit.VARS6:
+        assert r1[v] == pytest.approx(r0[v], rel=1e-9, abs=1e-12), v
+
+
 def _gauss_newton_step(data, c, values, noise=0.05):
     """One Gauss-Newton step of the 5-variable fit from ``values``, with a
     central-difference Jacobian of the oracle's hologram."""

@@ -1000,9 +1000,58 @@ def run_extra(args):
                     cus=props.multi_processor_count, rates=rates,
                     peak_published={'fp32': PEAK_FP32, 'fp32_unpacked': PEAK_FP32_UNPACKED,
                                     'fp64': 78.6})
+    elif args.extra == 'stages':
+        # the forward's per-pixel stages in FP32 or FP64 (variant bits as in
+        # tests/native/lmz_stages.hip): time per launch of one sphere over a
+        # 4096 x 4096 grid, one thread per pixel
+        import ctypes
+        from pylorenzmie_amd import _lmz
+        P = ctypes.c_void_p
+        lib = ctypes.CDLL(os.path.join(ROOT, 'tests', 'native', 'liblmz_stages.so'))
+        lib.stage_gpu.argtypes = [ctypes.c_int, P, P, P, ctypes.c_int, P,
+                                  ctypes.c_int, ctypes.c_int, P,
+                                  ctypes.POINTER(ctypes.c_float)]
+        chk = ctypes.CDLL(os.path.join(ROOT, 'tests', 'native', 'liblmz_hostcheck.so'))
+        chk.chk_mie.restype = ctypes.c_int
+        chk.chk_mie.argtypes = [ctypes.c_double] * 5 + [ctypes.c_int, P, P, P]
+        names = {0: 'fp64', 1: 'geometry', 2: 'angular', 4: 'xi', 8: 'products',
+                 16: 'accumulate', 32: 'epilogue', 24: 'products+accumulate',
+                 56: 'products+accumulate+epilogue', 60: 'all but geometry, angular',
+                 62: 'all but geometry', 63: 'fp32'}
+        side = 4096
+        yy, xx = np.mgrid[0:side, 0:side]
+        coords = np.ascontiguousarray(np.stack([xx.ravel(), yy.ravel(),
+                                                np.zeros(side * side)]), np.float32)
+        npix = side * side
+        out = np.zeros(npix)
+        cases = []
+        inst = Instrument(**INSTRUMENT)
+        for a_p in (0.75, 1.5, 3.0):
+            coef = np.zeros((400, 40), np.float32)
+            dcoef = np.zeros((400, 24), np.float32)
+            ab = np.zeros((401, 4))
+            N = chk.chk_mie(a_p, 1.5, 0.001, 1.34, 0.447, 400,
+                            coef.ctypes.data_as(P), dcoef.ctypes.data_as(P),
+                            ab.ctypes.data_as(P))
+            f = _lmz.make_feature(npix, 0, (0., 0., 0.), inst, 400)
+            p = np.array([2047.6, 2048.2, 200., a_p, 1.5, 0.001, 1.0, 0.])
+            ms = {}
+            for v, name in names.items():
+                t = ctypes.c_float()
+                if lib.stage_gpu(v, ctypes.byref(f), p.ctypes.data_as(P),
+                                 coef.ctypes.data_as(P), N, coords.ctypes.data_as(P),
+                                 npix, 5, out.ctypes.data_as(P), ctypes.byref(t)):
+                    raise RuntimeError('stage_gpu failed')
+                ms[name] = round(t.value, 4)
+            cases.append(dict(a_p=a_p, nmax=N, ms=ms,
+                              speed_vs_fp64={k: round(ms['fp64'] / v, 3)
+                                             for k, v in ms.items()}))
+        line.update(workload=f'one sphere (n_p 1.5, z_p 200 px) over a {side} x '
+                             f'{side} grid, one thread per pixel; the stages named '
+                             'run in FP32, the rest in FP64', cases=cases)
     else:
         raise SystemExit('--extra must be superposed, frame, de, batch, '
-                         'normalize or valu_peak')
+                         'normalize, valu_peak or stages')
     print(json.dumps(line), flush=True)
 
 
@@ -1054,7 +1103,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--seed', type=int, default=3)
     ap.add_argument('--extra', default='',
-                    help='superposed | frame | de | batch | normalize | valu_peak: measure that path instead '
+                    help='superposed | frame | de | batch | normalize | valu_peak | stages: measure that path instead '
                          'of the headline (one JSON line)')
     ap.add_argument('--config', type=int, default=0,
                     help='1-4: measure that configuration instead of the '

@@ -35,16 +35,44 @@ int check_batch(const lmz_batch *b) {
 // kernels
 // ====================================================================
 
-// One wavefront per feature: lanes 0 and 1 run the two downward D1 chains,
-// lane 0 the upward chain, then all 64 lanes the per-order coefficients and
-// tangents (orders lane + 1, lane + 65, ...).  The scratch of feature f is
+// The Mie coefficients in two launches.  First the three sequential chains
+// (D1(m x) downward; D1(x) downward, then the upward chain, which needs
+// D1(x) only), one lane per feature and chain: blockIdx.y picks the chain
+// group, so 64 features share each wavefront.  A wavefront per feature with
+// one or two busy lanes would issue every FP64 instruction of the chains
+// for the whole wavefront.  (The complex divisions branch on |Re| >= |Im|
+// and diverge across features; a branch-free form changed the last bits of
+// a_n, which moved fits on crops with neighbouring minima, so the chains
+// keep the division the rest of the library uses.)  Then the per-order
+// coefficients and tangents, one wavefront per feature with the orders
+// across its lanes (lane + 1, lane + 65, ...).  The scratch of feature f is
 // the contiguous block work + f * LMZ_WORK_PER_ORDER * (cap_max + 1).
+__global__ __launch_bounds__(64) void k_mie_chains(
+    const lmz_feature *__restrict__ feat, int nfeat, int cap_max,
+    const double *__restrict__ params, double *__restrict__ work,
+    const uint8_t *__restrict__ active) {
+    const int f = blockIdx.x * 64 + threadIdx.x;
+    if (f >= nfeat) return;
+    if (active && !active[f]) return;
+    const lmz_feature F = feat[f];
+    const double *p = params + (long long)LMZ_NPARAM * f;
+    const MieSetup ms = mie_setup(p[3], p[4], p[5], F.n_m, F.wavelength,
+                                  F.nmax_cap);
+    double *w = work + (long long)f * LMZ_WORK_PER_ORDER * (cap_max + 1);
+    if (blockIdx.y == 0) {
+        mie_d1_down(ms.x * ms.m, ms.N, w, 1, 0);
+    } else {
+        mie_d1_down(cd{ms.x, 0.0}, ms.N, w, 1, 4);
+        mie_up_chain(ms.x, ms.N, w, 1);
+    }
+}
+
 __global__ __launch_bounds__(64) void k_mie(
     const lmz_feature *__restrict__ feat, int nfeat, int cap_max,
     const double *__restrict__ params, const int32_t *__restrict__ sel,
     int flip, int nbuf, int32_t *__restrict__ nmax_out,
     float *__restrict__ coef, float *__restrict__ dcoef,
-    double *__restrict__ ab_out, double *__restrict__ work,
+    double *__restrict__ ab_out, const double *__restrict__ work,
     const uint8_t *__restrict__ active) {
     const int f = blockIdx.x;
     if (f >= nfeat) return;
@@ -56,12 +84,7 @@ __global__ __launch_bounds__(64) void k_mie(
     const long long slot0 = F.coef_offset + (long long)b * F.nmax_cap;
     const MieSetup ms = mie_setup(p[3], p[4], p[5], F.n_m, F.wavelength,
                                   F.nmax_cap);
-    double *w = work + (long long)f * LMZ_WORK_PER_ORDER * (cap_max + 1);
-    if (lane < 2)
-        mie_d1_down(lane ? cd{ms.x, 0.0} : ms.x * ms.m, ms.N, w, 1, 4 * lane);
-    __syncthreads();
-    if (lane == 0) mie_up_chain(ms.x, ms.N, w, 1);
-    __syncthreads();
+    const double *w = work + (long long)f * LMZ_WORK_PER_ORDER * (cap_max + 1);
     for (int n = lane + 1; n <= ms.N; n += 64)
         mie_order(ms, n, w, 1, coef ? coef + LMZ_COEF_STRIDE * slot0 : nullptr,
                   dcoef ? dcoef + LMZ_DCOEF_STRIDE * slot0 : nullptr,
@@ -175,7 +198,7 @@ __global__ __launch_bounds__(64) void k_coef_from_ab(
 
 // Next entry of a work list for the whole block.  The list (length
 // count[0]) is split into LMZ_XCDS contiguous chunks, chunk x fetched
-// through counter count[2 + x] (set to 0 by k_tile_list).  A block starts on
+// through counter count[2 + x] (set to 0 by k_tile_starts).  A block starts on
 // the chunk of its own XCD, so the tiles of one feature stay in one L2, and
 // moves on to the other chunks when its own is exhausted.  `k` (thread 0's
 // count of exhausted chunks, initially 0) only grows, so every block
@@ -707,6 +730,9 @@ void launch_mie(const lmz_batch *b, const double *params, const int32_t *sel,
                 int flip, int32_t *nmax, float *coef, float *dcoef,
                 double *ab_out, double *work, const uint8_t *active,
                 hipStream_t stream) {
+    hipLaunchKernelGGL(k_mie_chains, dim3((b->nfeat + 63) / 64, 2), dim3(64), 0,
+                       stream, b->d_feat, b->nfeat, b->cap_max, params, work,
+                       active);
     hipLaunchKernelGGL(k_mie, dim3(b->nfeat), dim3(64), 0, stream, b->d_feat,
                        b->nfeat, b->cap_max, params, sel, flip, b->coef_buffers,
                        nmax, coef, dcoef, ab_out, work, active);
@@ -835,11 +861,16 @@ void launch_fd_jacobian(const lmz_batch *b, const double *params,
     else go(k_fd_jacobian<256>, 256);
 }
 
-// Compacted list of the tiles of the features whose mask is set, and its
-// length: one block, features split into contiguous runs per thread.
-__global__ __launch_bounds__(1024) void k_tile_list(
+// Compacted list of the tiles of the features whose mask is set, in feature
+// order, and its length, in two launches.  k_tile_starts (one block):
+// thread t counts the tiles of features [t per, (t + 1) per), a block scan
+// gives each feature's first list position (start[f]) and the length.
+// k_tile_fill: one wavefront per feature writes its tile range, the tiles
+// across the lanes, so the stores are contiguous and every feature's range
+// is written in parallel.
+__global__ __launch_bounds__(1024) void k_tile_starts(
     const lmz_feature *__restrict__ feat, int nfeat,
-    const uint8_t *__restrict__ mask, int32_t *__restrict__ list,
+    const uint8_t *__restrict__ mask, int32_t *__restrict__ start,
     int32_t *__restrict__ count) {
     __shared__ int scan[1024];
     const int tid = threadIdx.x;
@@ -858,18 +889,29 @@ __global__ __launch_bounds__(1024) void k_tile_list(
     }
     int pos = scan[tid] - mine;
     for (int f = f0; f < f1; ++f) {
-        if (!mask[f]) continue;
-        const int t0 = feat[f].tile_offset, nt = feat[f].ntiles;
-        for (int i = 0; i < nt; ++i) list[pos++] = t0 + i;
+        start[f] = pos;
+        if (mask[f]) pos += feat[f].ntiles;
     }
     if (tid == 1023) count[0] = scan[1023];
     if (tid < LMZ_XCDS) count[2 + tid] = 0;   // fetch counters (next_tile)
 }
 
+__global__ __launch_bounds__(256) void k_tile_fill(
+    const lmz_feature *__restrict__ feat, int nfeat,
+    const uint8_t *__restrict__ mask, const int32_t *__restrict__ start,
+    int32_t *__restrict__ list) {
+    const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= nfeat || !mask[f]) return;
+    const int s0 = start[f], t0 = feat[f].tile_offset, nt = feat[f].ntiles;
+    for (int i = threadIdx.x & 63; i < nt; i += 64) list[s0 + i] = t0 + i;
+}
+
 void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
                       const TileList &tl, hipStream_t stream) {
-    hipLaunchKernelGGL(k_tile_list, dim3(1), dim3(1024), 0, stream, b->d_feat,
-                       b->nfeat, mask, tl.list, tl.count);
+    hipLaunchKernelGGL(k_tile_starts, dim3(1), dim3(1024), 0, stream, b->d_feat,
+                       b->nfeat, mask, tl.start, tl.count);
+    hipLaunchKernelGGL(k_tile_fill, dim3((b->nfeat + 3) / 4), dim3(256), 0, stream,
+                       b->d_feat, b->nfeat, mask, tl.start, tl.list);
 }
 
 // Workgroups of one pixel kernel resident per CU at once.
@@ -919,7 +961,7 @@ __global__ void k_mask_every(uint8_t *__restrict__ mask, int n, int stride) {
 }
 
 struct TuneScratch {
-    size_t mask, list, count, partials, chi2, total;
+    size_t mask, list, start, count, partials, chi2, total;
 };
 
 static TuneScratch tune_layout(const lmz_batch *b) {
@@ -932,6 +974,7 @@ static TuneScratch tune_layout(const lmz_batch *b) {
     };
     L.mask = take((size_t)b->nfeat);
     L.list = take(sizeof(int32_t) * (size_t)b->ntiles);
+    L.start = take(sizeof(int32_t) * (size_t)b->nfeat);
     L.count = take(sizeof(int32_t) * 16);
     L.partials = take(sizeof(double) * LMZ_NRED * (size_t)b->ntiles);
     L.chi2 = take(sizeof(double) * (size_t)b->ntiles);
@@ -1376,7 +1419,8 @@ int lmz_autotune(lmz_batch *b, const double *params, const int32_t *nmax,
     char *ws = static_cast<char *>(scratch);
     uint8_t *mask = reinterpret_cast<uint8_t *>(ws + L.mask);
     TileList tl{reinterpret_cast<int32_t *>(ws + L.list),
-                reinterpret_cast<int32_t *>(ws + L.count), 0};
+                reinterpret_cast<int32_t *>(ws + L.count), 0,
+                reinterpret_cast<int32_t *>(ws + L.start)};
     double *partials = reinterpret_cast<double *>(ws + L.partials);
     double *chi2 = reinterpret_cast<double *>(ws + L.chi2);
     hipEvent_t e0 = nullptr, e1 = nullptr;

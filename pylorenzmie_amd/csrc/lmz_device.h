@@ -35,6 +35,7 @@ struct TileList {
     int32_t *count;   // device, [2 + LMZ_XCDS]: list length, unused,
                       // per-XCD fetch counters
     int grid;         // blocks of the persistent launch
+    int32_t *start;   // device, [nfeat]: each feature's first list position
 };
 
 void launch_mie(const lmz_batch *b, const double *params, const int32_t *sel,

@@ -24,7 +24,8 @@ namespace lmz {
 
 struct LMLayout {
     size_t coef, dcoef, nmax, sel, work, partials, chi2_tiles, state, ptrial,
-        mask_j, mask_f, count, resid, list_j, list_f, fd_h, fd_params, fd_nmax,
+        mask_j, mask_f, count, resid, list_j, list_f, tile_start, fd_h,
+        fd_params, fd_nmax,
         fd_coef, total;
 };
 
@@ -50,6 +51,7 @@ static LMLayout lm_layout(const lmz_batch *b) {
     L.resid = take(sizeof(float) * 2 * (size_t)b->npix_total);
     L.list_j = take(sizeof(int32_t) * (size_t)b->ntiles);
     L.list_f = take(sizeof(int32_t) * (size_t)b->ntiles);
+    L.tile_start = take(sizeof(int32_t) * (size_t)b->nfeat);   // both lists
     // forward-difference Jacobian (settings.fd_step > 0)
     L.fd_h = take(sizeof(double) * LMZ_NPARAM * (size_t)b->nfeat);
     L.fd_params = take(sizeof(double) * 3 * LMZ_NPARAM * (size_t)b->nfeat);
@@ -286,10 +288,11 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
     int32_t *count = reinterpret_cast<int32_t *>(ws + L.count);
     float *resid = reinterpret_cast<float *>(ws + L.resid);
     const long long np = b->npix_total;
+    int32_t *tile_start = reinterpret_cast<int32_t *>(ws + L.tile_start);
     const TileList tl_j{reinterpret_cast<int32_t *>(ws + L.list_j), count + 16,
-                        plan->grid_j};
+                        plan->grid_j, tile_start};
     const TileList tl_f{reinterpret_cast<int32_t *>(ws + L.list_f), count + 32,
-                        plan->grid_f};
+                        plan->grid_f, tile_start};
     // J^T J and J^T r at the current point, r from its FP64 forward pass,
     // over the tiles of the features that accepted their last step
     launch_tile_list(b, mask_j, tl_j, s);

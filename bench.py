@@ -20,7 +20,12 @@ Also measured:
   roofline  the Jacobian kernel (dominant) and the trial-forward kernel,
           timed with HIP events on the stream they run on, against the FP32
           VALU peak, with algorithmic FLOP counts per pixel of 296 N + 480
-          (Jacobian) and 64 N + 70 (forward), N the order count;
+          (Jacobian) and 64 N + 70 (forward), N the order count; and
+          against the attainable rates of this build, measured in the
+          same run by the FMA-chain probe (tests/native/lmz_valu_peak.hip):
+          the kernel's counted FP32 FLOP and VALU instructions per launch
+          (from the PMC summary committed for these sources) over the
+          probe's two-operand FP32 FMA rate;
   cpu_baseline  the FP64 oracle (numpy model + scipy MINPACK LM) on the
           host's cores (one process per core, two features each, at most
           16 cores), started as a child process that never touches the
@@ -147,15 +152,76 @@ def run_batched(args, truth, start, data_dev):
         lm.run(settings, PARAMS)
         return lm.fetch()
 
+    mark = roctx_marker() if args.mark else None
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
+        if mark:
+            mark.push(f'lm step {i}')
         params, res = step()
+        if mark:
+            mark.pop()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return lm, dt, params, res, settings, p0
+
+
+class roctx_marker:
+    """roctx ranges (librocprofiler-sdk-roctx) around the timed steps, so
+    that a marker trace shows which kernels each step ran and its wall
+    time."""
+
+    def __init__(self):
+        import ctypes
+        self._lib = ctypes.CDLL('librocprofiler-sdk-roctx.so.1')
+        self._lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+        self._lib.roctxRangePushA.restype = ctypes.c_int
+        self._lib.roctxRangePop.restype = ctypes.c_int
+
+    def push(self, name: str) -> None:
+        self._lib.roctxRangePushA(name.encode())
+
+    def pop(self) -> None:
+        self._lib.roctxRangePop()
+
+
+def valu_probe(reps: int = 5) -> dict:
+    """Attainable vector-ALU issue rates of code built with the library's
+    flags (tests/native/lmz_valu_peak.hip): 16 independent chains per
+    thread of FMA with uniform operands (VOP3 encoding), FMA with per-lane
+    operands (the two-operand VOP2 v_fmac form), multiply and add, in FP32
+    and FP64, over 8 workgroups of 256 threads per CU; the best of reps
+    launches of each."""
+    import ctypes
+    import torch
+    lib = ctypes.CDLL(os.path.join(ROOT, 'tests', 'native', 'liblmz_valu_peak.so'))
+    lib.valu_peak.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_void_p,
+                              ctypes.POINTER(ctypes.c_float)]
+    props = torch.cuda.get_device_properties(0)
+    blocks, iters = props.multi_processor_count * 8, 4096
+    chains, nt = lib.valu_peak_chains(), lib.valu_peak_threads()
+    rates = {}
+    ops = (('fma_uniform', 2.), ('fma_lane', 2.), ('mul', 1.), ('add', 1.))
+    for dtype, name, tdt in ((0, 'fp32', torch.float32), (1, 'fp64', torch.float64)):
+        out = torch.empty(blocks * nt, dtype=tdt, device='cuda')
+        for op, (opname, fl) in enumerate(ops):
+            best = None
+            for _ in range(reps):
+                ms = ctypes.c_float()
+                if lib.valu_peak(dtype, op, blocks, iters,
+                                 ctypes.c_void_p(out.data_ptr()), ctypes.byref(ms)):
+                    raise RuntimeError('valu_peak launch failed')
+                best = ms.value if best is None else min(best, ms.value)
+            inst = float(blocks * nt) * iters * chains
+            rates[f'{name}_{opname}'] = {
+                'lane_instructions_per_s': inst / (best * 1e-3),
+                'tflops': fl * inst / (best * 1e-3) / 1e12, 'ms': best}
+    return {'workload': f'{blocks} workgroups of {nt} threads, {chains} '
+                        f'independent FMA chains x {iters} per thread',
+            'rates': rates}
 
 
 def kernel_times(lm, settings, args):
@@ -970,34 +1036,10 @@ def run_extra(args):
     elif args.extra == 'valu_peak':
         # the attainable FP32 and FP64 FMA rates of code built with the
         # library's flags: independent FMA chains in registers
-        import ctypes
-        lib = ctypes.CDLL(os.path.join(ROOT, 'tests', 'native',
-                                       'liblmz_valu_peak.so'))
-        lib.valu_peak.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                  ctypes.c_int, ctypes.c_void_p,
-                                  ctypes.POINTER(ctypes.c_float)]
         props = torch.cuda.get_device_properties(0)
-        blocks, iters = props.multi_processor_count * 8, 4096
-        chains, nt = lib.valu_peak_chains(), lib.valu_peak_threads()
-        rates = {}
-        ops = (('fma_uniform', 2.), ('fma_lane', 2.), ('mul', 1.), ('add', 1.))
-        for dtype, name, tdt in ((0, 'fp32', torch.float32), (1, 'fp64', torch.float64)):
-            out = torch.empty(blocks * nt, dtype=tdt, device='cuda')
-            for op, (opname, fl) in enumerate(ops):
-                best = None
-                for _ in range(5):
-                    ms = ctypes.c_float()
-                    if lib.valu_peak(dtype, op, blocks, iters,
-                                     ctypes.c_void_p(out.data_ptr()), ctypes.byref(ms)):
-                        raise RuntimeError('valu_peak launch failed')
-                    best = ms.value if best is None else min(best, ms.value)
-                inst = float(blocks * nt) * iters * chains
-                rates[f'{name}_{opname}'] = {
-                    'lane_instructions_per_s': inst / (best * 1e-3),
-                    'tflops': fl * inst / (best * 1e-3) / 1e12, 'ms': best}
-        line.update(workload=f'{blocks} workgroups of {nt} threads, {chains} '
-                             f'independent FMA chains x {iters} per thread',
-                    cus=props.multi_processor_count, rates=rates,
+        probe = valu_probe()
+        line.update(workload=probe.pop('workload'),
+                    cus=props.multi_processor_count, rates=probe['rates'],
                     peak_published={'fp32': PEAK_FP32, 'fp32_unpacked': PEAK_FP32_UNPACKED,
                                     'fp64': 78.6})
     elif args.extra == 'stages':
@@ -1099,6 +1141,9 @@ def main():
                     'start, fitted parameters and LM results to this .npz')
     ap.add_argument('--no-tune', action='store_true',
                     help='keep the default launch schedule (no autotuner)')
+    ap.add_argument('--mark', action='store_true',
+                    help='roctx range around each timed step (for '
+                         'rocprofv3 --marker-trace)')
     ap.add_argument('--no-fd', action='store_true')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--seed', type=int, default=3)
@@ -1192,6 +1237,18 @@ def main():
                                args.nfeat, args.side)
     counted = committed_figure('valu_counters', 'counted_fp32_flop_per_launch',
                                args.nfeat, args.side)
+    lane_inst = committed_figure('valu_counters', 'valu_lane_instructions_per_launch',
+                                 args.nfeat, args.side)
+    log('bench: VALU issue-rate probe')
+    probe = valu_probe()['rates']
+    vop2 = probe['fp32_fma_lane']
+    attainable = {
+        'source': 'tests/native/lmz_valu_peak.hip, measured in this run',
+        'fp32_fma_vop2_tflops': vop2['tflops'],
+        'fp32_fma_vop3_tflops': probe['fp32_fma_uniform']['tflops'],
+        'fp64_fma_tflops': probe['fp64_fma_lane']['tflops'],
+        'fp32_vop2_lane_instructions_per_s': vop2['lane_instructions_per_s']}
+    issue = (lane_inst / tj) if lane_inst else None
     fd = None
     fdb = None
     if not args.no_fd:
@@ -1259,6 +1316,13 @@ def main():
                      'algorithmic_bytes_per_launch': bytes_j,
                      'counted_fp32_flop_per_launch': counted,
                      'counted_achieved': (counted / tj / 1e12) if counted else None,
+                     'attainable': attainable,
+                     'frac_of_attainable_counted': ((counted / tj / 1e12) /
+                                                    attainable['fp32_fma_vop2_tflops']
+                                                    if counted else None),
+                     'valu_lane_instructions_per_launch': lane_inst,
+                     'frac_of_attainable_issue': ((issue / attainable['fp32_vop2_lane_instructions_per_s'])
+                                                  if issue else None),
                      'forward_kernel': {'achieved': ach_f, 'avg_launch_ms': tf * 1e3,
                                         'algorithmic_flop_per_launch': flop_f}},
         'reductions': reductions,

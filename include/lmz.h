@@ -422,6 +422,55 @@ int lmz_lm_run_layered(lmz_lm_plan *plan, const float *coords, int32_t flags,
                        hipStream_t stream);
 
 /* ------------------------------------------------------------------ *
+ * Differential-evolution estimates of many features, on the device.
+ * ------------------------------------------------------------------ */
+
+#define LMZ_DE_MAX_DIM 7
+#define LMZ_DE_MAX_POP 1024
+
+typedef struct lmz_de_settings {
+    int32_t ndim;           /* searched parameters, 1..LMZ_DE_MAX_DIM */
+    int32_t cols[LMZ_DE_MAX_DIM]; /* their columns in a params row (0..6) */
+    int32_t npop;           /* members per search, 5..LMZ_DE_MAX_POP
+                               (scipy: max(5, popsize * ndim)) */
+    int32_t maxiter;        /* generations after the initial population */
+    int32_t check_every;    /* generations between host reads of the
+                               number of converged searches */
+    int32_t reserved;
+    double tol, atol;       /* stop a search when std(E) <= atol + tol |mean(E)| */
+    double mutation_lo, mutation_hi;  /* dither range of the scale */
+    double recombination;   /* crossover probability */
+    uint64_t seed;
+} lmz_de_settings;
+
+/* Device workspace of lmz_de_run (256-byte aligned). */
+int lmz_de_workspace_bytes(const lmz_batch *batch,
+                           const lmz_de_settings *settings, size_t *bytes);
+
+/* nsearch = nfeat / npop differential-evolution searches advanced together,
+ * each the search scipy's differential_evolution runs for one feature in
+ * the reference's DEEstimator.estimate (analysis/DEEstimator.py:119-173:
+ * strategy best1bin, deferred updating, Latin-hypercube start, no polish):
+ * every generation is one trial kernel, the Mie and residual launches of
+ * all members of all unconverged searches, and one selection kernel; the
+ * host only reads the count of converged searches every check_every
+ * generations.  The batch holds the searches feature-major: features
+ * s*npop .. s*npop + npop - 1 are search s's members, each a copy of that
+ * search's pixels.  base: double [nsearch][LMZ_NPARAM], the parameters a
+ * search does not set; bounds: double [nsearch][ndim][2] (lo, hi); the
+ * energy is chi2 = sum ((I - d) / noise)^2 over the member's pixels
+ * (flags as lmz_hologram's; LMZ_FP32 allowed, the search only ranks).
+ * Outputs (device): best double [nsearch][ndim] (the member of lowest
+ * energy, in parameter units), energy double [nsearch], generations
+ * int32 [nsearch] (the generation whose selection met the stopping test,
+ * else maxiter).  Random draws are counter-based: a seed gives the same
+ * searches on any device and batch composition. */
+int lmz_de_run(const lmz_batch *batch, const lmz_de_settings *settings,
+               const double *base, const double *bounds, const float *coords,
+               int32_t flags, const float *data, double *best, double *energy,
+               int32_t *generations, void *workspace, hipStream_t stream);
+
+/* ------------------------------------------------------------------ *
  * Frames resident in device memory: background normalization and the
  * crops that feed a batch.
  * ------------------------------------------------------------------ */

@@ -98,6 +98,22 @@ class LMSettings(ctypes.Structure):
                 ('reserved', ctypes.c_int32)]
 
 
+class DESettings(ctypes.Structure):
+    """lmz_de_settings, include/lmz.h."""
+    _fields_ = [('ndim', ctypes.c_int32),
+                ('cols', ctypes.c_int32 * NJAC),
+                ('npop', ctypes.c_int32),
+                ('maxiter', ctypes.c_int32),
+                ('check_every', ctypes.c_int32),
+                ('reserved', ctypes.c_int32),
+                ('tol', ctypes.c_double),
+                ('atol', ctypes.c_double),
+                ('mutation_lo', ctypes.c_double),
+                ('mutation_hi', ctypes.c_double),
+                ('recombination', ctypes.c_double),
+                ('seed', ctypes.c_uint64)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
 
@@ -134,6 +150,10 @@ SIGNATURES = {
                                  _P, _P, _P, _P, _P, _P]),
     'lmz_lm_run_layered': (_I, [_P, _P, _I, _P, ctypes.POINTER(LMSettings),
                                 _P, _P, _P, _P, _P, _P]),
+    'lmz_de_workspace_bytes': (_I, [_P, ctypes.POINTER(DESettings),
+                                    ctypes.POINTER(ctypes.c_size_t)]),
+    'lmz_de_run': (_I, [_P, ctypes.POINTER(DESettings), _P, _P, _P, _I, _P,
+                        _P, _P, _P, _P, _P]),
     'lmz_median_scratch_bytes': (ctypes.c_size_t, []),
     'lmz_median': (_I, [_P, _I, ctypes.c_int64, _P, _P, _P]),
     'lmz_median_filter': (_I, [_P, _I, _I, _I, _I, _P, _P, _I, _P]),
@@ -202,9 +222,15 @@ def stream_handle(stream=None) -> int:
 
 
 def ptr(tensor) -> int | None:
-    """Device pointer of a tensor, or None."""
+    """Device pointer of a tensor, or None.  The C ABI reads every buffer
+    as densely packed row-major memory, so a strided tensor (a transposed
+    view, or one copied from a Fortran-ordered array) is refused."""
     if tensor is None:
         return None
+    if not tensor.is_contiguous():
+        raise LMZError(f'liblmz needs contiguous buffers; got a tensor of '
+                       f'shape {tuple(tensor.shape)} with strides '
+                       f'{tuple(tensor.stride())}')
     return tensor.data_ptr()
 
 

@@ -43,8 +43,9 @@ class PopulationChi2:
         if c.shape[0] == 2:
             c = np.vstack([c, np.zeros((1, c.shape[1]))])
         self.origin = (float(np.round(c[0].mean())), float(np.round(c[1].mean())), 0.)
-        self.rel = (c - np.asarray(self.origin)[:, None]).astype(np.float32)
-        self.data = np.asarray(data, np.float32).ravel()
+        self.rel = np.ascontiguousarray(
+            (c - np.asarray(self.origin)[:, None]).astype(np.float32))
+        self.data = np.ascontiguousarray(np.asarray(data, np.float32).ravel())
         self.instrument = instrument
         self.base = np.asarray(base, float)
         self.cap = int(cap)
@@ -58,8 +59,11 @@ class PopulationChi2:
                                    pix_offset=i * n) for i in range(size)]
         self.batch = _lmz.Batch(feats, 1)
         b = self.batch
-        self.coords = t.from_numpy(np.tile(self.rel, (1, size))).to(self.dev)
-        self.d = t.from_numpy(np.tile(self.data, size)).to(self.dev)
+        # np.tile with unit repetitions copies in the input's memory order;
+        # the kernels read [3][npix] row-major
+        self.coords = t.from_numpy(np.ascontiguousarray(
+            np.tile(self.rel, (1, size)))).to(self.dev)
+        self.d = t.from_numpy(np.ascontiguousarray(np.tile(self.data, size))).to(self.dev)
         self.params = t.zeros((size, _lmz.NPARAM), dtype=t.float64, device=self.dev)
         self.nmax = t.zeros(size, dtype=t.int32, device=self.dev)
         self.coef = t.zeros(b.coef_floats(), dtype=t.float32, device=self.dev)
@@ -186,17 +190,18 @@ def _objective(prob: dict, chi2):
     return objective
 
 
-class _FramePopulation:
-    """chi2 of every candidate of every feature in one device launch: the
-    batch holds NP copies of each feature's masked pixels, feature-major."""
+class _FrameSearch:
+    """All searches of a frame as one device batch (npop copies of each
+    feature's masked pixels, feature-major) run by lmz_de_run: the whole
+    differential evolution stays on the device."""
 
-    def __init__(self, probs: list, npop: int):
+    def __init__(self, probs: list, npop: int, lo: np.ndarray, hi: np.ndarray):
         t = _lmz.torch()
         self._t = t
         dev = _lmz.device()
         inst = probs[0]['instrument']
         cap = max(p['cap'] for p in probs)
-        feats, coords, data, owner = [], [], [], []
+        feats, coords, data = [], [], []
         off = 0
         for p in probs:
             c = np.atleast_2d(np.asarray(p['coords'], float))
@@ -212,41 +217,58 @@ class _FramePopulation:
                 data.append(d)
                 off += d.size
         self.batch = _lmz.Batch(feats, 1)
-        b = self.batch
+        self.nsearch, self.npop = len(probs), npop
+        self.ndim = lo.shape[1]
         self.coords = t.from_numpy(np.ascontiguousarray(
             np.concatenate(coords, axis=1))).to(dev)
         self.data = t.from_numpy(np.concatenate(data)).to(dev)
-        self.params = t.zeros((len(feats), _lmz.NPARAM), dtype=t.float64, device=dev)
-        self.nmax = t.zeros(len(feats), dtype=t.int32, device=dev)
-        self.coef = t.zeros(b.coef_floats(), dtype=t.float32, device=dev)
-        self.work = t.zeros(b.work_doubles, dtype=t.float64, device=dev)
-        self.chi2 = t.zeros(b.ntiles, dtype=t.float64, device=dev)
-        ntiles = [f.ntiles for f in b.features]
-        self.tile_owner = t.from_numpy(np.repeat(np.arange(len(feats)),
-                                                 ntiles)).to(dev)
-        self.n = len(feats)
+        base = np.zeros((len(probs), _lmz.NPARAM))
+        base[:, :7] = np.stack([p['base'] for p in probs])
+        self.base = t.from_numpy(base).to(dev)
+        self.bounds = t.from_numpy(np.ascontiguousarray(
+            np.stack([lo, hi], axis=-1))).to(dev)
+        self.cols = probs[0]['cols']
+        self.best = t.zeros((self.nsearch, self.ndim), dtype=t.float64, device=dev)
+        self.energy = t.zeros(self.nsearch, dtype=t.float64, device=dev)
+        self.gens = t.zeros(self.nsearch, dtype=t.int32, device=dev)
 
-    def __call__(self, rows: np.ndarray) -> np.ndarray:
+    def settings(self, *, maxiter, tol, atol, mutation, recombination, seed,
+                 check_every=4) -> _lmz.DESettings:
+        st = _lmz.DESettings()
+        st.ndim = self.ndim
+        for j, c in enumerate(self.cols):
+            st.cols[j] = c
+        st.npop = self.npop
+        st.maxiter = int(maxiter)
+        st.check_every = int(check_every)
+        st.tol, st.atol = float(tol), float(atol)
+        st.mutation_lo, st.mutation_hi = float(mutation[0]), float(mutation[1])
+        st.recombination = float(recombination)
+        st.seed = int(seed) & 0xffffffffffffffff
+        return st
+
+    def run(self, st: _lmz.DESettings):
+        """(best (nsearch, ndim) in parameter units, energy (nsearch,),
+        generations (nsearch,)) on the host."""
+        import ctypes
         t = self._t
-        host = np.zeros((self.n, _lmz.NPARAM))
-        host[:, :7] = rows
-        self.params.copy_(t.from_numpy(host))
         lib = _lmz.load_library()
-        s = _lmz.stream_handle()
         b = self.batch
-        _lmz.check(lib.lmz_mie_ab(b.handle, _lmz.ptr(self.params), None,
-                                  _lmz.ptr(self.nmax), _lmz.ptr(self.coef), None,
-                                  None, _lmz.ptr(self.work), s))
-        _lmz.check(lib.lmz_hologram(b.handle, _lmz.ptr(self.params), None,
-                                    _lmz.ptr(self.nmax), _lmz.ptr(self.coef),
-                                    _lmz.ptr(self.coords),
-                                    _lmz.CARTESIAN | _lmz.BOHREN | _lmz.FP32,
-                                    _lmz.ptr(self.data), None,
-                                    _lmz.ptr(self.chi2), None, s))
-        out = t.zeros(self.n, dtype=t.float64, device=self.chi2.device)
-        out.index_add_(0, self.tile_owner, self.chi2)
-        chi2 = out.cpu().numpy()
-        return np.where(np.isfinite(chi2), chi2, np.inf)
+        nbytes = ctypes.c_size_t()
+        _lmz.check(lib.lmz_de_workspace_bytes(b.handle, ctypes.byref(st),
+                                              ctypes.byref(nbytes)))
+        ws = t.empty(nbytes.value + 256, dtype=t.uint8, device=self.data.device)
+        wptr = (ws.data_ptr() + 255) & ~255
+        flags = _lmz.CARTESIAN | _lmz.BOHREN | _lmz.FP32
+        _lmz.check(lib.lmz_de_run(b.handle, ctypes.byref(st), _lmz.ptr(self.base),
+                                  _lmz.ptr(self.bounds), _lmz.ptr(self.coords),
+                                  flags, _lmz.ptr(self.data), _lmz.ptr(self.best),
+                                  _lmz.ptr(self.energy), _lmz.ptr(self.gens),
+                                  wptr, _lmz.stream_handle()))
+        out = (self.best.cpu().numpy(), self.energy.cpu().numpy(),
+               self.gens.cpu().numpy())
+        del ws
+        return out
 
 
 _BATCH_SETTINGS = {'tol', 'atol', 'maxiter', 'mutation', 'recombination',
@@ -256,17 +278,21 @@ _BATCH_SETTINGS = {'tol', 'atol', 'maxiter', 'mutation', 'recombination',
 def estimate_batch(estimators: list, holograms: list, maxiter: int | None = None
                    ) -> list:
     """One differential-evolution search per (estimator, hologram) pair,
-    all advanced together: every generation's candidates of every search in
-    one device launch, and scipy's best1bin step (scale in the ``mutation``
-    range drawn per generation, binomial crossover at ``recombination`` with
-    one forced dimension, out-of-range values redrawn, deferred selection,
-    stop when std(energies) <= atol + tol |mean(energies)|, at most
-    ``maxiter`` generations) applied to all of them at once as array
-    operations; scipy's population of max(5, popsize * D) members.  The
-    estimators must be DEEstimators with the same bounds keys, popsize,
-    settings and instrument; the searches draw from one random stream
-    seeded by the first estimator's seed.  Returns each estimator's
-    _finish (its model updated)."""
+    all run together on the device (lmz_de_run): every generation's
+    trials, their Mie coefficients and residuals for every unconverged
+    search in three launches, and the selection and stopping test in a
+    fourth; nothing returns to the host until the searches end.  Each is
+    scipy's search as the reference's DEEstimator runs it
+    (analysis/DEEstimator.py:119-173): best1bin with the scale drawn per
+    generation from the ``mutation`` range, binomial crossover at
+    ``recombination`` with one forced dimension, coordinates outside the
+    bounds redrawn, deferred selection, stop when std(energies) <= atol +
+    tol |mean(energies)|, at most ``maxiter`` generations, a Latin-
+    hypercube start of max(5, popsize * D) members.  The draws come from a
+    counter-based generator seeded by the first estimator's seed (None:
+    fresh entropy, as scipy takes it), not from scipy's stream.  The estimators must be DEEstimators
+    with the same bounds keys, popsize, settings and instrument.  Returns
+    each estimator's _finish (its model updated)."""
     if not estimators:
         return []
     e0 = estimators[0]
@@ -275,7 +301,7 @@ def estimate_batch(estimators: list, holograms: list, maxiter: int | None = None
             raise TypeError('estimate_batch needs DEEstimators, got '
                             f'{type(e).__name__}')
     names = list(e0.bounds.keys())
-    D, F = len(names), len(estimators)
+    D = len(names)
     npop = max(5, e0.popsize * D)
     settings = dict(e0.settings)
     unknown = set(settings) - _BATCH_SETTINGS
@@ -287,6 +313,9 @@ def estimate_batch(estimators: list, holograms: list, maxiter: int | None = None
         raise ValueError('estimate_batch does not polish (polish=False)')
     if settings.get('init', 'latinhypercube') != 'latinhypercube':
         raise ValueError('estimate_batch starts from a Latin hypercube')
+    if D > 7 or npop > 1024:
+        raise ValueError('estimate_batch searches at most 7 parameters with '
+                         'at most 1024 members')
     tol = float(settings.get('tol', 0.01))
     atol = float(settings.get('atol', 0.))
     mutation = settings.get('mutation', (0.5, 1.))
@@ -307,46 +336,12 @@ def estimate_batch(estimators: list, holograms: list, maxiter: int | None = None
                 (inst0.n_m, inst0.wavelength, inst0.magnification):
             raise ValueError('estimators differ in instrument')
     probs = [e._problem(h) for e, h in zip(estimators, holograms)]
-    lo = np.array([[e.bounds[k][0] for k in names] for e in estimators])
-    hi = np.array([[e.bounds[k][1] for k in names] for e in estimators])
-    base = np.stack([p['base'] for p in probs])
-    cols = probs[0]['cols']
-    pop_chi2 = _FramePopulation(probs, npop)
-    rng = np.random.default_rng(e0.seed)
-
-    def energies(unit):
-        rows = np.repeat(base, npop, axis=0)
-        rows[:, cols] = (lo[:, None, :] + unit * (hi - lo)[:, None, :]).reshape(-1, D)
-        return pop_chi2(rows).reshape(F, npop)
-
-    # Latin hypercube start, per search and dimension
-    unit = (np.argsort(rng.random((F, npop, D)), axis=1) + rng.random((F, npop, D))) / npop
-    e = energies(unit)
-    done = np.zeros(F, bool)
-    idx = np.arange(npop)
-    for _ in range(maxiter):
-        best = unit[np.arange(F), np.argmin(e, axis=1)]             # (F, D)
-        scale = rng.uniform(mlo, mhi, (F, 1, 1))
-        k0 = rng.integers(0, npop - 1, (F, npop))
-        r0 = k0 + (k0 >= idx)
-        a, b = np.minimum(idx, r0), np.maximum(idx, r0)
-        r1 = rng.integers(0, npop - 2, (F, npop))
-        r1 = r1 + (r1 >= a)
-        r1 = r1 + (r1 >= b)
-        take = np.arange(F)[:, None]
-        mutant = best[:, None, :] + scale * (unit[take, r0] - unit[take, r1])
-        cross = rng.random((F, npop, D)) < recombination
-        cross[take, idx[None, :], rng.integers(0, D, (F, npop))] = True
-        trial = np.where(cross, mutant, unit)
-        bad = (trial < 0.) | (trial > 1.)
-        trial[bad] = rng.random(int(bad.sum()))
-        et = energies(trial)
-        better = (et < e) & ~done[:, None]
-        unit = np.where(better[..., None], trial, unit)
-        e = np.where(better, et, e)
-        with np.errstate(invalid='ignore'):     # a search holding inf waits
-            done |= np.std(e, axis=1) <= atol + tol * np.abs(np.mean(e, axis=1))
-        if done.all():
-            break
-    x = lo + unit[np.arange(F), np.argmin(e, axis=1)] * (hi - lo)
+    lo = np.array([[e.bounds[k][0] for k in names] for e in estimators], float)
+    hi = np.array([[e.bounds[k][1] for k in names] for e in estimators], float)
+    search = _FrameSearch(probs, npop, lo, hi)
+    st = search.settings(maxiter=maxiter, tol=tol, atol=atol, mutation=(mlo, mhi),
+                         recombination=recombination,
+                         seed=(np.random.SeedSequence().entropy
+                               if e0.seed is None else e0.seed))
+    x, _, _ = search.run(st)
     return [est._finish(xi) for est, xi in zip(estimators, x)]

@@ -148,10 +148,11 @@ class Frame(Hologram):
     def estimate(self, batched: bool = False) -> None:
         """Starting values of every feature: one DE search after another
         (analysis/Frame.py:158-161), each generation's population on the
-        GPU; or, with ``batched``, all searches advanced together with
-        every generation of every feature in one launch
-        (DEEstimator.estimate_batch; the features' estimators must share
-        bounds keys and popsize)."""
+        GPU; or, with ``batched``, all searches run together on the device
+        (DEEstimator.estimate_batch, lmz_de_run: every generation of every
+        feature's search in four launches, no host work between
+        generations; the features' estimators must share bounds keys,
+        popsize and settings)."""
         if batched and self.features:
             from .DEEstimator import estimate_batch
             estimate_batch([f.estimator for f in self.features], self.features)

@@ -1,13 +1,16 @@
 // Host build of the per-pixel and per-feature arithmetic in
 // pylorenzmie_amd/csrc/lmz_pixel.h, the LM step logic in
-// pylorenzmie_amd/csrc/lmz_lmstep.h and the frame algorithms in
+// pylorenzmie_amd/csrc/lmz_lmstep.h, the differential-evolution step in
+// pylorenzmie_amd/csrc/lmz_de.h and the frame algorithms in
 // pylorenzmie_amd/csrc/lmz_frame.h, for CPU tests of the kernel formulas
 // against the FP64 oracle, numpy and scipy.  Test instrument only: the library's product
 // path is the gfx950 kernels in liblmz.so.
 #include <algorithm>
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
+#include "../../pylorenzmie_amd/csrc/lmz_de.h"
 #include "../../pylorenzmie_amd/csrc/lmz_frame.h"
 #include "../../pylorenzmie_amd/csrc/lmz_lmstep.h"
 #include "../../pylorenzmie_amd/csrc/lmz_pixel.h"
@@ -142,6 +145,91 @@ void chk_dense_coords(int width, int npix, float *ax, float *ay) {
     for (int j = 0; j < npix; ++j) {
         float z;
         pixel_coords(fu, j, nullptr, 0, ax[j], ay[j], z);
+    }
+}
+
+
+// ---- differential evolution (lmz_de.h), driven in the order the kernels
+// of lmz_de.hip run it: k_de_init, energies, k_de_select(0), then per
+// generation k_de_trial for every member, energies, k_de_select.
+
+// Latin-hypercube start of search s: unit[npop][ndim].
+void chk_de_lhs(uint64_t seed, int s, int npop, int ndim, double *unit) {
+    std::vector<double> keys(npop);
+    for (int j = 0; j < ndim; ++j) {
+        for (int i = 0; i < npop; ++i) keys[i] = de_lhs_key(seed, (uint32_t)s, i, j);
+        for (int i = 0; i < npop; ++i)
+            unit[i * ndim + j] = de_lhs(seed, (uint32_t)s, i, j, npop, keys.data());
+    }
+}
+
+// One best1bin trial of member i; also returns the scale used.
+double chk_de_trial(uint64_t seed, int gen, int s, int i, int npop, int ndim,
+                    const double *u, int best, double mlo, double mhi,
+                    double CR, double *trial) {
+    const double F = de_scale(seed, (uint32_t)gen, (uint32_t)s, mlo, mhi);
+    de_trial(seed, (uint32_t)gen, (uint32_t)s, i, npop, ndim, u, best, F, CR, trial);
+    return F;
+}
+
+// sizeof(lmz_de_settings) and the offsets of npop, tol and seed
+void chk_de_settings_layout(int *out) {
+    out[0] = (int)sizeof(lmz_de_settings);
+    out[1] = (int)offsetof(lmz_de_settings, npop);
+    out[2] = (int)offsetof(lmz_de_settings, tol);
+    out[3] = (int)offsetof(lmz_de_settings, seed);
+}
+
+int chk_de_converged(const double *E, int n, double tol, double atol) {
+    return de_converged(E, n, tol, atol) ? 1 : 0;
+}
+
+// The whole search of nsearch independent problems whose energy is the
+// bowl 1 + sum_j ((x_j - target[s][j]) / scale[s][j])^2 in parameter units
+// (x_j = lo_j + u_j (hi_j - lo_j), bounds [s][j][2]), or, with
+// flat != 0, a constant.  Outputs as lmz_de_run's.
+void chk_de_bowl(uint64_t seed, int nsearch, int npop, int ndim, int maxiter,
+                 double tol, double atol, double mlo, double mhi, double CR,
+                 const double *bounds, const double *target, const double *scale,
+                 int flat, double *best_out, double *energy_out, int *gens) {
+    auto energy = [&](int s, const double *u) {
+        if (flat) return 1.0;
+        double e = 1.0;
+        for (int j = 0; j < ndim; ++j) {
+            const double *b = bounds + 2 * (s * ndim + j);
+            const double x = b[0] + u[j] * (b[1] - b[0]);
+            const double d = (x - target[s * ndim + j]) / scale[s * ndim + j];
+            e += d * d;
+        }
+        return e;
+    };
+    for (int s = 0; s < nsearch; ++s) {
+        std::vector<double> unit(npop * ndim), trial(npop * ndim), E(npop);
+        chk_de_lhs(seed, s, npop, ndim, unit.data());
+        for (int i = 0; i < npop; ++i) E[i] = de_energy(energy(s, &unit[i * ndim]));
+        int best = de_argmin(E.data(), npop);
+        gens[s] = 0;
+        for (int gen = 1; gen <= maxiter; ++gen) {
+            const double F = de_scale(seed, (uint32_t)gen, (uint32_t)s, mlo, mhi);
+            for (int i = 0; i < npop; ++i)
+                de_trial(seed, (uint32_t)gen, (uint32_t)s, i, npop, ndim,
+                         unit.data(), best, F, CR, &trial[i * ndim]);
+            for (int i = 0; i < npop; ++i) {
+                const double et = de_energy(energy(s, &trial[i * ndim]));
+                if (et < E[i]) {
+                    E[i] = et;
+                    for (int j = 0; j < ndim; ++j) unit[i * ndim + j] = trial[i * ndim + j];
+                }
+            }
+            best = de_argmin(E.data(), npop);
+            gens[s] = gen;
+            if (de_converged(E.data(), npop, tol, atol)) break;
+        }
+        energy_out[s] = E[best];
+        for (int j = 0; j < ndim; ++j) {
+            const double *b = bounds + 2 * (s * ndim + j);
+            best_out[s * ndim + j] = b[0] + unit[best * ndim + j] * (b[1] - b[0]);
+        }
     }
 }
 

@@ -132,3 +132,20 @@ def test_last_error_is_per_thread(lib):
     assert seen[0] in (b'', None)
     assert seen[1] != mine
     assert lib.lmz_last_error() == mine
+
+
+def test_ptr_refuses_strided_tensors():
+    """Every buffer crosses the C ABI as packed row-major memory: a
+    transposed view (or a tensor made from a Fortran-ordered array) is
+    refused instead of being read in the wrong order."""
+    import numpy as np
+    import torch
+    from pylorenzmie_amd import _lmz
+    a = torch.zeros((3, 5), dtype=torch.float32)
+    assert _lmz.ptr(a) == a.data_ptr()
+    assert _lmz.ptr(None) is None
+    with pytest.raises(_lmz.LMZError):
+        _lmz.ptr(a.t())
+    f = torch.from_numpy(np.asfortranarray(np.zeros((3, 5), np.float32)))
+    with pytest.raises(_lmz.LMZError):
+        _lmz.ptr(f)

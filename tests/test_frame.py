@@ -53,43 +53,48 @@ def test_detection_is_outside_estimation_is_de():
     assert set(est.bounds) == {'z_p', 'a_p', 'n_p'} and est.fraction == 0.02
 
 
-def test_batched_de_finds_each_minimum(monkeypatch):
-    """DEEstimator.estimate_batch, the search itself, on the CPU: with the
-    device objective replaced by a bowl whose minimum differs per feature,
-    every search lands at its own minimum (the best1bin step, the index
-    draws and the per-search stopping rule), and each estimator's model
-    receives it."""
+def test_batched_de_settings_reach_the_device_search(monkeypatch):
+    """DEEstimator.estimate_batch on the CPU, with the device search
+    (_FrameSearch / lmz_de_run) replaced by a stub: scipy's population
+    floor max(5, popsize * D), the settings it passes to lmz_de_settings,
+    each estimator's model receiving its own result, and the settings and
+    estimators it refuses.  The search itself is tested on the host build
+    of lmz_de.h (tests/test_de_step.py) and on the GPU
+    (tests/test_gpu_estimator.py)."""
     import importlib
+    from pylorenzmie_amd import _lmz
     from pylorenzmie_amd.analysis import DEEstimator
     from pylorenzmie_amd.theory import LorenzMie
     de_mod = importlib.import_module('pylorenzmie_amd.analysis.DEEstimator')
-    targets = np.array([[120., 0.8, 1.45], [300., 1.6, 1.60], [75., 0.5, 1.38]])
-    scale = np.array([50., 0.2, 0.05])
+    seen = {}
 
-    class Bowl:
-        def __init__(self, probs, npop):
-            self.npop = npop
+    class Stub(de_mod._FrameSearch):
+        def __init__(self, probs, npop, lo, hi):
+            seen.update(npop=npop, lo=lo, hi=hi, cols=probs[0]['cols'])
+            self.nsearch, self.npop, self.ndim = len(probs), npop, lo.shape[1]
+            self.cols = probs[0]['cols']
 
-        def __call__(self, rows):
-            x = rows[:, [2, 3, 4]].reshape(len(targets), self.npop, 3)
-            return (((x - targets[:, None, :]) / scale) ** 2).sum(-1).ravel()
+        def run(self, st):
+            seen['st'] = st
+            x = 0.5 * (seen['lo'] + seen['hi']) + np.arange(self.nsearch)[:, None]
+            return x, np.zeros(self.nsearch), np.zeros(self.nsearch, np.int32)
 
-    monkeypatch.setattr(de_mod, '_FramePopulation', Bowl)
-    ests, holos = [], []
-    for i in range(3):
-        m = LorenzMie()
-        ests.append(DEEstimator(model=m, seed=5,
-                                bounds={'z_p': (50., 400.), 'a_p': (0.3, 2.0),
-                                        'n_p': (1.35, 1.8)}))
-        holos.append(Hologram(np.random.default_rng(i).random((20, 20))))
+    monkeypatch.setattr(de_mod, '_FrameSearch', Stub)
+    bounds = {'z_p': (50., 400.), 'a_p': (0.3, 2.0), 'n_p': (1.35, 1.8)}
+    ests = [DEEstimator(model=LorenzMie(), seed=5, bounds=dict(bounds))
+            for _ in range(3)]
     for e in ests:
-        e.settings['tol'] = 1e-6
+        e.settings.update(tol=1e-6, atol=0.5, mutation=(0.4, 0.9),
+                          recombination=0.8, maxiter=77)
+    holos = [Hologram(np.random.default_rng(i).random((20, 20))) for i in range(3)]
     out = de_mod.estimate_batch(ests, holos)
+    st = seen['st']
+    assert seen['npop'] == st.npop == 30 and st.ndim == 3
+    assert list(st.cols)[:3] == seen['cols'] == [2, 3, 4]
+    assert (st.tol, st.atol, st.mutation_lo, st.mutation_hi, st.recombination,
+            st.maxiter, st.seed) == (1e-6, 0.5, 0.4, 0.9, 0.8, 77, 5)
     for i, (res, est) in enumerate(zip(out, ests)):
-        got = np.array([res['z_p'], res['a_p'], res['n_p']])
-        err = np.abs(got - targets[i]) / scale
-        print(i, got, err)
-        assert np.all(err < 0.05), i
+        assert res['z_p'] == pytest.approx(225. + i)
         assert est.model.particle.z_p == res['z_p']
     with pytest.raises(ValueError):
         de_mod.estimate_batch([ests[0], DEEstimator(model=LorenzMie(), popsize=5)],
@@ -97,19 +102,8 @@ def test_batched_de_finds_each_minimum(monkeypatch):
     # scipy's population floor max(5, popsize * D): one bound, popsize 2
     small = [DEEstimator(model=LorenzMie(), seed=1, popsize=2,
                          bounds={'z_p': (50., 400.)}) for _ in range(3)]
-    seen = []
-
-    class Line:
-        def __init__(self, probs, npop):
-            seen.append(npop)
-            self.npop = npop
-
-        def __call__(self, rows):
-            return ((rows[:, 2] - 200.) / 50.) ** 2
-    monkeypatch.setattr(de_mod, '_FramePopulation', Line)
-    out = de_mod.estimate_batch(small, holos)
-    assert seen == [5]
-    assert all(abs(r['z_p'] - 200.) < 5. for r in out)
+    de_mod.estimate_batch(small, holos)
+    assert seen['npop'] == 5 and seen['st'].ndim == 1
     # settings the batched step cannot honour are refused, others differ
     odd = DEEstimator(model=LorenzMie(), seed=1, popsize=2,
                       bounds={'z_p': (50., 400.)})
@@ -124,3 +118,4 @@ def test_batched_de_finds_each_minimum(monkeypatch):
     from pylorenzmie_amd.analysis import RadialEstimator
     with pytest.raises(TypeError):
         de_mod.estimate_batch([RadialEstimator(model=LorenzMie())], holos[:1])
+    assert _lmz.DESettings is not None

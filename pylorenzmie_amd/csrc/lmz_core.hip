@@ -126,13 +126,17 @@ __global__ __launch_bounds__(64) void k_mie_layered(
         ab_out ? ab_out + 4 * slot0 : nullptr);
 }
 
-// Layered spheres in the fitting parametrization, one lane per feature:
-// the table's radii scaled by params a_p / (the table's outer radius), the
-// outer layer's index n_p + i k_p from params, inner indices from the
-// table; coefficient tables of buffer (sel ^ flip) with the a_p, n_p, k_p
-// tangents (dcoef, may be NULL).  A trial point outside the domain
-// (a_p <= 0, n_p <= 0, non-finite) gets no orders (n_max 0: no scattered
-// field), which the LM rejects by its chi2.
+// Layered spheres in the fitting parametrization: the table's radii scaled
+// by params a_p / (the table's outer radius), the outer layer's index
+// n_p + i k_p from params, inner indices from the table; coefficient tables
+// of buffer (sel ^ flip) with the a_p, n_p, k_p tangents (dcoef, may be
+// NULL).  One lane per feature and pass (blockIdx.y): 0 the coefficients
+// and n_max, 1 and 2 the a_p and n_p (with k_p) tangents, each carried with
+// its own values through the sequential recursion, in scratch column block
+// pass * nfeat + f of the LMZ_WORK_PASSES interleaved blocks.  The three
+// recursions of a feature run side by side instead of one after another.
+// A trial point outside the domain (a_p <= 0, n_p <= 0, non-finite) gets no
+// orders (n_max 0: no scattered field), which the LM rejects by its chi2.
 __global__ __launch_bounds__(64) void k_mie_layered_fit(
     const lmz_feature *__restrict__ feat, int nfeat,
     const double *__restrict__ params, const int32_t *__restrict__ sel,
@@ -141,8 +145,10 @@ __global__ __launch_bounds__(64) void k_mie_layered_fit(
     float *__restrict__ coef, float *__restrict__ dcoef,
     double *__restrict__ work, const uint8_t *__restrict__ active) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pass = blockIdx.y;
     if (f >= nfeat) return;
     if (active && !active[f]) return;
+    if (pass > 0 && !dcoef) return;
     const lmz_feature F = feat[f];
     const int b = sel ? (sel[f] ^ flip) : 0;
     const double *p = params + (long long)LMZ_NPARAM * f;
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(64) void k_mie_layered_fit(
         ok = r[l] > (l ? r[l - 1] : 0.0) && isfinite(r[l]);
     }
     if (!ok) {
-        nmax_out[(long long)f * nbuf + b] = 0;
+        if (pass == 0) nmax_out[(long long)f * nbuf + b] = 0;
         return;
     }
     const double scale = p[3] / r[L - 1];
@@ -167,15 +173,18 @@ __global__ __launch_bounds__(64) void k_mie_layered_fit(
     nl[L - 1] = p[4];
     kl[L - 1] = p[5];
     const long long slot0 = F.coef_offset + (long long)b * F.nmax_cap;
-    const int N = mie_layered_feature(r, nl, kl, L, F.n_m, F.wavelength,
-                                      F.nmax_cap, work + f, nfeat,
-                                      coef + LMZ_COEF_STRIDE * slot0, nullptr);
-    if (dcoef)
-        for (int dir = 0; dir < 2; ++dir)
-            mie_layered_tangent(r, nl, kl, L, F.n_m, F.wavelength, dir, N,
-                                work + f, nfeat,
-                                dcoef + LMZ_DCOEF_STRIDE * slot0);
-    nmax_out[(long long)f * nbuf + b] = N;
+    double *w = work + (long long)pass * nfeat + f;
+    const long long wstride = (long long)LMZ_WORK_PASSES * nfeat;
+    if (pass == 0) {
+        const int N = mie_layered_feature(r, nl, kl, L, F.n_m, F.wavelength,
+                                          F.nmax_cap, w, wstride,
+                                          coef + LMZ_COEF_STRIDE * slot0, nullptr);
+        nmax_out[(long long)f * nbuf + b] = N;
+    } else {
+        const int N = layered_nmax(r, nl, kl, L, F.n_m, F.wavelength, F.nmax_cap);
+        mie_layered_tangent(r, nl, kl, L, F.n_m, F.wavelength, pass - 1, N, w,
+                            wstride, dcoef + LMZ_DCOEF_STRIDE * slot0);
+    }
 }
 
 // Per-order tables from FP64 a_n, b_n supplied by the caller.
@@ -743,8 +752,9 @@ void launch_mie_layered(const lmz_batch *b, const double *params,
                         const int32_t *nlayers, int32_t *nmax, float *coef,
                         float *dcoef, double *work, const uint8_t *active,
                         hipStream_t stream) {
-    hipLaunchKernelGGL(k_mie_layered_fit, dim3((b->nfeat + 63) / 64), dim3(64),
-                       0, stream, b->d_feat, b->nfeat, params, sel, flip,
+    hipLaunchKernelGGL(k_mie_layered_fit,
+                       dim3((b->nfeat + 63) / 64, dcoef ? LMZ_WORK_PASSES : 1),
+                       dim3(64), 0, stream, b->d_feat, b->nfeat, params, sel, flip,
                        b->coef_buffers, layers, nlayers, nmax, coef, dcoef,
                        work, active);
 }
@@ -1131,7 +1141,8 @@ int lmz_batch_query(const lmz_batch *b, lmz_batch_sizes *s) {
     s->coef_buffers = b->coef_buffers;
     s->coef_slots = b->coef_slots;
     s->cap_max = b->cap_max;
-    s->work_doubles = (long long)LMZ_WORK_PER_ORDER * (b->cap_max + 1) * b->nfeat;
+    s->work_doubles = (long long)LMZ_WORK_PER_ORDER * LMZ_WORK_PASSES *
+                      (b->cap_max + 1) * b->nfeat;
     s->coef_stride = LMZ_COEF_STRIDE;
     s->dcoef_stride = LMZ_DCOEF_STRIDE;
     return LMZ_OK;

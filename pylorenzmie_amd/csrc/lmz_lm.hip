@@ -39,7 +39,8 @@ static LMLayout lm_layout(const lmz_batch *b) {
     L.dcoef = take(sizeof(float) * LMZ_DCOEF_STRIDE * (size_t)b->coef_slots);
     L.nmax = take(sizeof(int32_t) * 2 * (size_t)b->nfeat);
     L.sel = take(sizeof(int32_t) * (size_t)b->nfeat);
-    L.work = take(sizeof(double) * LMZ_WORK_PER_ORDER * (size_t)(b->cap_max + 1) *
+    L.work = take(sizeof(double) * LMZ_WORK_PER_ORDER * LMZ_WORK_PASSES *
+                  (size_t)(b->cap_max + 1) *
                   b->nfeat);
     L.partials = take(sizeof(double) * LMZ_NRED * (size_t)b->ntiles);
     L.chi2_tiles = take(sizeof(double) * (size_t)b->ntiles);

@@ -681,8 +681,11 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
 // ====================================================================
 // Mie coefficients in FP64 with a_p, n_p, k_p tangents
 // ====================================================================
-// doubles of scratch per order and feature (lmz_batch_sizes.work_doubles)
+// doubles of scratch per order and feature and pass; the batch's scratch
+// (lmz_batch_sizes.work_doubles) holds LMZ_WORK_PASSES passes, which the
+// layered-sphere fit runs side by side (value, a_p and n_p tangents)
 #define LMZ_WORK_PER_ORDER 16
+#define LMZ_WORK_PASSES 3
 // Sphere.wiscombe_yang, theory/Sphere.py:91-123 (one layer: the rolled
 // term equals |x m|)
 LMZ_HD int wiscombe_yang(double x, cd m) {
@@ -928,6 +931,23 @@ LMZ_HD cd cexp_i2(cd z) {          // exp(2 i z)
 // H_a, H_b; then Eqs. 5-6 on the outer boundary.  FP64, no tangents; one
 // layer reproduces mie_feature's coefficients.  layer l: radius[l] [um],
 // index n[l] + i k[l].  work as mie_feature.  Returns n_max (clamped).
+// n_max of a layered sphere, clamped to [1, cap] (mie_layered_feature's).
+LMZ_HD int layered_nmax(const double *radius, const double *nl,
+                        const double *kl, int L, double n_m,
+                        double wavelength, int cap) {
+    const double kum = 2.0 * M_PI / wavelength * n_m;
+    double x[LMZ_MAX_LAYERS];
+    cd m[LMZ_MAX_LAYERS];
+    for (int l = 0; l < L; ++l) {
+        x[l] = kum * radius[l];
+        m[l] = cd{nl[l] / n_m, kl[l] / n_m};
+    }
+    int N = wiscombe_yang_layers(x, m, L);
+    if (N > cap) N = cap;
+    if (N < 1) N = 1;
+    return N;
+}
+
 LMZ_HD int mie_layered_feature(const double *radius, const double *nl,
                                const double *kl, int L, double n_m,
                                double wavelength, int cap, double *work,
@@ -939,9 +959,7 @@ LMZ_HD int mie_layered_feature(const double *radius, const double *nl,
         x[l] = kum * radius[l];
         m[l] = cd{nl[l] / n_m, kl[l] / n_m};
     }
-    int N = wiscombe_yang_layers(x, m, L);
-    if (N > cap) N = cap;
-    if (N < 1) N = 1;
+    const int N = layered_nmax(radius, nl, kl, L, n_m, wavelength, cap);
 #define W(n, j) work[((long long)(n) * LMZ_WORK_PER_ORDER + (j)) * wstride]
     // D1 downward from D1[N] = 0 into work columns (c, c + 1), orders 0..N
     auto d1_down = [&](cd z, int c) {

@@ -152,6 +152,28 @@ def test_large_sphere_jacobian(torch):
         assert rel <= 1e-3, name
 
 
+@pytest.mark.parametrize('q', [
+    (20.2, 19.7, 5. / MAG + 60., 5.0, 1.9, 0.002, 1.0),
+    (20.2, 19.7, 3. / MAG + 40., 3.0, 2.0, 0.0, 0.9)])
+def test_high_index_jacobian_against_fine_differences(torch, q):
+    """Large high-index spheres, whose coefficients swing fast with n_p:
+    every column of the FP32 Jacobian against 7-point FP64 differences of
+    the oracle with steps of 1e-6 relative (at 1e-4 the differences
+    themselves miss the swings of the n_p column by ~7e-2)."""
+    from pylorenzmie_amd.theory import Instrument, LorenzMie, Sphere
+    c = ref.meshgrid((40, 40))
+    s = Sphere(x_p=q[0], y_p=q[1], z_p=q[2], a_p=q[3], n_p=q[4], k_p=q[5])
+    m = LorenzMie(coordinates=c, particle=s, instrument=Instrument(**INSTRUMENT),
+                  alpha=q[6])
+    J = m.jac()
+    Jref = ref.jacobian(c, dict(zip(ref.PARAMS, q)), INSTRUMENT, order=6,
+                        rel_step=1e-6)
+    for i, name in enumerate(ref.PARAMS):
+        rel = np.abs(J[name] - Jref[i]).max() / np.abs(Jref[i]).max()
+        print(f'{name}: {rel:.2e}')
+        assert rel <= 1e-3, name
+
+
 def lm_crops(data_host, side, caps):
     return [dict(data=data_host[i], width=side, origin=(0., 0., 0.),
                  nmax_cap=caps[i]) for i in range(len(caps))]

@@ -121,6 +121,44 @@ def test_mie_matches_golden_and_tangents(chk, forward_golden):
         assert np.abs(db - fd[:, 1]).max() < 1e-5 * scale
 
 
+@pytest.mark.parametrize('a,n,k', [(5.0, 1.9, 0.002), (9.0, 1.45, 0.002),
+                                   (3.0, 2.0, 0.0)])
+def test_large_sphere_tangents_match_fine_differences(chk, a, n, k):
+    """Large and high-index spheres (N to ~250), where the coefficients
+    swing fast with n_p and a_p: the FP64 tangents (stored as FP32) against
+    5-point differences of the FP64 oracle with steps of 1e-7 relative,
+    fine enough that the stencil resolves the swings (1e-4 does not at
+    a_p = 5 um, n_p = 1.9)."""
+    N, _, dcoef, _ = mie(chk, a, n, k)
+    nn = np.arange(1, N + 1)
+    en = (2 * nn + 1) / (nn * (nn + 1))
+    q = np.array([a, n, k])
+    for j in range(3):
+        h = 1e-7 * max(abs(q[j]), 1e-2)
+        fd = 0
+        for s, w in ((1, 8 / 12), (2, -1 / 12)):
+            qp, qm = q.copy(), q.copy()
+            qp[j] += s * h
+            qm[j] -= s * h
+            fd = fd + w * (ref.mie_coefficients(*qp, 1.34, 0.447, nmax=N) -
+                           ref.mie_coefficients(*qm, 1.34, 0.447, nmax=N))
+        fd = fd[1:] / h
+        if j < 2:
+            b = 9 * j
+            dP = dcoef[:N, b + 3] + 1j * dcoef[:N, b + 4]
+            dQ = -(dcoef[:N, b + 6] + 1j * dcoef[:N, b + 7])
+        else:
+            dP = dcoef[:N, 20] + 1j * dcoef[:N, 21]
+            dQ = dcoef[:N, 22] + 1j * dcoef[:N, 23]
+        da = dP / (1j ** (nn + 1) * en)
+        db = dQ / (1j ** nn * en)
+        scale = np.abs(fd).max()
+        ea = np.abs(da - fd[:, 0]).max() / scale
+        eb = np.abs(db - fd[:, 1]).max() / scale
+        print(f'a_p {a} n_p {n} param {j}: N {N}, tangent vs FD {ea:.1e} {eb:.1e}')
+        assert ea < 2e-6 and eb < 2e-6, j
+
+
 def test_forward_matches_golden(chk, forward_golden):
     g = forward_golden
     _, h, _ = pixels(chk, g['cfg0_params'], grid3((201, 201)))

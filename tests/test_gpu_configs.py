@@ -157,15 +157,14 @@ def test_config3_batched_lm(torch):
     # truth's residuals are the noise itself); the others sit in
     # neighbouring minima, mostly large spheres on small crops (a_p > 2.4 um
     # on 48-80 px crops: the hologram is larger than the crop).  From the
-    # same starts the FP64 oracle's MINPACK LM (1e-12 tolerances) reaches
-    # the same minimum as this LM on 15 of 25 such small crops, a lower one
-    # on 5 and a higher one on 5.
+    # same starts the FP64 oracle's MINPACK LM lands in the same minima
+    # (25 such crops: tests/test_gpu_headline_parity.py).
     n2 = (noise.double() ** 2).cpu().numpy()
     chi2_truth = np.add.reduceat(n2, np.concatenate([[0], np.cumsum(npix)[:-1]]))
     good = ok & (res[:, 35] <= chi2_truth * (1 + 1e-9))
     print(f'fits at or below the chi2 of the truth: {good.mean():.3f}; '
           f'reduced chi2 of the others: median {np.median(redchi[~good]):.2f}')
-    assert good.mean() >= 0.75
+    assert good.mean() >= 0.77
     z = []
     for i in np.flatnonzero(good)[:400]:
         _, dp = stats_from_jtj(res[i, :28], res[i, 35], int(npix[i]), list(range(7)))

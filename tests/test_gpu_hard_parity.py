@@ -15,7 +15,10 @@ From such starts the chi2 surfaces have many neighbouring minima and the
 oracle itself stops in one on many of these features.  Parity is stated
 per feature: the device fit lands in the oracle's minimum (every parameter
 within 1e-4 relative, k_p 1e-5 absolute) or in one whose chi2 is no
-higher than the oracle's (to 1e-6 relative).  lmdif's own forward
+higher than the oracle's (to 1e-6 relative).  Measured: large spheres 13
+of 16 in the oracle's minimum and none higher (14 with differences, one
+higher where the oracle's fit is no fit); layered 15 of 16, none higher
+(16 of 16 with differences).  lmdif's own forward
 differences (LMSettings.fd_step = 1e-5) run the oracle's algorithm on the
 device."""
 import os
@@ -112,7 +115,7 @@ def test_large_fd_jacobian_follows_oracle(large):
     the path)."""
     p, res = run_lm(large, fd_step=1e-5)
     same, crel = compare('large fd', large, p, res)
-    assert same.sum() >= 13
+    assert same.sum() >= 14
     higher = ~same & (crel > 1e-6)
     redchi = large['oracle_chi2'] / (large['index'].shape[1] - 7)
     assert higher.sum() <= 1 and np.all(redchi[higher] > 100.)
@@ -121,19 +124,18 @@ def test_large_fd_jacobian_follows_oracle(large):
 def test_large_analytic_no_worse_than_oracle(large):
     p, res = run_lm(large)
     same, crel = compare('large', large, p, res)
-    assert same.sum() >= 12
-    assert np.sum(~same & (crel > 1e-6)) <= 1
+    assert same.sum() >= 13
+    assert np.sum(~same & (crel > 1e-6)) == 0
 
 
 def test_layered_fd_jacobian_follows_oracle(layered):
     p, res = run_lm(layered, layered=True, fd_step=1e-5)
     same, crel = compare('layered fd', layered, p, res)
-    assert same.sum() >= 15
-    assert np.all(same | (crel <= 1e-6))
+    assert same.all()
 
 
 def test_layered_analytic_no_worse_than_oracle(layered):
     p, res = run_lm(layered, layered=True)
     same, crel = compare('layered', layered, p, res)
-    assert same.sum() >= 13
-    assert np.sum(~same & (crel > 1e-6)) <= 1
+    assert same.sum() >= 15
+    assert np.sum(~same & (crel > 1e-6)) == 0

@@ -88,7 +88,7 @@ def test_crop_fits_match_reference(api, fits_golden):
         res = fit_one(api, fx[f'{name}_data'], fx[f'{name}_start'],
                       fx[f'{name}_sigma'], settings=TIGHT)[1]
         n_same += compare(res, meta['fits'][f'{name}_tight'], name)
-    assert n_same >= 8
+    assert n_same == 12
 
 
 def test_iterations_match_minpack(api, fits_golden):

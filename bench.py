@@ -336,9 +336,16 @@ def oracle_parity(fx, params, res):
     same = np.abs(rel_c) <= 1e-7
     ok = (np.delete(rel, 5, axis=1).max(1) <= 1e-4) & (rel[:, 5] <= 1e-5)
     capped = (res[:n, 38].astype(int) & 8) != 0
+    # the same MINPACK given an exact Jacobian (the oracle's 5-point
+    # differences), the method the device's analytic Jacobian stands for
+    dj = np.abs(params[:n, :7] - fx['oracle7j_values'])
+    relj = dj / np.abs(fx['oracle7j_values'])
+    relj[:, 5] = dj[:, 5]
+    okj = (np.delete(relj, 5, axis=1).max(1) <= 1e-4) & (relj[:, 5] <= 1e-5)
     return {'reference': 'FP64 oracle (numpy model + scipy MINPACK, 1e-12 '
                          'tolerances) fits of the first 32 features, '
                          'tests/golden/headline.npz',
+            'within_1e-4_of_exact_jacobian_minpack': int(okj.sum()),
             'features': int(n),
             'same_minimum': int(same.sum()),
             'same_minimum_within_1e-4': int((same & ok).sum()),

@@ -42,6 +42,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lmz.BatchSizes) == 48
     assert ctypes.sizeof(_lmz.LMSettings) == 64
     assert ctypes.sizeof(_lmz.Schedule) == 16
+    assert ctypes.sizeof(_lmz.DESettings) == 96
 
 
 def _feature(**kw):
@@ -84,6 +85,8 @@ def test_calls_reject_null_batch(lib):
     assert lib.lmz_autotune(None, *([None] * 5), 3, None, None, 0, None, None) == 1
     assert lib.lmz_mie_ab_layered(None, *([None] * 7)) == 1
     assert lib.lmz_hologram_vjp(None, *([None] * 5), 3, *([None] * 4)) == 1
+    assert lib.lmz_de_run(None, None, None, None, None, 3, *([None] * 6)) == 1
+    assert lib.lmz_de_workspace_bytes(None, None, None) == 1
 
 
 def test_frame_calls_reject_bad_arguments(lib):

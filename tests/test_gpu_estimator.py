@@ -199,6 +199,32 @@ def test_device_search_reproducible(api):
     assert start[2].tolist() == [0, 0] and np.all(start[1] >= a[1])
 
 
+def test_device_search_rejects_bad_settings(api):
+    """lmz_de_run checks its settings on the host before any launch."""
+    import ctypes
+    from pylorenzmie_amd import _lmz
+    de_mod, search, probs, truth = _searches(api, sizes=((100, 0.1),))
+    lib = _lmz.load_library()
+    good = dict(maxiter=10, tol=0.01, atol=0., mutation=(0.5, 1.),
+                recombination=0.7, seed=1)
+    n = ctypes.c_size_t()
+    assert lib.lmz_de_workspace_bytes(search.batch.handle, ctypes.byref(
+        search.settings(**good)), ctypes.byref(n)) == 0 and n.value > 0
+    bad = []
+    st = search.settings(**good); st.npop = 4; bad.append(st)        # < 5
+    st = search.settings(**good); st.npop = 7; bad.append(st)        # nfeat % npop
+    st = search.settings(**good); st.cols[1] = 9; bad.append(st)     # no such column
+    st = search.settings(**good); st.cols[1] = st.cols[0]; bad.append(st)
+    st = search.settings(**good); st.ndim = 0; bad.append(st)
+    st = search.settings(**dict(good, mutation=(1.0, 0.5))); bad.append(st)
+    st = search.settings(**dict(good, recombination=1.5)); bad.append(st)
+    st = search.settings(**good); st.check_every = 0; bad.append(st)
+    for st in bad:
+        assert lib.lmz_de_workspace_bytes(search.batch.handle, ctypes.byref(st),
+                                          ctypes.byref(n)) == 1   # LMZ_EINVAL
+        print(lib.lmz_last_error())
+
+
 # ------------------------------------------------------- RadialEstimator
 def test_radial_estimator_contract(api):
     """tests/test_radialestimator.py of the reference: a Series with the

@@ -572,14 +572,17 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
         cmac(Stc, dpn, mQX);
         cmac(Spc, dpn, PD);
         cmac(Spc, dtau, mQX);
-        // coefficient tangents
+        // coefficient tangents; dPR_q pi xi as dPR_q (pi xi), the product
+        // pi xi shared by both directions (four FMAs a direction, not two
+        // multiplies and four FMAs)
+        const cf pxi{pn * xi.re, pn * xi.im};
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
             const float *d = dp + 9 * q;
-            const cf a1 = cmul3(d[0], d[1], d[2], xi);
             const cf a2 = cmul3(d[3], d[4], d[5], D);
             const cf b3 = cmul3(d[6], d[7], d[8], xi);   // -dQ xi
-            cmac(Tr[q], pn, a1);
+            Tr[q].re = fmaf(d[0], pxi.re, fmaf(d[2], pxi.im, Tr[q].re));
+            Tr[q].im = fmaf(d[0], pxi.im, fmaf(d[1], pxi.re, Tr[q].im));
             cmac(Tt[q], tau, a2);
             cmac(Tt[q], pn, b3);
             cmac(Tp[q], pn, a2);

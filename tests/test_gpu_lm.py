@@ -229,3 +229,44 @@ def test_robust_fit_matches_oracle(api):
                                             0.05 * want['d' + v]), v
     err = lambda r: sum(((r[v] - truth[v]) / want['d' + v]) ** 2 for v in V5)
     assert err(got) < err(fits[False])
+
+
+def test_graph_replays_equal_eager_launches(api, fits_golden):
+    """The LM's iteration graph (captured once, replayed up to max_iter
+    times per run, reused across runs) gives exactly the result of the same
+    launches issued one by one, on every run: captured, replayed with
+    eager device work in between, eager, and replayed again.  Nothing in
+    the graph may depend on memory only a first replay sees (a captured
+    memset, a buffer left by another run)."""
+    import torch
+    from pylorenzmie_amd import _lmz
+    from pylorenzmie_amd.analysis.lm import LMBatch, LMSettings
+    theory, analysis = api
+    fx, _ = fits_golden
+    names = [f'crop{i:02d}' for i in range(12)] + ['cfg0']
+    inst = theory.Instrument(**INSTRUMENT)
+    crops, starts = [], []
+    for n in names:
+        d = fx[f'{n}_data'].astype(np.float32)
+        p = fx[f'{n}_start']
+        crops.append(dict(data=d, width=d.shape[1], origin=(0., 0., 0.),
+                          noise=float(fx[f'{n}_sigma']),
+                          nmax_cap=_lmz.nmax_cap(p[3], p[4], p[5], inst.n_m,
+                                                 inst.wavelength)))
+        starts.append(np.append(p, 1.0))
+    lm = LMBatch(crops, inst)
+    runs = []
+    for use_graph in (True, True, False, True):
+        lm.load(np.array(starts))
+        # unrelated device work between runs
+        junk = torch.randn(1 << 20, device='cuda')
+        junk.mul_(2.).add_(1.)
+        lm.run(LMSettings(max_iter=50, ftol=1e-10, xtol=1e-10, gtol=1e-10,
+                          use_graph=use_graph), VARS6 + ['alpha'])
+        runs.append(lm.fetch())
+    lm.close()
+    p0, r0 = runs[0]
+    assert ((r0[:, 38].astype(int) & 7) != 0).sum() >= 10
+    for p, r in runs[1:]:
+        np.testing.assert_array_equal(p, p0)
+        np.testing.assert_array_equal(r, r0)

@@ -209,29 +209,48 @@ __global__ __launch_bounds__(64) void k_coef_from_ab(
 // count[0]) is split into LMZ_XCDS contiguous chunks, chunk x fetched
 // through counter count[2 + x] (set to 0 by k_tile_starts).  A block starts on
 // the chunk of its own XCD, so the tiles of one feature stay in one L2, and
-// moves on to the other chunks when its own is exhausted.  `k` (thread 0's
-// count of exhausted chunks, initially 0) only grows, so every block
-// reaches the end value count[0] after at most count[0] + LMZ_XCDS fetches.
-__device__ __forceinline__ int next_tile(int32_t *count, int &slot, int &k) {
+// moves on to the other chunks when its own is exhausted.  Each fetch claims
+// LMZ_CLAIM consecutive entries (fewer at a chunk's end), which the block
+// then takes one by one from its cursor without a fetch or a barrier.
+// `k` (thread 0's count of exhausted chunks, initially 0) only grows, so
+// every block reaches the end value count[0] after at most count[0] +
+// LMZ_XCDS fetches.
+#ifndef LMZ_CLAIM
+#define LMZ_CLAIM 2
+#endif
+struct TileCursor {
+    int cur = 0, end = 0;   // claimed entries [cur, end), the same in every thread
+    int k = 0;              // thread 0: chunks found exhausted
+};
+
+__device__ __forceinline__ int next_tile(int32_t *count, int (&slot)[2],
+                                         TileCursor &c) {
+    if (c.cur < c.end) return c.cur++;
     if (threadIdx.x == 0) {
         const int n = count[0];
         const int q = n / LMZ_XCDS, r = n % LMZ_XCDS;
         const int home = (int)(blockIdx.x % LMZ_XCDS);
-        int w = n;
-        while (k < LMZ_XCDS) {
-            const int x = (home + k) % LMZ_XCDS;
-            const int i = atomicAdd(count + 2 + x, 1);
-            if (i < q + (x < r ? 1 : 0)) {
-                w = x * q + (x < r ? x : r) + i;
+        int w = n, e = n;
+        while (c.k < LMZ_XCDS) {
+            const int x = (home + c.k) % LMZ_XCDS;
+            const int len = q + (x < r ? 1 : 0);
+            const int i = atomicAdd(count + 2 + x, LMZ_CLAIM);
+            if (i < len) {
+                const int base = x * q + (x < r ? x : r);
+                w = base + i;
+                e = base + (i + LMZ_CLAIM < len ? i + LMZ_CLAIM : len);
                 break;
             }
-            ++k;
+            ++c.k;
         }
-        slot = w;
+        slot[0] = w;
+        slot[1] = e;
     }
     __syncthreads();
-    const int w = slot;
+    const int w = slot[0], e = slot[1];
     __syncthreads();
+    c.cur = w + 1;
+    c.end = e;
     return w;
 }
 
@@ -255,13 +274,13 @@ __global__ __launch_bounds__(NT) void k_forward(
     float *__restrict__ out, long long out_stride, cf *__restrict__ field,
     double *__restrict__ chi2_tiles, const uint8_t *__restrict__ active,
     const int32_t *__restrict__ list, int32_t *count) {
-  __shared__ int next_w;
-  int chunks_done = 0;
+  __shared__ int next_w[2];
+  TileCursor cursor;
   const int nwork = count ? count[0] : (int)gridDim.x;
-  for (int w = list ? next_tile(count, next_w, chunks_done)
+  for (int w = list ? next_tile(count, next_w, cursor)
                     : xcd_tile((int)blockIdx.x, (int)gridDim.x);
        w < nwork;
-       w = list ? next_tile(count, next_w, chunks_done) : nwork) {
+       w = list ? next_tile(count, next_w, cursor) : nwork) {
     const int t = list ? list[w] : w;
     const int f = tile_feature[t];
     if (active && !active[f]) continue;
@@ -344,18 +363,18 @@ __global__ __launch_bounds__(NT) void k_jacobian(
     float *__restrict__ resid, double *__restrict__ partials,
     const uint8_t *__restrict__ active, const int32_t *__restrict__ list,
     int32_t *count) {
-  __shared__ int next_w;
+  __shared__ int next_w[2];
   // Each thread's normal-equation partials live in LDS (its own column of
   // acc_s), not in registers: the 36 registers they would hold keep the
   // kernel below the register budget of four waves per SIMD.
   __shared__ float acc_s[LMZ_NRED][NT];
   float *mine = &acc_s[0][threadIdx.x];
-  int chunks_done = 0;
+  TileCursor cursor;
   const int nwork = count ? count[0] : (int)gridDim.x;
-  for (int w = list ? next_tile(count, next_w, chunks_done)
+  for (int w = list ? next_tile(count, next_w, cursor)
                     : xcd_tile((int)blockIdx.x, (int)gridDim.x);
        w < nwork;
-       w = list ? next_tile(count, next_w, chunks_done) : nwork) {
+       w = list ? next_tile(count, next_w, cursor) : nwork) {
     const int t = list ? list[w] : w;
     const int f = tile_feature[t];
     if (active && !active[f]) continue;
@@ -475,13 +494,13 @@ __global__ __launch_bounds__(NT) void k_fd_jacobian(
     const float *__restrict__ rin, long long rin_stride,
     double *__restrict__ partials, const int32_t *__restrict__ list,
     int32_t *count) {
-  __shared__ int next_w;
-  int chunks_done = 0;
+  __shared__ int next_w[2];
+  TileCursor cursor;
   const int nwork = count ? count[0] : (int)gridDim.x;
-  for (int w = list ? next_tile(count, next_w, chunks_done)
+  for (int w = list ? next_tile(count, next_w, cursor)
                     : xcd_tile((int)blockIdx.x, (int)gridDim.x);
        w < nwork;
-       w = list ? next_tile(count, next_w, chunks_done) : nwork) {
+       w = list ? next_tile(count, next_w, cursor) : nwork) {
     const int t = list ? list[w] : w;
     const int f = tile_feature[t];
     const lmz_feature &F = feat[f];

@@ -238,6 +238,8 @@ LMZ_HD void pixel_geometry(const FeatUniform &fu, float ax, float ay, float az,
 //   [16..27] PR = i^(n+1)(2n+1) a_n, P, Q                        (FP64)
 //   [28] 1 + 1/n   [29] -n(n+1)/2   [30] (n+1)(n+2)/2   [31] 2n - 1
 //   [32] -n                                                       (FP32)
+//   [33..35] sigma_n P (re, im, -im)   [36..38] -sigma_n Q (re, im, -im)
+//   [39] 1 - 2n                  (FP32, the Jacobian's signed loop, jac_order)
 // E_n = i^n (2n+1)/(n(n+1)) as theory/LorenzMie.py:297-303.  The FP32
 // entries carry the signs the per-pixel loops need, so that every complex
 // product is a multiply and a multiply-add into its own result and every
@@ -246,7 +248,8 @@ LMZ_HD void pixel_geometry(const FeatUniform &fu, float ax, float ay, float az,
 // with a negated operand (VOP3) issue one at a time.
 #define LMZ_COEF_STRIDE 40
 // dcoef row: for each of a_p and n_p (9 floats each, at 0 and 9): the
-// tangents of PR (re, im, -im), of P (re, im, -im) and of -Q (re, im, -im);
+// tangents of PR (re, im, -im), of P (re, im, -im) and of -Q (re, im, -im),
+// times sigma_n (order_sign);
 // then the k_p tangents of PR, P, Q (re, im) at 18..23.
 #define LMZ_DCOEF_STRIDE 24
 
@@ -513,86 +516,118 @@ struct JacPixel {
 
 #define LMZ_RHO_AXIS 3.0e-4f   // k rho below which K is taken as 0
 
+// State of the Jacobian's order loop.
+struct JacLoop {
+    // Sums, each accumulating products of positive factors (see the table
+    // layout): mSrR = -S_r,R, mStR = -S_t,R, mSpR = -S_p,R.
+    cf Sr, St, Sp, mSrR, mStR, mSpR, Src, Stc, Spc;
+    cf Tr[2], Tt[2], Tp[2];      // coefficient tangents: a_p, n_p
+    Angular32 ang;
+    float dpm, dpn;              // d pi_{n-1} / dc, d pi_n / dc
+    cf x1, x2;                   // sigma xi of orders n - 1, n - 2
+};
+
+// One order n of the Jacobian's loop; ODD = n odd.  The Riccati-Bessel
+// functions are carried as sigma_n xi_n and sigma_n D_n with the period-4
+// sign sigma_n = +, +, -, -, ... for n = -1, 0, 1, 2, ... (order_sign), so
+// that xi's three-term recurrence becomes sigma_n xi_n = s_n (2n-1)/kr
+// sigma_{n-1} xi_{n-1} + sigma_{n-2} xi_{n-2} with s_n = sigma_n
+// sigma_{n-1} = (-1)^n: a two-operand FMA into the register of the order
+// that is no longer needed.  sigma_n D_n = n/kr sigma_n xi_n - s_n
+// sigma_{n-1} xi_{n-1}.  The coefficient rows the products read (slots
+// 33-38 and the dcoef triples) carry sigma_n, so every product is that of
+// the unsigned functions.
+template <bool ODD>
+LMZ_HD void jac_order(JacLoop &L, int n, const float *coef, const float *dcoef,
+                      float w, float cl, float invRl, float invR2) {
+    const float *cp = coef + LMZ_COEF_STRIDE * (n - 1);
+    const float *dp = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
+    const float nn1 = cp[7];
+    const float c1 = cp[28];
+    float pn, tau;
+    L.ang.step(w, cp, pn, tau);
+    // tangent of Wiscombe's recurrence with respect to c
+    const float dSv = fmaf(L.dpn, cl, pn);
+    const float dT = dSv - L.dpm;
+    const float dtau = fmaf(cp[32], dT, L.dpm);     // dpm - n dT
+    const float ninv = cp[6] * invRl;
+    const float cx = (ODD ? cp[39] : cp[31]) * invRl;   // s_n (2n - 1) / kr
+    const cf xi{fmaf(cx, L.x1.re, L.x2.re), fmaf(cx, L.x1.im, L.x2.im)};
+    const cf D = ODD ? cf{fmaf(ninv, xi.re, L.x1.re), fmaf(ninv, xi.im, L.x1.im)}
+                     : cf{fmaf(ninv, xi.re, -L.x1.re), fmaf(ninv, xi.im, -L.x1.im)};
+    const float mgn = fmaf(nn1, invR2, -1.f);      // -(1 - n(n+1)/R^2)
+    // PR = n(n+1) P: the S_r terms use P xi and P D with pi_n n(n+1)
+    const cf PD = cmul3(cp[33], cp[34], cp[35], D);
+    const cf PX = cmul3(cp[33], cp[34], cp[35], xi);
+    const cf mQX = cmul3(cp[36], cp[37], cp[38], xi); // -Q xi
+    const cf mQD = cmul3(cp[36], cp[37], cp[38], D);  // -Q D
+    const float pnn = pn * nn1;
+    // value
+    cmac(L.Sr, pnn, PX);
+    cmac(L.St, tau, PD);
+    cmac(L.St, pn, mQX);
+    cmac(L.Sp, pn, PD);
+    cmac(L.Sp, tau, mQX);
+    // d/dR (negated)
+    cmac(L.mSrR, pnn, PD);
+    const float mtg = tau * mgn, mpg = pn * mgn;
+    cmac(L.mStR, mtg, PX);
+    cmac(L.mStR, pn, mQD);
+    cmac(L.mSpR, mpg, PX);
+    cmac(L.mSpR, tau, mQD);
+    // d/dc
+    cmac(L.Src, L.dpn * nn1, PX);
+    cmac(L.Stc, dtau, PD);
+    cmac(L.Stc, L.dpn, mQX);
+    cmac(L.Spc, L.dpn, PD);
+    cmac(L.Spc, dtau, mQX);
+    // coefficient tangents; dPR_q pi xi as dPR_q (pi xi), the product pi xi
+    // shared by both directions (four FMAs a direction, not two multiplies
+    // and four FMAs)
+    const cf pxi{pn * xi.re, pn * xi.im};
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const float *d = dp + 9 * q;
+        const cf a2 = cmul3(d[3], d[4], d[5], D);
+        const cf b3 = cmul3(d[6], d[7], d[8], xi);   // -dQ xi
+        L.Tr[q].re = fmaf(d[0], pxi.re, fmaf(d[2], pxi.im, L.Tr[q].re));
+        L.Tr[q].im = fmaf(d[0], pxi.im, fmaf(d[1], pxi.re, L.Tr[q].im));
+        cmac(L.Tt[q], tau, a2);
+        cmac(L.Tt[q], pn, b3);
+        cmac(L.Tp[q], pn, a2);
+        cmac(L.Tp[q], tau, b3);
+    }
+    L.dpm = L.dpn;
+    L.dpn = fmaf(c1, dT, dSv);
+    L.x2 = L.x1;
+    L.x1 = xi;
+}
+
 LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
                            const float *coef, const float *dcoef, int nmax,
                            JacPixel &out) {
-    // Sums of the loop, each accumulating products of positive factors (see
-    // the table layout): mSrR = -S_r,R, mStR = -S_t,R, mSpR = -S_p,R.
-    cf Sr{0.f, 0.f}, St{0.f, 0.f}, Sp{0.f, 0.f};
-    cf mSrR{0.f, 0.f}, mStR{0.f, 0.f}, mSpR{0.f, 0.f};
-    cf Src{0.f, 0.f}, Stc{0.f, 0.f}, Spc{0.f, 0.f};
-    cf Tr[2], Tt[2], Tp[2];    // coefficient tangents: a_p, n_p
+    JacLoop L;
+    L.Sr = L.St = L.Sp = L.mSrR = L.mStR = L.mSpR = cf{0.f, 0.f};
+    L.Src = L.Stc = L.Spc = cf{0.f, 0.f};
 #pragma unroll
-    for (int q = 0; q < 2; ++q) Tr[q] = Tt[q] = Tp[q] = cf{0.f, 0.f};
-    Angular32 ang;
-    ang.init();
+    for (int q = 0; q < 2; ++q) L.Tr[q] = L.Tt[q] = L.Tp[q] = cf{0.f, 0.f};
+    L.ang.init();
+    L.dpm = 0.f;
+    L.dpn = 0.f;
+    L.x2 = g.xi_m1;    // sigma_{-1} = sigma_0 = 1
+    L.x1 = g.xi_0;
     const float w = g.w;
-    float dpm = 0.f, dpn = 0.f;   // d pi_{n-1} / dc, d pi_n / dc
-    cf x2 = g.xi_m1, x1 = g.xi_0;
     const float cl = g.cl, invRl = g.invRl, invR2 = g.invR * g.invR;
-#pragma unroll 2
-    for (int n = 1; n <= nmax; ++n) {
-        const float *cp = coef + LMZ_COEF_STRIDE * (n - 1);
-        const float *dp = dcoef + LMZ_DCOEF_STRIDE * (n - 1);
-        const float nn1 = cp[7];
-        const float c1 = cp[28];
-        float pn, tau;
-        ang.step(w, cp, pn, tau);
-        // tangent of Wiscombe's recurrence with respect to c
-        const float dSv = fmaf(dpn, cl, pn);
-        const float dT = dSv - dpm;
-        const float dtau = fmaf(cp[32], dT, dpm);     // dpm - n dT
-        const float ninv = cp[6] * invRl;
-        const float cx = cp[31] * invRl;              // (2n - 1) / kr
-        const cf xi{fmaf(cx, x1.re, -x2.re), fmaf(cx, x1.im, -x2.im)};
-        const cf D{fmaf(ninv, xi.re, -x1.re), fmaf(ninv, xi.im, -x1.im)};
-        const float mgn = fmaf(nn1, invR2, -1.f);      // -(1 - n(n+1)/R^2)
-        // PR = n(n+1) P: the S_r terms use P xi and P D with pi_n n(n+1)
-        const cf PD = cmul3(cp[2], cp[3], cp[0], D);
-        const cf PX = cmul3(cp[2], cp[3], cp[0], xi);
-        const cf mQX = cmul3(cp[4], cp[5], cp[1], xi); // -Q xi
-        const cf mQD = cmul3(cp[4], cp[5], cp[1], D);  // -Q D
-        const float pnn = pn * nn1;
-        // value
-        cmac(Sr, pnn, PX);
-        cmac(St, tau, PD);
-        cmac(St, pn, mQX);
-        cmac(Sp, pn, PD);
-        cmac(Sp, tau, mQX);
-        // d/dR (negated)
-        cmac(mSrR, pnn, PD);
-        const float mtg = tau * mgn, mpg = pn * mgn;
-        cmac(mStR, mtg, PX);
-        cmac(mStR, pn, mQD);
-        cmac(mSpR, mpg, PX);
-        cmac(mSpR, tau, mQD);
-        // d/dc
-        cmac(Src, dpn * nn1, PX);
-        cmac(Stc, dtau, PD);
-        cmac(Stc, dpn, mQX);
-        cmac(Spc, dpn, PD);
-        cmac(Spc, dtau, mQX);
-        // coefficient tangents; dPR_q pi xi as dPR_q (pi xi), the product
-        // pi xi shared by both directions (four FMAs a direction, not two
-        // multiplies and four FMAs)
-        const cf pxi{pn * xi.re, pn * xi.im};
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const float *d = dp + 9 * q;
-            const cf a2 = cmul3(d[3], d[4], d[5], D);
-            const cf b3 = cmul3(d[6], d[7], d[8], xi);   // -dQ xi
-            Tr[q].re = fmaf(d[0], pxi.re, fmaf(d[2], pxi.im, Tr[q].re));
-            Tr[q].im = fmaf(d[0], pxi.im, fmaf(d[1], pxi.re, Tr[q].im));
-            cmac(Tt[q], tau, a2);
-            cmac(Tt[q], pn, b3);
-            cmac(Tp[q], pn, a2);
-            cmac(Tp[q], tau, b3);
-        }
-        dpm = dpn;
-        dpn = fmaf(c1, dT, dSv);
-        x2 = x1;
-        x1 = xi;
+    int n = 1;
+    for (; n + 1 <= nmax; n += 2) {
+        jac_order<true>(L, n, coef, dcoef, w, cl, invRl, invR2);
+        jac_order<false>(L, n + 1, coef, dcoef, w, cl, invRl, invR2);
     }
+    if (n <= nmax) jac_order<true>(L, n, coef, dcoef, w, cl, invRl, invR2);
+    cf Sr = L.Sr, St = L.St, Sp = L.Sp;
+    const cf mSrR = L.mSrR, mStR = L.mStR, mSpR = L.mSpR;
+    cf Src = L.Src, Stc = L.Stc, Spc = L.Spc;
+    cf Tr[2] = {L.Tr[0], L.Tr[1]}, Tt[2] = {L.Tt[0], L.Tt[1]}, Tp[2] = {L.Tp[0], L.Tp[1]};
 
     // ---- undo the mirror (S_p, and d/dR, d/dc of S_r and S_t, change sign)
     // and the signs of the negated d/dR sums
@@ -701,6 +736,10 @@ LMZ_HD int wiscombe_yang(double x, cd m) {
     return (int)fmax(ns, xm);
 }
 
+// sigma_n = +1, +1, -1, -1, ... for n = -1, 0, 1, 2, ... (period 4): the
+// sign the Jacobian's Riccati-Bessel functions carry (jac_order).
+LMZ_HD float order_sign(int n) { return ((n + 1) & 2) ? -1.f : 1.f; }
+
 // Output of one order: FP32 table rows and optional FP64 a_n, b_n.
 LMZ_HD void put_d(float *row, int i, double v) { __builtin_memcpy(row + i, &v, sizeof v); }
 
@@ -727,15 +766,24 @@ LMZ_HD void write_order(int n, cd a, cd b, float *crow) {
     crow[30] = (float)(0.5 * (n + 1.0) * (n + 2.0));
     crow[31] = (float)(2.0 * n - 1.0);
     crow[32] = (float)(-n);
-    for (int i = 33; i < LMZ_COEF_STRIDE; ++i) crow[i] = 0.f;
+    // the P and -Q triples of slots 2, 3, 0 and 4, 5, 1 times sigma_n, for
+    // the Jacobian's signed Riccati-Bessel functions (jac_order)
+    const float sg = order_sign(n);
+    crow[33] = sg * crow[2]; crow[34] = sg * crow[3]; crow[35] = sg * crow[0];
+    crow[36] = sg * crow[4]; crow[37] = sg * crow[5]; crow[38] = sg * crow[1];
+    crow[39] = (float)(1.0 - 2.0 * n);   // s_n (2n - 1) for odd n
+    for (int i = 40; i < LMZ_COEF_STRIDE; ++i) crow[i] = 0.f;
 }
 
 // Tangent rows: q = 0 (a_p), 1 (n_p) as (Re, Im, -Im) triples of PR, P,
-// -Q at 9 q; write_dorder_k: the k_p tangents (Re, Im of PR, P, Q) at 18.
+// -Q at 9 q, times sigma_n (order_sign: the Jacobian reads them with its
+// signed Riccati-Bessel functions); write_dorder_k: the k_p tangents (Re,
+// Im of PR, P, Q) at 18, unsigned.
 LMZ_HD void write_dorder(int n, cd da, cd db, float *drow) {
     const double en = (2.0 * n + 1.0) / ((double)n * (n + 1.0));
-    const cd P = mul_ipow(n + 1, en * da);
-    const cd Q = mul_ipow(n, en * db);
+    const double sg = order_sign(n);
+    const cd P = mul_ipow(n + 1, (sg * en) * da);
+    const cd Q = mul_ipow(n, (sg * en) * db);
     const cd PR = ((double)n * (n + 1.0)) * P;
     drow[0] = (float)PR.re; drow[1] = (float)PR.im; drow[2] = -(float)PR.im;
     drow[3] = (float)P.re;  drow[4] = (float)P.im;  drow[5] = -(float)P.im;

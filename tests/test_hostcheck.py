@@ -16,6 +16,12 @@ COEF_STRIDE = 40
 DCOEF_STRIDE = 24
 
 
+def order_sign(nn):
+    """sigma_n = +1, +1, -1, -1, ... for n = -1, 0, 1, 2, ...: the sign the
+    a_p and n_p tangent rows of dcoef carry (lmz_pixel.h order_sign)."""
+    return np.where(((np.asarray(nn) + 1) & 2) != 0, -1., 1.)
+
+
 @pytest.fixture(scope='module')
 def chk():
     if not os.path.exists(LIB):
@@ -107,8 +113,9 @@ def test_mie_matches_golden_and_tangents(chk, forward_golden):
         fd = fd[1:] / h
         if j < 2:   # (Re, Im, -Im) triples of dPR, dP, -dQ at 9 j
             b = 9 * j
-            dP = dcoef[:N, b + 3] + 1j * dcoef[:N, b + 4]
-            dQ = -(dcoef[:N, b + 6] + 1j * dcoef[:N, b + 7])
+            sg = order_sign(nn)
+            dP = sg * (dcoef[:N, b + 3] + 1j * dcoef[:N, b + 4])
+            dQ = -sg * (dcoef[:N, b + 6] + 1j * dcoef[:N, b + 7])
             assert np.array_equal(dcoef[:N, b + 5], -dcoef[:N, b + 4])
             assert np.array_equal(dcoef[:N, b + 8], -dcoef[:N, b + 7])
         else:       # k_p: (Re, Im) of dPR, dP, dQ at 18
@@ -145,8 +152,9 @@ def test_large_sphere_tangents_match_fine_differences(chk, a, n, k):
         fd = fd[1:] / h
         if j < 2:
             b = 9 * j
-            dP = dcoef[:N, b + 3] + 1j * dcoef[:N, b + 4]
-            dQ = -(dcoef[:N, b + 6] + 1j * dcoef[:N, b + 7])
+            sg = order_sign(nn)
+            dP = sg * (dcoef[:N, b + 3] + 1j * dcoef[:N, b + 4])
+            dQ = -sg * (dcoef[:N, b + 6] + 1j * dcoef[:N, b + 7])
         else:
             dP = dcoef[:N, 20] + 1j * dcoef[:N, 21]
             dQ = dcoef[:N, 22] + 1j * dcoef[:N, 23]
@@ -314,8 +322,9 @@ def test_layered_tangents_match_differences(chk, seed):
             fd = fd + w * (coeffs(r_p, n_p, k_p) - coeffs(r_m, n_m, k_m))
         fd = fd / h
         if base < 18:
-            dP = dcoef[:N, base + 3] + 1j * dcoef[:N, base + 4]
-            dQ = -(dcoef[:N, base + 6] + 1j * dcoef[:N, base + 7])
+            sg = order_sign(np.arange(1, N + 1))
+            dP = sg * (dcoef[:N, base + 3] + 1j * dcoef[:N, base + 4])
+            dQ = -sg * (dcoef[:N, base + 6] + 1j * dcoef[:N, base + 7])
         else:
             dP = dcoef[:N, 20] + 1j * dcoef[:N, 21]
             dQ = dcoef[:N, 22] + 1j * dcoef[:N, 23]

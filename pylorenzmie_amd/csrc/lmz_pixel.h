@@ -855,13 +855,20 @@ LMZ_HD void mie_up_chain(double x, int N, double *work, long long wstride) {
     cd pz{0.5 * (1.0 - c2x), -0.5 * s2x};            // Eq. 18a
     cd dpz{s2x, -c2x};                               // d/dx = -i e^{2ix}
     cd d3{0.0, 1.0}, dd3{0.0, 0.0};                  // Eq. 18b
-    LMZ_W(0, 8) = psi; LMZ_W(0, 9) = dpsi;
-    LMZ_W(0, 10) = zeta.re; LMZ_W(0, 11) = zeta.im;
-    LMZ_W(0, 12) = dzeta.re; LMZ_W(0, 13) = dzeta.im;
     for (int n = 1; n <= N; ++n) {
+        // D1 and dD1 of orders n - 1 and n are read before the stores of
+        // order n - 1, which are held from the previous pass: a load issued
+        // after stores waits for them to complete (one memory counter
+        // covers both), which put two memory round trips on the chain's
+        // critical path per order.
+        const double d1m = LMZ_W(n - 1, 4), dd1m = LMZ_W(n - 1, 6);
+        const double d1n = LMZ_W(n, 4), dd1n = LMZ_W(n, 6);
+        LMZ_W(n - 1, 8) = psi; LMZ_W(n - 1, 9) = dpsi;
+        LMZ_W(n - 1, 10) = zeta.re; LMZ_W(n - 1, 11) = zeta.im;
+        LMZ_W(n - 1, 12) = dzeta.re; LMZ_W(n - 1, 13) = dzeta.im;
         const double nx = n / x, dnx = -n / (x * x);
-        const double w1 = nx - LMZ_W(n - 1, 4);
-        const double dw1 = dnx - LMZ_W(n - 1, 6);
+        const double w1 = nx - d1m;
+        const double dw1 = dnx - dd1m;
         const cd w3 = cd{nx, 0.0} - d3;
         const cd dw3 = cd{dnx, 0.0} - dd3;
         const double psi_m = psi;
@@ -874,12 +881,12 @@ LMZ_HD void mie_up_chain(double x, int N, double *work, long long wstride) {
         pz = pz_m * (w1 * w3);                         // Eq. 18c
         dpz = dpz * (w1 * w3) + pz_m * (dw1 * w3 + w1 * dw3);
         const cd ipz = cinv(pz);
-        d3 = cd{LMZ_W(n, 4), 0.0} + mul_i(ipz);        // Eq. 18d
-        dd3 = cd{LMZ_W(n, 6), 0.0} - mul_i(dpz * (ipz * ipz));
-        LMZ_W(n, 8) = psi; LMZ_W(n, 9) = dpsi;
-        LMZ_W(n, 10) = zeta.re; LMZ_W(n, 11) = zeta.im;
-        LMZ_W(n, 12) = dzeta.re; LMZ_W(n, 13) = dzeta.im;
+        d3 = cd{d1n, 0.0} + mul_i(ipz);                // Eq. 18d
+        dd3 = cd{dd1n, 0.0} - mul_i(dpz * (ipz * ipz));
     }
+    LMZ_W(N, 8) = psi; LMZ_W(N, 9) = dpsi;
+    LMZ_W(N, 10) = zeta.re; LMZ_W(N, 11) = zeta.im;
+    LMZ_W(N, 12) = dzeta.re; LMZ_W(N, 13) = dzeta.im;
 }
 
 // Phase 3: a_n, b_n (Eqs. 5 and 6 with H_a = H_b = D1(m x)) and their

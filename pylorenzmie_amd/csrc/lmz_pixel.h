@@ -624,96 +624,81 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
         jac_order<false>(L, n + 1, coef, dcoef, w, cl, invRl, invR2);
     }
     if (n <= nmax) jac_order<true>(L, n, coef, dcoef, w, cl, invRl, invR2);
-    cf Sr = L.Sr, St = L.St, Sp = L.Sp;
-    const cf mSrR = L.mSrR, mStR = L.mStR, mSpR = L.mSpR;
-    cf Src = L.Src, Stc = L.Stc, Spc = L.Spc;
-    cf Tr[2] = {L.Tr[0], L.Tr[1]}, Tt[2] = {L.Tt[0], L.Tt[1]}, Tp[2] = {L.Tp[0], L.Tp[1]};
-
-    // ---- undo the mirror (S_p, and d/dR, d/dc of S_r and S_t, change sign)
-    // and the signs of the negated d/dR sums
-    cf SrR, StR, SpR;
-    {
-        const float m = g.mir;
-        Sp = m * Sp;
-        SrR = (-m) * mSrR; StR = (-m) * mStR;
-        SpR = -1.f * mSpR;
-        Src = m * Src; Stc = m * Stc;
-#pragma unroll
-        for (int q = 0; q < 2; ++q) Tp[q] = m * Tp[q];
-    }
-    // ---- field and its (R, c) partials
+    // ---- projection by adjoint weights.  The field is E = M S, S = (S_r,
+    // S_t, S_p), with the real 3 x 3 geometry matrix of field_from_sums:
+    //   E_x = u^2 A - v^2 B,  E_y = u v (A + B),  E_z = u s C,
+    //   A = a_r S_r + a_t S_t,  B = S_p / R,  C = c_r S_r - S_t / R.
+    // Every derivative of I = |F|^2, F = x_hat + alpha E, is 2 alpha
+    // Re[conj(F) . dE], and Re[conj(F) . M X] = sum_k credot(g_k, X_k) with
+    // g = M^T F:  g_r = a_r W + c_r U, g_t = a_t W - U / R, g_p = V / R for
+    // W = u^2 F_x + u v F_y, V = u v F_y - v^2 F_x, U = u s F_z.  So the sums'
+    // derivatives (d/dR, d/dc, the coefficient tangents) enter through three
+    // complex dot products each, and the geometry's own derivatives through
+    // a few scalars.  The loop's S_p, T_p and d/dc of S_r, S_t carry the
+    // mirror sign m, its d/dR sums are negated (and those of S_r, S_t carry
+    // m as well); both are folded into the weights.
+    const cf Sr = L.Sr, St = L.St, SpL = L.Sp;
+    const float m = g.mir;
     const float s = g.s, s2 = g.s * g.s, u = g.u, v = g.v;
     const float c = g.c, invR = g.invR;
-    const float u2 = u * u, v2 = v * v, uv = u * v;
+    const float u2 = u * u, v2 = v * v, uv = u * v, us = u * s;
     const float a_r = s2 * invR2, a_t = c * invR;          // A = a_r Sr + a_t St
     const float c_r = c * invR2;                            // C = c_r Sr - invR St
     const cf A = a_r * Sr + a_t * St;
-    const cf B = invR * Sp;
+    const cf B = (m * invR) * SpL;
     const cf C = c_r * Sr - invR * St;
+    const cf ApB = A + B;
     out.E[0] = u2 * A - v2 * B;
-    out.E[1] = uv * (A + B);
-    out.E[2] = (u * s) * C;
-
-    const float twoR = 2.f * invR;
-    const cf A_R = a_r * SrR - (a_r * twoR) * Sr + a_t * StR - (a_t * invR) * St;
-    const cf A_c = a_r * Src - ((2.f * c) * invR2) * Sr + a_t * Stc + invR * St;
-    const cf B_R = invR * SpR - invR2 * Sp;
-    const cf B_c = invR * Spc;
-    const cf C_R = c_r * SrR - (c_r * twoR) * Sr - invR * StR + invR2 * St;
-    const cf C_c = c_r * Src + invR2 * Sr - invR * Stc;
-
-    // chain rule to X, Y, Z
-    const float R_X = s * u, R_Y = s * v, R_Z = c;
-    const float csR = c * s * invR;
-    const float c_X = -csR * u, c_Y = -csR * v, c_Z = s2 * invR;
-    const cf A_X = R_X * A_R + c_X * A_c, A_Y = R_Y * A_R + c_Y * A_c,
-             A_Z = R_Z * A_R + c_Z * A_c;
-    const cf B_X = R_X * B_R + c_X * B_c, B_Y = R_Y * B_R + c_Y * B_c,
-             B_Z = R_Z * B_R + c_Z * B_c;
-    const cf C_X = R_X * C_R + c_X * C_c, C_Y = R_Y * C_R + c_Y * C_c,
-             C_Z = R_Z * C_R + c_Z * C_c;
-    const cf K = (g.rho > LMZ_RHO_AXIS) ? (1.f / g.rho) * (A + B) : cf{0.f, 0.f};
-    const float us = u * s;
-    cf dE[3][3];  // [q][component]
-    dE[0][0] = (2.f * u * v2) * K + u2 * A_X - v2 * B_X;
-    dE[1][0] = (-2.f * u2 * v) * K + u2 * A_Y - v2 * B_Y;
-    dE[2][0] = u2 * A_Z - v2 * B_Z;
-    dE[0][1] = (v * (v2 - u2)) * K + uv * (A_X + B_X);
-    dE[1][1] = (u * (u2 - v2)) * K + uv * (A_Y + B_Y);
-    dE[2][1] = uv * (A_Z + B_Z);
-    dE[0][2] = ((v2 + u2 * c * c) * invR) * C + us * C_X;
-    dE[1][2] = (-uv * s2 * invR) * C + us * C_Y;
-    dE[2][2] = (-us * c * invR) * C + us * C_Z;
-
-    // ---- intensity and its derivatives
+    out.E[1] = uv * ApB;
+    out.E[2] = us * C;
     const float al = fu.alpha;
     const cf F0{fmaf(al, out.E[0].re, 1.f), al * out.E[0].im};
     const cf F1 = al * out.E[1], F2 = al * out.E[2];
     out.I = credot(F0, F0) + credot(F1, F1) + credot(F2, F2);
-    const float two_al = 2.f * al;
-    float dI[3];
+
+    const cf W = u2 * F0 + uv * F1, V = uv * F1 - v2 * F0, U = us * F2;
+    const cf gr = a_r * W + c_r * U, gt = a_t * W - invR * U, gp = invR * V;
+    const cf gpm = m * gp;
+    const float rr = credot(gr, Sr), rt = credot(gt, St), rp = credot(gpm, SpL);
+    // d/dR: the sums' partials, and the explicit R^-2 (S_r) and R^-1 (S_t,
+    // S_p) factors of M
+    const float GR =
+        -m * (credot(gr, L.mSrR) + credot(gt, L.mStR)) - credot(gp, L.mSpR) -
+        invR * (2.f * rr + rt + rp);
+    // d/dc: the sums' partials, and d a_r/dc = -2 c / R^2, d a_t/dc = 1/R,
+    // d c_r/dc = 1/R^2 (through W and U)
+    const cf er = invR2 * U - (2.f * c_r) * W, et = invR * W;
+    const float Gc = m * (credot(gr, L.Src) + credot(gt, L.Stc)) +
+                     credot(gp, L.Spc) + credot(er, Sr) + credot(et, St);
+    // derivatives of the prefactors u^2, v^2, u v (K = (A + B) / rho, zero on
+    // the axis, where dI/dX = dI/dY = 0) and u s
+    const float irho = (g.rho > LMZ_RHO_AXIS) ? 1.f / g.rho : 0.f;
+    const float k0 = irho * credot(F0, ApB), k1 = irho * credot(F1, ApB);
+    const float tz = invR * credot(F2, C);
+    const float PX = (2.f * u * v2) * k0 + (v * (v2 - u2)) * k1 + (v2 + u2 * c * c) * tz;
+    const float PY = (-2.f * u2 * v) * k0 + (u * (u2 - v2)) * k1 - (uv * s2) * tz;
+    const float PZ = -(us * c) * tz;
+    // chain rule to X, Y, Z: R_X = s u, R_Y = s v, R_Z = c; c_X = -c s u / R,
+    // c_Y = -c s v / R, c_Z = s^2 / R
+    const float csR = c * s * invR;
+    const float DX = s * u * GR - csR * u * Gc + PX;
+    const float DY = s * v * GR - csR * v * Gc + PY;
+    const float DZ = c * GR + s2 * invR * Gc + PZ;
+    const float two_al = 2.f * al, k2a = fu.k * two_al;
+    out.J[0] = -k2a * DX;
+    out.J[1] = -k2a * DY;
+    out.J[2] = k2a * (out.E[0].im + DZ);
+    // a_p, n_p tangents; k_p: dE/dk_p = i dE/dn_p (the n_p sums times i)
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
-        dI[q] = two_al * (credot(F0, dE[q][0]) + credot(F1, dE[q][1]) +
-                          credot(F2, dE[q][2]));
-    const float k = fu.k;
-    out.J[0] = -k * dI[0];
-    out.J[1] = -k * dI[1];
-    out.J[2] = k * fmaf(two_al, out.E[0].im, dI[2]);
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const cf Aq = a_r * Tr[q] + a_t * Tt[q];
-        const cf Bq = invR * Tp[q];
-        const cf Cq = c_r * Tr[q] - invR * Tt[q];
-        const cf Ex = u2 * Aq - v2 * Bq, Ey = uv * (Aq + Bq), Ez = us * Cq;
-        out.J[3 + q] = two_al * (credot(F0, Ex) + credot(F1, Ey) + credot(F2, Ez));
-        if (q == 1) {  // k_p: dE/dk_p = i dE/dn_p
-            const cf iEx{-Ex.im, Ex.re}, iEy{-Ey.im, Ey.re}, iEz{-Ez.im, Ez.re};
-            out.J[5] = two_al * (credot(F0, iEx) + credot(F1, iEy) + credot(F2, iEz));
-        }
+    for (int q = 0; q < 2; ++q)
+        out.J[3 + q] = two_al * (credot(gr, L.Tr[q]) + credot(gt, L.Tt[q]) +
+                                 credot(gpm, L.Tp[q]));
+    {
+        const cf iTr{-L.Tr[1].im, L.Tr[1].re}, iTt{-L.Tt[1].im, L.Tt[1].re},
+            iTp{-L.Tp[1].im, L.Tp[1].re};
+        out.J[5] = two_al * (credot(gr, iTr) + credot(gt, iTt) + credot(gpm, iTp));
     }
-    out.J[6] = 2.f * (credot(F0, out.E[0]) + credot(F1, out.E[1]) +
-                      credot(F2, out.E[2]));
+    out.J[6] = 2.f * (rr + rt + rp);
 }
 
 // ====================================================================

@@ -399,7 +399,7 @@ __global__ __launch_bounds__(NT) void k_jacobian(
         float ax, ay, az;
         pixel_coords(fu, j, coords, npix_total, ax, ay, az);
         PixGeom g;
-        pixel_geometry(fu, ax, ay, az, bohren, g);
+        pixel_geometry<true>(fu, ax, ay, az, bohren, g);
         JacPixel jp;
         jacobian_pixel(fu, g, cf0, df0, fu.nmax, jp);
         const long long gi = fu.pix_offset + j;
@@ -671,7 +671,7 @@ __global__ __launch_bounds__(LMZ_THREADS) void k_vjp(
         float ax, ay, az;
         pixel_coords(fu, j, coords, npix_total, ax, ay, az);
         PixGeom pg;
-        pixel_geometry(fu, ax, ay, az, bohren, pg);
+        pixel_geometry<true>(fu, ax, ay, az, bohren, pg);
         JacPixel jp;
         jacobian_pixel(fu, pg, cf0, df0, fu.nmax, jp);
         const float gv = g[fu.pix_offset + j];

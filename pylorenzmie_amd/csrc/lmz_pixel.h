@@ -158,10 +158,30 @@ LMZ_HD void pixel_coords(const FeatUniform &fu, int j, const float *coords,
 // linear with real coefficients, so every xi_n carries the same factor).
 // For the outgoing wave the phase k (r - z_p) is formed as
 // (rho^2 / (r + dz) - z) k, free of cancellation near the axis.
+// 1 / sqrt(x) and 1 / x by the hardware approximations (within 1 ulp) in
+// the kernels; the host build of the checker divides.  For the Jacobian's
+// FP32 geometry, where a correctly rounded sqrtf or division costs ten or
+// more instructions per pixel.
+LMZ_HD float rsqrt_hw(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / std::sqrt(x);
+#endif
+}
+
+LMZ_HD float rcp_hw(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 struct PixGeom {
     float X, Y, Z;      // k (x - x_p), k (y - y_p), k (z_p - z)
     float R, invR;      // kr, 1 / kr
-    float rho;          // k rho
+    float rho, irho;    // k rho, 1 / (k rho) (0 on the axis)
     float c, s;         // cos theta, sin theta
     float u, v;         // cos phi, sin phi
     // Recurrence inputs.  For cos theta < 0 the recurrences run on the mirror
@@ -175,6 +195,11 @@ struct PixGeom {
     cf xi_m1, xi_0;     // recurrence start, times exp(-i k z_p)
 };
 
+// HW: kr, k rho, their inverses and R - |Z| from the hardware reciprocal
+// square root and reciprocal (the Jacobian, whose FP32 arithmetic is
+// accurate to ~1e-5 anyway); otherwise from correctly rounded sqrtf and
+// division (the FP32 hologram, whose phase k r carries the error of kr).
+template <bool HW = false>
 LMZ_HD void pixel_geometry(const FeatUniform &fu, float ax, float ay, float az,
                            bool bohren, PixGeom &g) {
     const float dx = (ax - fu.ix) - fu.fx;
@@ -184,15 +209,23 @@ LMZ_HD void pixel_geometry(const FeatUniform &fu, float ax, float ay, float az,
     g.Y = fu.k * dy;
     g.Z = fu.k * dz;
     const float rho2 = fmaf(g.X, g.X, g.Y * g.Y);
-    g.rho = sqrtf(rho2);
-    g.R = sqrtf(fmaf(g.Z, g.Z, rho2));
-    g.invR = 1.0f / g.R;
+    const float R2 = fmaf(g.Z, g.Z, rho2);
+    if constexpr (HW) {
+        g.invR = rsqrt_hw(R2);
+        g.R = R2 * g.invR;
+        g.irho = rho2 > 0.f ? rsqrt_hw(rho2) : 0.f;
+        g.rho = rho2 * g.irho;
+    } else {
+        g.R = sqrtf(R2);
+        g.invR = 1.0f / g.R;
+        g.rho = sqrtf(rho2);
+        g.irho = rho2 > 0.f ? 1.0f / g.rho : 0.f;
+    }
     g.c = g.Z * g.invR;
     g.s = g.rho * g.invR;
-    if (g.rho > 0.f) {
-        const float ir = 1.0f / g.rho;
-        g.u = g.X * ir;
-        g.v = g.Y * ir;
+    if (rho2 > 0.f) {
+        g.u = g.X * g.irho;
+        g.v = g.Y * g.irho;
     } else {  // arctan2(0, 0) = 0
         g.u = 1.f;
         g.v = 0.f;
@@ -201,7 +234,8 @@ LMZ_HD void pixel_geometry(const FeatUniform &fu, float ax, float ay, float az,
     const float sig = bohren ? sz : -sz;
     const float kzpix = fmaf(fu.k, az, fu.kz0);  // k * z of the pixel
     const float az_ = fabsf(g.Z);
-    const float t = rho2 / (g.R + az_);                // R - |Z|
+    const float t = HW ? rho2 * rcp_hw(g.R + az_)      // R - |Z|
+                       : rho2 / (g.R + az_);
     g.w = t * g.invR;
     g.cl = az_ * g.invR;
     g.mir = (g.Z < 0.f) ? -1.f : 1.f;
@@ -672,7 +706,7 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
                      credot(gp, L.Spc) + credot(er, Sr) + credot(et, St);
     // derivatives of the prefactors u^2, v^2, u v (K = (A + B) / rho, zero on
     // the axis, where dI/dX = dI/dY = 0) and u s
-    const float irho = (g.rho > LMZ_RHO_AXIS) ? 1.f / g.rho : 0.f;
+    const float irho = (g.rho > LMZ_RHO_AXIS) ? g.irho : 0.f;
     const float k0 = irho * credot(F0, ApB), k1 = irho * credot(F1, ApB);
     const float tz = invR * credot(F2, C);
     const float PX = (2.f * u * v2) * k0 + (v * (v2 - u2)) * k1 + (v2 + u2 * c * c) * tz;

@@ -75,7 +75,7 @@ void chk_pixels(const lmz_feature *feat, const double *params,
             if (holo) holo[j] = (float)intensity(E, fu.alphad);
         }
         PixGeom g;
-        pixel_geometry(fu, ax, ay, az, bohren, g);
+        pixel_geometry<true>(fu, ax, ay, az, bohren, g);
         if (jac) {
             JacPixel jp;
             jacobian_pixel(fu, g, coef, dcoef, nmax, jp);

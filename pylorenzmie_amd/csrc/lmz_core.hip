@@ -234,6 +234,10 @@ __device__ __forceinline__ int next_tile(int32_t *count, int (&slot)[2],
         while (c.k < LMZ_XCDS) {
             const int x = (home + c.k) % LMZ_XCDS;
             const int len = q + (x < r ? 1 : 0);
+            if (len == 0) {   // an empty chunk: nothing to claim
+                ++c.k;
+                continue;
+            }
             const int i = atomicAdd(count + 2 + x, LMZ_CLAIM);
             if (i < len) {
                 const int base = x * q + (x < r ? x : r);
@@ -434,6 +438,130 @@ __global__ __launch_bounds__(NT) void k_jacobian(
         block_reduce_store<LMZ_NRED, NT>(acc, partials + (long long)LMZ_NRED * t);
         __syncthreads();   // the reduction buffer is reused by the next tile
     }
+  }
+}
+
+// Radial tables of the Jacobian's sums (lmz_pixel.h, rtab_params): one
+// thread per sample, 64 samples per block, blockIdx.y the sample block;
+// thread 0 of the first block writes the feature's grid (m = 0 when the
+// feature is not tabulated) and sorts the active features into mask_tab
+// (k_jacobian_rtab) and mask_loop (k_jacobian).  The coefficient tables are
+// those k_jacobian reads (buffer sel[f] ^ flip).
+__global__ __launch_bounds__(64) void k_jac_rtab(
+    const lmz_feature *__restrict__ feat, const double *__restrict__ params,
+    const int32_t *__restrict__ sel, int flip, int nbuf,
+    const int32_t *__restrict__ nmax, const float *__restrict__ coef,
+    const float *__restrict__ dcoef, int flags,
+    const uint8_t *__restrict__ active, RTabPar *__restrict__ rpar,
+    float *__restrict__ rtab, uint8_t *__restrict__ mask_tab,
+    uint8_t *__restrict__ mask_loop) {
+    const int f = blockIdx.x;
+    if (active && !active[f]) {
+        if (blockIdx.y == 0 && threadIdx.x == 0) mask_tab[f] = mask_loop[f] = 0;
+        return;
+    }
+    const lmz_feature &F = feat[f];
+    const int b = sel ? (sel[f] ^ flip) : 0;
+    const int N = nmax[(long long)f * nbuf + b];
+    const FeatUniform fu = feature_uniform(
+        F, params + (long long)LMZ_NPARAM * f, N,
+        F.coef_offset + b * F.nmax_cap);
+    const RTabPar rp = rtab_params(fu);
+    if (blockIdx.y == 0 && threadIdx.x == 0) {
+        rpar[f] = rp;
+        mask_tab[f] = rp.m > 0 ? 1 : 0;
+        mask_loop[f] = rp.m > 0 ? 0 : 1;
+    }
+    const int j = blockIdx.y * 64 + threadIdx.x;
+    if (j >= rp.m) return;
+    rtab_sample(fu, rp, flags & LMZ_BOHREN, j,
+                coef + (long long)LMZ_COEF_STRIDE * fu.slot0,
+                dcoef + (long long)LMZ_DCOEF_STRIDE * fu.slot0,
+                rtab + ((long long)f * LMZ_RTAB_CAP + j) * LMZ_RTAB_STRIDE);
+}
+
+// Normal-equation partials of the tabulated features' tiles (the LM's
+// Jacobian pass on dense crops): each pixel interpolates the radial table
+// of its feature (k_jac_rtab), kept in LDS while it fits (LMZ_RTAB_LDS
+// rows, restaged when the block's feature changes), and projects as
+// k_jacobian.  Persistent grid over a device-built tile list; residuals
+// from rin (the FP64 forward pass) or data.  The partials stay in registers:
+// without the order loop the kernel has them to spare.
+template <int NT = LMZ_THREADS>
+__global__ __launch_bounds__(NT) void k_jacobian_rtab(
+    const lmz_feature *__restrict__ feat,
+    const int32_t *__restrict__ tile_feature,
+    const double *__restrict__ params, const int32_t *__restrict__ sel,
+    int flip, int nbuf, const int32_t *__restrict__ nmax,
+    const float *__restrict__ coords, long long npix_total, int flags,
+    const float *__restrict__ data, const float *__restrict__ rin,
+    long long rin_stride, double *__restrict__ partials,
+    const RTabPar *__restrict__ rpar, const float *__restrict__ rtab,
+    const int32_t *__restrict__ list, int32_t *count) {
+  __shared__ int next_w[2];
+  __shared__ float4 stab[LMZ_RTAB_LDS * LMZ_RTAB_LDS_STRIDE / 4];
+  TileCursor cursor;
+  int staged = -1;   // feature whose table stab holds
+  const int nwork = count[0];
+  for (int w = next_tile(count, next_w, cursor); w < nwork;
+       w = next_tile(count, next_w, cursor)) {
+    const int t = list[w];
+    const int f = tile_feature[t];
+    const lmz_feature &F = feat[f];
+    const int b = sel ? (sel[f] ^ flip) : 0;
+    const int N = nmax[(long long)f * nbuf + b];
+    const FeatUniform fu = feature_uniform(
+        F, params + (long long)LMZ_NPARAM * f, N,
+        F.coef_offset + b * F.nmax_cap);
+    const bool bohren = flags & LMZ_BOHREN;
+    const RTabPar rp = rpar[f];
+    const float *gtab = rtab + (long long)f * (LMZ_RTAB_CAP * LMZ_RTAB_STRIDE);
+    const bool lds = rp.m <= LMZ_RTAB_LDS;
+    if (lds && staged != f) {
+        // every thread has finished the previous tile (the barriers of the
+        // reduction and of next_tile)
+        const float4 *src = reinterpret_cast<const float4 *>(gtab);
+        constexpr int G = LMZ_RTAB_STRIDE / 4, S = LMZ_RTAB_LDS_STRIDE / 4;
+        for (int i = threadIdx.x; i < rp.m * G; i += NT)
+            stab[(i / G) * S + i % G] = src[i];
+        __syncthreads();
+        staged = f;
+    }
+    const float *rinf = rin ? rin + b * rin_stride : nullptr;
+    float acc[LMZ_NRED];
+#pragma unroll
+    for (int i = 0; i < LMZ_NRED; ++i) acc[i] = 0.f;
+#pragma unroll 1
+    for (int q = 0; q < LMZ_TILE / NT; ++q) {
+        const int j = (t - fu.tile0) * LMZ_TILE + q * NT + threadIdx.x;
+        if (j >= fu.npix) break;
+        float ax, ay, az;
+        pixel_coords(fu, j, coords, npix_total, ax, ay, az);
+        PixGeom g;
+        pixel_geometry<true>(fu, ax, ay, az, bohren, g);
+        JacPixel jp;
+        if (lds)
+            jacobian_pixel_rtab<LMZ_RTAB_LDS_STRIDE>(
+                fu, g, rp, reinterpret_cast<const float *>(stab), jp);
+        else
+            jacobian_pixel_rtab<LMZ_RTAB_STRIDE>(fu, g, rp, gtab, jp);
+        const long long gi = fu.pix_offset + j;
+        const float r = rinf ? rinf[gi] : (jp.I - data[gi]) * fu.inv_noise;
+        float Jn[LMZ_NJAC];
+#pragma unroll
+        for (int c = 0; c < LMZ_NJAC; ++c) Jn[c] = jp.J[c] * fu.inv_noise;
+        int idx = 0;
+#pragma unroll
+        for (int a = 0; a < LMZ_NJAC; ++a)
+#pragma unroll
+            for (int c = a; c < LMZ_NJAC; ++c, ++idx)
+                acc[idx] = fmaf(Jn[a], Jn[c], acc[idx]);
+#pragma unroll
+        for (int a = 0; a < LMZ_NJAC; ++a) acc[28 + a] = fmaf(Jn[a], r, acc[28 + a]);
+        acc[35] = fmaf(r, r, acc[35]);
+    }
+    block_reduce_store<LMZ_NRED, NT>(acc, partials + (long long)LMZ_NRED * t);
+    __syncthreads();   // the reduction buffer is reused by the next tile
   }
 }
 
@@ -848,6 +976,33 @@ void launch_jacobian(const lmz_batch *b, const double *params,
                        tl ? tl->count : nullptr);
 }
 
+void launch_jac_rtab(const lmz_batch *b, const double *params,
+                     const int32_t *sel, int flip, const int32_t *nmax,
+                     const float *coef, const float *dcoef, int flags,
+                     const uint8_t *active, RTabPar *rpar, float *rtab,
+                     uint8_t *mask_tab, uint8_t *mask_loop,
+                     hipStream_t stream) {
+    hipLaunchKernelGGL(k_jac_rtab, dim3(b->nfeat, LMZ_RTAB_CAP / 64), dim3(64),
+                       0, stream, b->d_feat, params, sel, flip,
+                       b->coef_buffers, nmax, coef, dcoef, flags, active, rpar,
+                       rtab, mask_tab, mask_loop);
+}
+
+void launch_jacobian_rtab(const lmz_batch *b, const double *params,
+                          const int32_t *sel, int flip, const int32_t *nmax,
+                          const float *coords, int flags, const float *data,
+                          const float *rin, long long rin_stride,
+                          double *partials, const RTabPar *rpar,
+                          const float *rtab, const TileList *tl,
+                          hipStream_t stream) {
+    hipLaunchKernelGGL((k_jacobian_rtab<LMZ_THREADS>), dim3(tl->grid),
+                       dim3(LMZ_THREADS), 0, stream, b->d_feat,
+                       b->d_tile_feature, params, sel, flip, b->coef_buffers,
+                       nmax, coords, b->npix_total, flags, data, rin,
+                       rin_stride, partials, rpar, rtab,
+                       (const int32_t *)tl->list, tl->count);
+}
+
 void launch_fd_jacobian(const lmz_batch *b, const double *params,
                         const int32_t *sel, int flip, const int32_t *nmax,
                         const float *coef, const FdTables &fd, int vary,
@@ -979,6 +1134,19 @@ int persistent_grids(const lmz_batch *b, int *grid_jacobian, int *grid_forward) 
                                             : occupancy(false, sc.fwd_threads);
     *grid_jacobian = cus * bj;
     *grid_forward = cus * bf;
+    return LMZ_OK;
+}
+
+// Persistent grid of k_jacobian_rtab: as many workgroups as are resident.
+int rtab_grid(int *grid) {
+    int cus = 0;
+    if (int rc = device_cus(&cus)) return rc;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, reinterpret_cast<const void *>(&k_jacobian_rtab<LMZ_THREADS>),
+            LMZ_THREADS, 0) != hipSuccess || n < 1)
+        n = 1;
+    *grid = cus * n;
     return LMZ_OK;
 }
 

@@ -64,6 +64,22 @@ void launch_jacobian(const lmz_batch *b, const double *params,
                      const float *rin, long long rin_stride, float *jac,
                      float *resid, double *partials, const uint8_t *active,
                      const TileList *tl, hipStream_t stream);
+// Radial tables of the Jacobian's sums for the active features (rpar
+// [nfeat], rtab [nfeat][LMZ_RTAB_CAP][LMZ_RTAB_STRIDE]); mask_tab marks the
+// tabulated ones, mask_loop the rest (both [nfeat], 0 for inactive ones).
+void launch_jac_rtab(const lmz_batch *b, const double *params,
+                     const int32_t *sel, int flip, const int32_t *nmax,
+                     const float *coef, const float *dcoef, int flags,
+                     const uint8_t *active, RTabPar *rpar, float *rtab,
+                     uint8_t *mask_tab, uint8_t *mask_loop, hipStream_t stream);
+// Normal-equation partials of the tabulated features' tiles in tl.
+void launch_jacobian_rtab(const lmz_batch *b, const double *params,
+                          const int32_t *sel, int flip, const int32_t *nmax,
+                          const float *coords, int flags, const float *data,
+                          const float *rin, long long rin_stride,
+                          double *partials, const RTabPar *rpar,
+                          const float *rtab, const TileList *tl,
+                          hipStream_t stream);
 // Buffers of the forward-difference Jacobian: steps, the a_p / n_p / k_p
 // parameter rows and their coefficient tables.
 struct FdTables {
@@ -90,6 +106,7 @@ void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
                       const TileList &tl, hipStream_t stream);
 // Persistent grids of the LM's pixel launches under the batch's schedule.
 int persistent_grids(const lmz_batch *b, int *grid_jacobian, int *grid_forward);
+int rtab_grid(int *grid);
 int hip_fail(hipError_t e, const char *what);
 int check_batch(const lmz_batch *b);
 

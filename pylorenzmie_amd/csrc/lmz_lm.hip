@@ -26,7 +26,7 @@ struct LMLayout {
     size_t coef, dcoef, nmax, sel, work, partials, chi2_tiles, state, ptrial,
         mask_j, mask_f, count, resid, list_j, list_f, tile_start, fd_h,
         fd_params, fd_nmax,
-        fd_coef, total;
+        fd_coef, rpar, rtab, mask_jt, mask_jl, list_jl, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -48,7 +48,7 @@ static LMLayout lm_layout(const lmz_batch *b) {
     L.ptrial = take(sizeof(double) * LMZ_NPARAM * (size_t)b->nfeat);
     L.mask_j = take((size_t)b->nfeat);
     L.mask_f = take((size_t)b->nfeat);
-    L.count = take(sizeof(int32_t) * 48);   // active count, two TileLists
+    L.count = take(sizeof(int32_t) * 64);   // active count, three TileLists
     L.resid = take(sizeof(float) * 2 * (size_t)b->npix_total);
     L.list_j = take(sizeof(int32_t) * (size_t)b->ntiles);
     L.list_f = take(sizeof(int32_t) * (size_t)b->ntiles);
@@ -58,6 +58,12 @@ static LMLayout lm_layout(const lmz_batch *b) {
     L.fd_params = take(sizeof(double) * 3 * LMZ_NPARAM * (size_t)b->nfeat);
     L.fd_nmax = take(sizeof(int32_t) * 3 * 2 * (size_t)b->nfeat);
     L.fd_coef = take(sizeof(float) * 3 * LMZ_COEF_STRIDE * (size_t)b->coef_slots);
+    // radial tables of the Jacobian's sums (dense crops)
+    L.rpar = take(sizeof(RTabPar) * (size_t)b->nfeat);
+    L.rtab = take(sizeof(float) * LMZ_RTAB_CAP * LMZ_RTAB_STRIDE * (size_t)b->nfeat);
+    L.mask_jt = take((size_t)b->nfeat);    // tabulated / order-loop features
+    L.mask_jl = take((size_t)b->nfeat);
+    L.list_jl = take(sizeof(int32_t) * (size_t)b->ntiles);
     L.total = off;
     return L;
 }
@@ -212,6 +218,7 @@ using namespace lmz;
 struct lmz_lm_plan {
     const lmz_batch *batch = nullptr;
     int grid_j = 0, grid_f = 0;       // persistent grids of the pixel kernels
+    int grid_t = 0;                   // and of k_jacobian_rtab
     hipStream_t s = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     int32_t *h_count = nullptr;       // pinned
@@ -296,8 +303,8 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
                         plan->grid_f, tile_start};
     // J^T J and J^T r at the current point, r from its FP64 forward pass,
     // over the tiles of the features that accepted their last step
-    launch_tile_list(b, mask_j, tl_j, s);
     if (set.fd_step > 0.0) {
+        launch_tile_list(b, mask_j, tl_j, s);
         FdTables fd;
         fd.eps = set.fd_step;
         fd.h = reinterpret_cast<double *>(ws + L.fd_h);
@@ -312,9 +319,24 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
         launch_fd_jacobian(b, params, sel, 0, nmax, coef, fd, vary, coords,
                            flags, resid, np, partials, mask_j, &tl_j, work, s);
     } else {
+        // dense crops from radial tables of the sums (k_jacobian_rtab), the
+        // rest by the order loop at every pixel (k_jacobian)
+        RTabPar *rpar = reinterpret_cast<RTabPar *>(ws + L.rpar);
+        float *rtab = reinterpret_cast<float *>(ws + L.rtab);
+        uint8_t *mask_jt = reinterpret_cast<uint8_t *>(ws + L.mask_jt);
+        uint8_t *mask_jl = reinterpret_cast<uint8_t *>(ws + L.mask_jl);
+        const TileList tl_t{tl_j.list, tl_j.count, plan->grid_t, tile_start};
+        const TileList tl_l{reinterpret_cast<int32_t *>(ws + L.list_jl),
+                            count + 48, plan->grid_j, tile_start};
+        launch_jac_rtab(b, params, sel, 0, nmax, coef, dcoef, flags, mask_j,
+                        rpar, rtab, mask_jt, mask_jl, s);
+        launch_tile_list(b, mask_jt, tl_t, s);
+        launch_jacobian_rtab(b, params, sel, 0, nmax, coords, flags, data,
+                             resid, np, partials, rpar, rtab, &tl_t, s);
+        launch_tile_list(b, mask_jl, tl_l, s);
         launch_jacobian(b, params, sel, 0, nmax, coef, dcoef, coords, flags,
                         data, resid, np, nullptr, nullptr, partials, nullptr,
-                        &tl_j, s);
+                        &tl_l, s);
     }
     hipLaunchKernelGGL(k_lm_solve, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
                        partials, st, params, ptrial, mask_j, mask_f, set);
@@ -417,6 +439,7 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
                 "workspace must be 256-byte aligned");
 
     if (int rc = persistent_grids(b, &plan->grid_j, &plan->grid_f)) return rc;
+    if (int rc = rtab_grid(&plan->grid_t)) return rc;
     const LMLayout L = lm_layout(b);
     char *ws = static_cast<char *>(workspace);
     LMState *st = reinterpret_cast<LMState *>(ws + L.state);

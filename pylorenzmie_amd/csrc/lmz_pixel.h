@@ -637,10 +637,11 @@ LMZ_HD void jac_order(JacLoop &L, int n, const float *coef, const float *dcoef,
     L.x1 = xi;
 }
 
-LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
-                           const float *coef, const float *dcoef, int nmax,
-                           JacPixel &out) {
-    JacLoop L;
+// The order loop of one pixel (or one radial table sample): the sums of
+// JacLoop from the starting values (xi_m1, xi_0).
+LMZ_HD void jacobian_loop(float w, float cl, float invRl, float invR,
+                          cf xi_m1, cf xi_0, const float *coef,
+                          const float *dcoef, int nmax, JacLoop &L) {
     L.Sr = L.St = L.Sp = L.mSrR = L.mStR = L.mSpR = cf{0.f, 0.f};
     L.Src = L.Stc = L.Spc = cf{0.f, 0.f};
 #pragma unroll
@@ -648,16 +649,22 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
     L.ang.init();
     L.dpm = 0.f;
     L.dpn = 0.f;
-    L.x2 = g.xi_m1;    // sigma_{-1} = sigma_0 = 1
-    L.x1 = g.xi_0;
-    const float w = g.w;
-    const float cl = g.cl, invRl = g.invRl, invR2 = g.invR * g.invR;
+    L.x2 = xi_m1;      // sigma_{-1} = sigma_0 = 1
+    L.x1 = xi_0;
+    const float invR2 = invR * invR;
     int n = 1;
     for (; n + 1 <= nmax; n += 2) {
         jac_order<true>(L, n, coef, dcoef, w, cl, invRl, invR2);
         jac_order<false>(L, n + 1, coef, dcoef, w, cl, invRl, invR2);
     }
     if (n <= nmax) jac_order<true>(L, n, coef, dcoef, w, cl, invRl, invR2);
+}
+
+// The intensity and its seven derivatives of one pixel from the sums of
+// the order loop (L's sums; its loop state is not read).
+LMZ_HD void jacobian_project(const FeatUniform &fu, const PixGeom &g,
+                             const JacLoop &L, JacPixel &out) {
+    const float invR2 = g.invR * g.invR;
     // ---- projection by adjoint weights.  The field is E = M S, S = (S_r,
     // S_t, S_p), with the real 3 x 3 geometry matrix of field_from_sums:
     //   E_x = u^2 A - v^2 B,  E_y = u v (A + B),  E_z = u s C,
@@ -733,6 +740,152 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
         out.J[5] = two_al * (credot(gr, iTr) + credot(gt, iTt) + credot(gpm, iTp));
     }
     out.J[6] = 2.f * (rr + rt + rp);
+}
+
+LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
+                           const float *coef, const float *dcoef, int nmax,
+                           JacPixel &out) {
+    JacLoop L;
+    jacobian_loop(g.w, g.cl, g.invRl, g.invR, g.xi_m1, g.xi_0, coef, dcoef,
+                  nmax, L);
+    jacobian_project(fu, g, L, out);
+}
+
+// ====================================================================
+// radial tables of the Jacobian's sums (dense crops)
+// ====================================================================
+// On a dense crop every pixel lies in the plane z = 0, so the sums of the
+// order loop depend on the pixel only through its distance rho from the
+// particle's axis (R = sqrt(rho^2 + Z^2), cos theta = Z / R), and, since the
+// loop is linear in its starting values (xi_-1, xi_0) = e^{i psi} (1,
+// -i sig m), they are e^{i psi(rho)} times the sums started from (1, -i sig
+// m), which vary slowly in rho: no faster than N / |Z| radians per unit of
+// k rho (the angular functions of order N).  So the loop runs once per
+// sample of a uniform grid in k rho, with spacing h = LMZ_RTAB_HW |Z| / N
+// (at most 1), and each pixel takes its sums by LMZ_RTAB_P-point Lagrange
+// interpolation times e^{i psi}; within 1e-6 of the column scale of each
+// Jacobian column on the feature (measured against the loop at every pixel).
+// The sums are even in rho, so samples at rho < 0 take |rho|.
+#define LMZ_RTAB_CAP 1024    // samples per feature
+#define LMZ_RTAB_STRIDE 32   // floats per sample: 15 complex sums, 2 spare
+#define LMZ_RTAB_P 6         // interpolation points
+#define LMZ_RTAB_HW 0.35f    // h N / |Z|
+#define LMZ_RTAB_LDS 256      // rows of a table the Jacobian kernel keeps in LDS
+#define LMZ_RTAB_LDS_STRIDE 36   // floats per LDS row: rows 4 banks apart
+
+struct RTabPar {
+    float rho0, h, inv_h;    // grid k rho_j = rho0 + j h
+    int m;                   // samples; 0: the feature uses the order loop
+};
+
+// The table grid of a feature at its current parameters: every pixel of the
+// crop (height rows of width) lies between the nearest and farthest points
+// of the crop from the particle's axis, with LMZ_RTAB_P / 2 samples to
+// spare either side.  m = 0 for explicit coordinates (not known to be
+// planar), for Z = 0, or when the grid would need more than LMZ_RTAB_CAP
+// samples.
+LMZ_HD RTabPar rtab_params(const FeatUniform &fu) {
+    RTabPar p{0.f, 0.f, 0.f, 0};
+    if (fu.width <= 0 || fu.nmax < 1) return p;
+    const float Z = fu.k * (fu.iz + fu.fz);     // pixel_geometry at z = 0
+    if (!(fabsf(Z) > 0.f)) return p;
+    const int height = (fu.npix + fu.width - 1) / fu.width;
+    const float px = fu.ix + fu.fx, py = fu.iy + fu.fy;   // axis in the crop
+    const float xm = (float)(fu.width - 1), ym = (float)(height - 1);
+    const float nx = fmaxf(fmaxf(-px, px - xm), 0.f);
+    const float ny = fmaxf(fmaxf(-py, py - ym), 0.f);
+    const float fx = fmaxf(px, xm - px), fy = fmaxf(py, ym - py);
+    const float rmin = fu.k * sqrtf(fmaf(nx, nx, ny * ny));
+    const float rmax = fu.k * sqrtf(fmaf(fx, fx, fy * fy));
+    const float h = fminf(LMZ_RTAB_HW * fabsf(Z) / (float)fu.nmax, 1.f);
+    const float span = (rmax - rmin) / h + (float)(2 * LMZ_RTAB_P);
+    if (!(span < (float)LMZ_RTAB_CAP)) return p;
+    p.h = h;
+    p.inv_h = 1.f / h;
+    p.rho0 = rmin - (float)(LMZ_RTAB_P / 2 + 1) * h;
+    p.m = (int)span;
+    return p;
+}
+
+// Sample j of the table: the order loop at k rho = |rho0 + j h|, z = 0,
+// started from (1, -i sig m); 30 floats in JacLoop's sum order.
+LMZ_HD void rtab_sample(const FeatUniform &fu, const RTabPar &p, bool bohren,
+                        int j, const float *coef, const float *dcoef,
+                        float *out) {
+    const float Z = fu.k * (fu.iz + fu.fz);
+    const float rho = fabsf(fmaf((float)j, p.h, p.rho0));
+    const float rho2 = rho * rho;
+    const float R2 = fmaf(Z, Z, rho2);
+    const float invR = rsqrt_hw(R2);
+    const float R = R2 * invR;
+    const float az_ = fabsf(Z);
+    const float w = rho2 * rcp_hw(R + az_) * invR;   // 1 - |cos theta|
+    const float cl = az_ * invR;
+    const float mir = (Z < 0.f) ? -1.f : 1.f;
+    const float sz = (Z > 0.f) ? 1.f : -1.f;
+    const float sig = bohren ? sz : -sz;
+    JacLoop L;
+    jacobian_loop(w, cl, mir * invR, invR, cf{1.f, 0.f}, cf{0.f, -sig * mir},
+                  coef, dcoef, fu.nmax, L);
+    const cf v[15] = {L.Sr, L.St, L.Sp, L.mSrR, L.mStR, L.mSpR, L.Src, L.Stc,
+                      L.Spc, L.Tr[0], L.Tr[1], L.Tt[0], L.Tt[1], L.Tp[0],
+                      L.Tp[1]};
+#pragma unroll
+    for (int i = 0; i < 15; ++i) {
+        out[2 * i] = v[i].re;
+        out[2 * i + 1] = v[i].im;
+    }
+    out[30] = 0.f;
+    out[31] = 0.f;
+}
+
+// The Jacobian of one pixel of a tabulated feature: its sums interpolated
+// in k rho and turned by e^{i psi} (g.xi_m1), then projected as the loop's.
+// RS: floats from one table row to the next (LMZ_RTAB_STRIDE in global
+// memory, LMZ_RTAB_LDS_STRIDE in LDS).
+template <int RS = LMZ_RTAB_STRIDE>
+LMZ_HD void jacobian_pixel_rtab(const FeatUniform &fu, const PixGeom &g,
+                                const RTabPar &p, const float *tab,
+                                JacPixel &out) {
+    const float t = (g.rho - p.rho0) * p.inv_h;
+    int i0 = (int)floorf(t) - (LMZ_RTAB_P / 2 - 1);
+    i0 = i0 < 0 ? 0 : (i0 > p.m - LMZ_RTAB_P ? p.m - LMZ_RTAB_P : i0);
+    const float d = t - (float)i0;
+    // Lagrange weights on nodes 0..5 at d: prefix and suffix products of
+    // (d - k) over the constant denominators
+    float e[LMZ_RTAB_P], pre[LMZ_RTAB_P], suf[LMZ_RTAB_P];
+#pragma unroll
+    for (int k = 0; k < LMZ_RTAB_P; ++k) e[k] = d - (float)k;
+    pre[0] = 1.f;
+#pragma unroll
+    for (int k = 1; k < LMZ_RTAB_P; ++k) pre[k] = pre[k - 1] * e[k - 1];
+    suf[LMZ_RTAB_P - 1] = 1.f;
+#pragma unroll
+    for (int k = LMZ_RTAB_P - 2; k >= 0; --k) suf[k] = suf[k + 1] * e[k + 1];
+    const float den[LMZ_RTAB_P] = {-1.f / 120.f, 1.f / 24.f, -1.f / 12.f,
+                                   1.f / 12.f, -1.f / 24.f, 1.f / 120.f};
+    float wgt[LMZ_RTAB_P];
+#pragma unroll
+    for (int k = 0; k < LMZ_RTAB_P; ++k) wgt[k] = pre[k] * suf[k] * den[k];
+    float acc[30];
+    const float *row = tab + i0 * RS;
+#pragma unroll
+    for (int c = 0; c < 30; ++c) acc[c] = wgt[0] * row[c];
+#pragma unroll
+    for (int k = 1; k < LMZ_RTAB_P; ++k) {
+        const float *rk = row + k * RS;
+#pragma unroll
+        for (int c = 0; c < 30; ++c) acc[c] = fmaf(wgt[k], rk[c], acc[c]);
+    }
+    const cf ph = g.xi_m1;   // e^{i psi}
+    JacLoop L;
+    cf *dst[15] = {&L.Sr, &L.St, &L.Sp, &L.mSrR, &L.mStR, &L.mSpR, &L.Src,
+                   &L.Stc, &L.Spc, &L.Tr[0], &L.Tr[1], &L.Tt[0], &L.Tt[1],
+                   &L.Tp[0], &L.Tp[1]};
+#pragma unroll
+    for (int i = 0; i < 15; ++i)
+        *dst[i] = cmul(ph, cf{acc[2 * i], acc[2 * i + 1]});
+    jacobian_project(fu, g, L, out);
 }
 
 // ====================================================================

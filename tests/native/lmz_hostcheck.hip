@@ -84,6 +84,36 @@ void chk_pixels(const lmz_feature *feat, const double *params,
     }
 }
 
+// Jacobian of every pixel of a dense crop (feat->width > 0) by the order
+// loop (jac_loop) and by the radial table (jac_tab, when the feature is
+// tabulated); returns the table's sample count (0: not tabulated) and its
+// spacing in k rho.
+int chk_jac_rtab(const lmz_feature *feat, const double *params,
+                 const float *coef, const float *dcoef, int nmax, int flags,
+                 float *jac_loop, float *jac_tab, float *h_out) {
+    const FeatUniform fu = feature_uniform(*feat, params, nmax, 0);
+    const bool bohren = flags & LMZ_BOHREN;
+    const RTabPar p = rtab_params(fu);
+    std::vector<float> tab((size_t)LMZ_RTAB_STRIDE * (p.m > 0 ? p.m : 1));
+    for (int j = 0; j < p.m; ++j)
+        rtab_sample(fu, p, bohren, j, coef, dcoef, &tab[(size_t)j * LMZ_RTAB_STRIDE]);
+    for (int j = 0; j < fu.npix; ++j) {
+        float ax, ay, az;
+        pixel_coords(fu, j, nullptr, fu.npix, ax, ay, az);
+        PixGeom g;
+        pixel_geometry<true>(fu, ax, ay, az, bohren, g);
+        JacPixel a, b;
+        jacobian_pixel(fu, g, coef, dcoef, nmax, a);
+        if (p.m > 0) jacobian_pixel_rtab(fu, g, p, tab.data(), b);
+        for (int c = 0; c < LMZ_NJAC; ++c) {
+            jac_loop[c * fu.npix + j] = a.J[c];
+            jac_tab[c * fu.npix + j] = p.m > 0 ? b.J[c] : 0.f;
+        }
+    }
+    *h_out = p.h;
+    return p.m;
+}
+
 // FP32 hologram of one feature (the residual kernel's fast mode).
 void chk_holo32(const lmz_feature *feat, const double *params,
                 const float *coef, int nmax, const float *coords, int npix,

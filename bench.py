@@ -5,9 +5,10 @@ as the config-3 distribution (a_p U[0.4, 3.0] um, n_p U[1.38, 1.65], k_p
 U[0, 0.01], z_p U[50, 400] px, x/y at the crop centre +-5 px, alpha
 U[0.8, 1.2]), Gaussian noise sigma = 0.05, starts perturbed by +-2 px,
 x(1 +- 0.05) a_p, +-0.02 n_p.  One step fits all 4096 with the batched
-device-resident LM (forward-mode Jacobian kernel, FP64 solve, HIP graph per
-iteration, <= 50 iterations), from the starting parameters already in HBM to
-the results copied back to the host.
+device-resident LM (forward-mode Jacobian from radial tables of the multipole
+sums, FP64 trial forward pass and solve, HIP graph per iteration, <= 50
+iterations), from the starting parameters already in HBM to the results
+copied back to the host.
 
 Also measured:
   fd      the plain GPU finite-difference form (hipOptimizer: one feature at
@@ -17,15 +18,19 @@ Also measured:
           FP64 hologram (lmz_lm_settings.fd_step = 1e-5, the reference's
           diff_step) in place of the analytic one -> the adjoint's share of
           the speedup;
-  roofline  the Jacobian kernel (dominant) and the trial-forward kernel,
-          timed with HIP events on the stream they run on, against the FP32
-          VALU peak, with algorithmic FLOP counts per pixel of 296 N + 480
-          (Jacobian) and 64 N + 70 (forward), N the order count; and
-          against the attainable rates of this build, measured in the
-          same run by the FMA-chain probe (tests/native/lmz_valu_peak.hip):
-          the kernel's counted FP32 FLOP and VALU instructions per launch
-          (from the PMC summary committed for these sources) over the
-          probe's two-operand FP32 FMA rate;
+  roofline  the trial-forward kernel k_forward<2> (dominant: FP64 sums
+          over every pixel and order), timed with HIP events on the stream
+          it runs on, against the FP64 vector peak, with an algorithmic
+          FLOP count per pixel of 64 N + 70, N the order count; and against
+          the attainable FP64 FMA rate of this build, measured in the same
+          run by the FMA-chain probe (tests/native/lmz_valu_peak.hip): the
+          kernel's counted FP64 FLOP and VALU instructions per launch (from
+          the PMC summary committed for these sources) over the probe's;
+  jacobian  the LM's Jacobian pass (lmz_jacobian_tabulated: radial tables
+          of the sums and their interpolation, the J^T J partials) and, for
+          comparison, the order loop at every pixel (lmz_jacobian, 296 N +
+          480 FLOP per pixel against the FP32 peak and the probe's
+          two-operand FP32 FMA rate);
   cpu_baseline  the FP64 oracle (numpy model + scipy MINPACK LM) on the
           host's cores (one process per core, two features each, at most
           16 cores), started as a child process that never touches the
@@ -70,6 +75,7 @@ sys.path.insert(0, ROOT)
 INSTRUMENT = dict(wavelength=0.447, magnification=0.048, n_m=1.340)
 PEAK_FP32 = 157.3        # TFLOP/s, MI355X FP32 vector (packed)
 PEAK_FP32_UNPACKED = 78.6
+PEAK_FP64 = 78.6         # TFLOP/s, MI355X FP64 vector
 PEAK_HBM = 8000.0        # GB/s
 PARAMS = ('x_p', 'y_p', 'z_p', 'a_p', 'n_p', 'k_p', 'alpha')
 
@@ -255,9 +261,24 @@ def kernel_times(lm, settings, args):
         _lmz.check(lib.lmz_hologram(b.handle, _lmz.ptr(lm.params), None, _lmz.ptr(nmax),
                                     _lmz.ptr(coef), None, flags, _lmz.ptr(lm.data),
                                     None, _lmz.ptr(chi2), None, s))
+    # the LM's Jacobian pass on dense crops: radial tables of the sums,
+    # interpolated per pixel (k_jac_rtab, k_jacobian_rtab and their tile
+    # lists), the same partials
+    import ctypes
+    nb = ctypes.c_size_t()
+    _lmz.check(lib.lmz_jacobian_tabulated_bytes(b.handle, ctypes.byref(nb)))
+    tabwork = torch.empty(nb.value // 8 + 64, dtype=torch.float64, device=dev)
+
+    def jtab():
+        _lmz.check(lib.lmz_jacobian_tabulated(
+            b.handle, _lmz.ptr(lm.params), _lmz.ptr(nmax), _lmz.ptr(coef),
+            _lmz.ptr(dcoef), flags, _lmz.ptr(lm.data), _lmz.ptr(partials),
+            _lmz.ptr(tabwork), s))
     jac()
     fwd()
+    jtab()
     reps = args.kernel_reps
+    et = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
     ev[0].record()
     for _ in range(reps):
         jac()
@@ -265,9 +286,15 @@ def kernel_times(lm, settings, args):
     for _ in range(reps):
         fwd()
     ev[2].record()
+    et[0].record()
+    for _ in range(reps):
+        jtab()
+    et[1].record()
     torch.cuda.synchronize()
     tj = ev[0].elapsed_time(ev[1]) / reps * 1e-3
     tf = ev[1].elapsed_time(ev[2]) / reps * 1e-3
+    tt = et[0].elapsed_time(et[1]) / reps * 1e-3
+    del tabwork
     nm = nmax.cpu().numpy()[0::2].astype(float)
     P = np.array(lm.npix, float)
     flop_j = float(np.sum(P * (296. * nm + 480.)))
@@ -294,7 +321,7 @@ def kernel_times(lm, settings, args):
     torch.cuda.synchronize()
     tn = ev[2].elapsed_time(ev[3]) / max(reps, 5) * 1e-3
     del J, r
-    return tj, tf, tn, flop_j, flop_f, float(np.sum(P)), float(np.mean(nm))
+    return tj, tf, tn, tt, flop_j, flop_f, float(np.sum(P)), float(np.mean(nm))
 
 
 HEADLINE_FIXTURE = os.path.join(ROOT, 'tests', 'golden', 'headline.npz')
@@ -689,16 +716,19 @@ def run_config(args):
         st = res[:, 38].astype(int)
         ok = ((st & 7) != 0) & ((st & 56) == 0)
         lm.params.copy_(torch.from_numpy(np.hstack([start, np.zeros((n, 1))])).cuda())
-        tj, tf, tn, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
+        tj, tf, tn, tt, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
         bytes_n = 32. * npx + 36 * 8. * n
         line.update(workload=f'batched 7-parameter LM, {n} crops of side '
                              'log-U[48, 512], sigma in {0.02, 0.05, 0.1}',
                     pixels=float(npx), fits_per_s=n / dt, seconds=dt,
                     converged_frac=float(ok.mean()),
                     iterations_mean=float(res[:, 37].mean()),
-                    jacobian_ms=tj * 1e3, jacobian_tflops=flop_j / tj / 1e12,
-                    jacobian_frac_fp32_peak=flop_j / tj / 1e12 / PEAK_FP32,
+                    jacobian_tabulated_ms=tt * 1e3,
+                    jacobian_loop_ms=tj * 1e3,
+                    jacobian_loop_tflops=flop_j / tj / 1e12,
+                    jacobian_loop_frac_fp32_peak=flop_j / tj / 1e12 / PEAK_FP32,
                     forward_ms=tf * 1e3, forward_tflops=flop_f / tf / 1e12,
+                    forward_frac_fp64_peak=flop_f / tf / 1e12 / PEAK_FP64,
                     normal_eq_gbs=bytes_n / tn / 1e9,
                     normal_eq_frac_hbm_peak=bytes_n / tn / 1e9 / PEAK_HBM)
     elif cfg == 4:
@@ -1216,11 +1246,11 @@ def main():
     log(f"bench: {agree['frac_within_1e-4_rel_and_kp_1e-5_abs']:.4f} of "
         f"{agree['features_compared']} fits within 1e-4 of the converged minimum")
     log('bench: kernel timing')
-    tj, tf, tn, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
+    tj, tf, tn, tt, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
     ach_j = flop_j / tj / 1e12
     ach_f = flop_f / tf / 1e12
-    log(f'bench: jacobian {tj * 1e3:.2f} ms {ach_j:.1f} TFLOP/s; '
-        f'forward {tf * 1e3:.2f} ms {ach_f:.1f} TFLOP/s')
+    log(f'bench: forward {tf * 1e3:.2f} ms {ach_f:.1f} TFLOP/s; jacobian '
+        f'tabulated {tt * 1e3:.2f} ms, order loop {tj * 1e3:.2f} ms')
 
     # J reads the data (4 B / pixel) and writes 36 FP64 partials per tile
     bytes_j = 4. * npx + 36 * 8. * lm.batch.ntiles
@@ -1246,6 +1276,14 @@ def main():
                                args.nfeat, args.side)
     lane_inst = committed_figure('valu_counters', 'valu_lane_instructions_per_launch',
                                  args.nfeat, args.side)
+    traffic_f = committed_figure('k_jacobian_traffic', 'forward_hbm_bytes',
+                                 args.nfeat, args.side)
+    counted_f = committed_figure('valu_counters', 'forward_counted_fp64_flop',
+                                 args.nfeat, args.side)
+    lane_f = committed_figure('valu_counters', 'forward_lane_instructions',
+                              args.nfeat, args.side)
+    lane_t = committed_figure('valu_counters', 'tabulated_lane_instructions',
+                              args.nfeat, args.side)
     log('bench: VALU issue-rate probe')
     probe = valu_probe()['rates']
     vop2 = probe['fp32_fma_lane']
@@ -1256,6 +1294,10 @@ def main():
         'fp64_fma_tflops': probe['fp64_fma_lane']['tflops'],
         'fp32_vop2_lane_instructions_per_s': vop2['lane_instructions_per_s']}
     issue = (lane_inst / tj) if lane_inst else None
+    issue_f = (lane_f / tf) if lane_f else None
+    fp64_lane = probe['fp64_fma_lane']['lane_instructions_per_s']
+    attainable['fp64_lane_instructions_per_s'] = fp64_lane
+    bytes_f = 4. * npx + 8. * lm.batch.ntiles
     fd = None
     fdb = None
     if not args.no_fd:
@@ -1312,26 +1354,46 @@ def main():
                    'features': args.nfeat, 'side': args.side,
                    'mean_nmax': nmean, 'ftol': args.ftol, 'xtol': args.xtol,
                    'gtol': args.gtol, 'schedule': args.schedule},
-        'roofline': {'bound': 'valu', 'kernel': 'k_jacobian',
-                     'achieved': ach_j, 'peak': PEAK_FP32, 'unit': 'TFLOP/s',
-                     'frac': ach_j / PEAK_FP32,
-                     'frac_of_unpacked_peak': ach_j / PEAK_FP32_UNPACKED,
-                     'avg_launch_ms': tj * 1e3,
-                     'algorithmic_flop_per_launch': flop_j,
+        'roofline': {'bound': 'valu', 'kernel': 'k_forward<2>',
+                     'dtype': 'f64',
+                     'achieved': ach_f, 'peak': PEAK_FP64, 'unit': 'TFLOP/s',
+                     'frac': ach_f / PEAK_FP64,
+                     'avg_launch_ms': tf * 1e3,
+                     'algorithmic_flop_per_launch': flop_f,
                      'pixels_per_launch': npx,
-                     'traffic': traffic,
-                     'algorithmic_bytes_per_launch': bytes_j,
-                     'counted_fp32_flop_per_launch': counted,
-                     'counted_achieved': (counted / tj / 1e12) if counted else None,
+                     'traffic': traffic_f,
+                     'algorithmic_bytes_per_launch': bytes_f,
+                     'counted_fp64_flop_per_launch': counted_f,
+                     'counted_achieved': (counted_f / tf / 1e12) if counted_f else None,
                      'attainable': attainable,
-                     'frac_of_attainable_counted': ((counted / tj / 1e12) /
-                                                    attainable['fp32_fma_vop2_tflops']
-                                                    if counted else None),
-                     'valu_lane_instructions_per_launch': lane_inst,
-                     'frac_of_attainable_issue': ((issue / attainable['fp32_vop2_lane_instructions_per_s'])
-                                                  if issue else None),
-                     'forward_kernel': {'achieved': ach_f, 'avg_launch_ms': tf * 1e3,
-                                        'algorithmic_flop_per_launch': flop_f}},
+                     'frac_of_attainable_counted': ((counted_f / tf / 1e12) /
+                                                    attainable['fp64_fma_tflops']
+                                                    if counted_f else None),
+                     'valu_lane_instructions_per_launch': lane_f,
+                     'frac_of_attainable_issue': ((issue_f / fp64_lane)
+                                                  if issue_f else None)},
+        'jacobian': {'kernels': ['k_jac_rtab', 'k_jacobian_rtab'],
+                     'dtype': 'f32 (f64 tile sums)',
+                     'avg_launch_ms': tt * 1e3,
+                     'pixels_per_launch': npx,
+                     'valu_lane_instructions_per_launch': lane_t,
+                     'frac_of_attainable_issue': (
+                         (lane_t / tt / attainable['fp32_vop2_lane_instructions_per_s'])
+                         if lane_t else None),
+                     'order_loop': {
+                         'kernel': 'k_jacobian', 'bound': 'valu',
+                         'avg_launch_ms': tj * 1e3,
+                         'achieved': ach_j, 'peak': PEAK_FP32, 'unit': 'TFLOP/s',
+                         'frac': ach_j / PEAK_FP32,
+                         'algorithmic_flop_per_launch': flop_j,
+                         'traffic': traffic,
+                         'algorithmic_bytes_per_launch': bytes_j,
+                         'counted_fp32_flop_per_launch': counted,
+                         'valu_lane_instructions_per_launch': lane_inst,
+                         'frac_of_attainable_issue': (
+                             (issue / attainable['fp32_vop2_lane_instructions_per_s'])
+                             if issue else None)},
+                     'order_loop_over_tabulated': tj / tt},
         'reductions': reductions,
         'fit_quality': {'converged_frac': float(ok.mean()),
                         'iterations_mean': float(iters.mean()),

@@ -253,6 +253,23 @@ int lmz_jacobian(const lmz_batch *batch, const double *params,
                  double *partials, const uint8_t *active,
                  hipStream_t stream);
 
+/* Normal-equation partials as lmz_jacobian's (J scaled by 1/noise,
+ * residuals from data), the way the batched LM forms them: on dense crops
+ * (width > 0, every pixel in the plane z = 0) the multipole sums and their
+ * tangents depend on the pixel only through its distance from the
+ * particle's axis, so the order loop runs once per sample of a radial grid
+ * and each pixel interpolates them; crops with explicit coordinates take
+ * the order loop at every pixel.  Agrees with lmz_jacobian to FP32
+ * rounding.  work: lmz_jacobian_tabulated_bytes, 256-byte aligned.
+ * No counterpart in the reference: the Jacobian pass of lmz_lm_run, for
+ * callers that want J^T J of dense crops without J in memory. */
+int lmz_jacobian_tabulated_bytes(const lmz_batch *batch, size_t *bytes);
+int lmz_jacobian_tabulated(const lmz_batch *batch, const double *params,
+                           const int32_t *nmax, const float *coef,
+                           const float *dcoef, int32_t flags,
+                           const float *data, double *partials, void *work,
+                           hipStream_t stream);
+
 /* Vector-Jacobian product of the hologram: grad_params[f][c] = sum over
  * feature f's pixels of grad_out[px] dI/d(param c), for the seven
  * parameters x_p y_p z_p a_p n_p k_p alpha, accumulated in FP64.  J is

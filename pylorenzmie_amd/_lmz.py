@@ -142,6 +142,8 @@ SIGNATURES = {
                           ctypes.POINTER(Schedule), _P]),
     'lmz_normal_eq': (_I, [_P, _P, _P, _P, _P]),
     'lmz_lm_workspace_bytes': (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    'lmz_jacobian_tabulated_bytes': (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    'lmz_jacobian_tabulated': (_I, [_P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
     'lmz_lm_plan_create': (_I, [_P, ctypes.POINTER(_P)]),
     'lmz_lm_plan_destroy': (_I, [_P]),
     'lmz_lm_run': (_I, [_P, _P, _I, _P, ctypes.POINTER(LMSettings), _P, _P,

@@ -1523,6 +1523,80 @@ int lmz_jacobian(const lmz_batch *b, const double *params, const int32_t *sel,
     return LMZ_OK;
 }
 
+// Workspace of lmz_jacobian_tabulated: tables, their grids, the two masks
+// and tile lists, list counters, list starts.
+struct TabLayout {
+    size_t rpar, rtab, mask_t, mask_l, list_t, list_l, count, start, total;
+};
+
+static TabLayout tab_layout(const lmz_batch *b) {
+    TabLayout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 255) & ~size_t(255);
+        return o;
+    };
+    L.rpar = take(sizeof(RTabPar) * (size_t)b->nfeat);
+    L.rtab = take(sizeof(float) * LMZ_RTAB_CAP * LMZ_RTAB_STRIDE * (size_t)b->nfeat);
+    L.mask_t = take((size_t)b->nfeat);
+    L.mask_l = take((size_t)b->nfeat);
+    L.list_t = take(sizeof(int32_t) * (size_t)b->ntiles);
+    L.list_l = take(sizeof(int32_t) * (size_t)b->ntiles);
+    L.count = take(sizeof(int32_t) * 32);
+    L.start = take(sizeof(int32_t) * (size_t)b->nfeat);
+    L.total = off;
+    return L;
+}
+
+int lmz_jacobian_tabulated_bytes(const lmz_batch *b, size_t *bytes) {
+    LMZ_REQUIRE(bytes != nullptr, "bytes is NULL");
+    if (int rc = check_batch(b)) return rc;
+    *bytes = tab_layout(b).total;
+    return LMZ_OK;
+}
+
+int lmz_jacobian_tabulated(const lmz_batch *b, const double *params,
+                           const int32_t *nmax, const float *coef,
+                           const float *dcoef, int32_t flags,
+                           const float *data, double *partials, void *work,
+                           hipStream_t stream) {
+    if (int rc = forward_args(b, params, nmax, coef, nullptr, nullptr)) return rc;
+    LMZ_REQUIRE(b->all_dense, "lmz_jacobian_tabulated needs dense crops");
+    LMZ_REQUIRE(dcoef != nullptr, "dcoef (coefficient tangents) is required");
+    LMZ_REQUIRE(flags & LMZ_CARTESIAN,
+                "the Jacobian is defined for the Cartesian hologram");
+    LMZ_REQUIRE(data && partials && work, "data, partials and work are required");
+    LMZ_REQUIRE((reinterpret_cast<uintptr_t>(work) & 255) == 0,
+                "work must be 256-byte aligned");
+    int grid_j = 0, grid_f = 0, grid_t = 0;
+    if (int rc = persistent_grids(b, &grid_j, &grid_f)) return rc;
+    if (int rc = rtab_grid(&grid_t)) return rc;
+    const TabLayout L = tab_layout(b);
+    char *ws = static_cast<char *>(work);
+    RTabPar *rpar = reinterpret_cast<RTabPar *>(ws + L.rpar);
+    float *rtab = reinterpret_cast<float *>(ws + L.rtab);
+    uint8_t *mask_t = reinterpret_cast<uint8_t *>(ws + L.mask_t);
+    uint8_t *mask_l = reinterpret_cast<uint8_t *>(ws + L.mask_l);
+    int32_t *count = reinterpret_cast<int32_t *>(ws + L.count);
+    int32_t *start = reinterpret_cast<int32_t *>(ws + L.start);
+    const TileList tl_t{reinterpret_cast<int32_t *>(ws + L.list_t), count,
+                        grid_t, start};
+    const TileList tl_l{reinterpret_cast<int32_t *>(ws + L.list_l), count + 16,
+                        grid_j, start};
+    launch_jac_rtab(b, params, nullptr, 0, nmax, coef, dcoef, flags, nullptr,
+                    rpar, rtab, mask_t, mask_l, stream);
+    launch_tile_list(b, mask_t, tl_t, stream);
+    launch_jacobian_rtab(b, params, nullptr, 0, nmax, nullptr, flags, data,
+                         nullptr, 0LL, partials, rpar, rtab, &tl_t, stream);
+    launch_tile_list(b, mask_l, tl_l, stream);
+    launch_jacobian(b, params, nullptr, 0, nmax, coef, dcoef, nullptr, flags,
+                    data, nullptr, 0LL, nullptr, nullptr, partials, nullptr,
+                    &tl_l, stream);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
 int lmz_hologram_vjp(const lmz_batch *b, const double *params,
                      const int32_t *nmax, const float *coef,
                      const float *dcoef, const float *coords, int32_t flags,

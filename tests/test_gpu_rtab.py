@@ -60,3 +60,60 @@ def test_tabulated_and_loop_jacobians_reach_the_same_minimum(fx):
     print('chi2 rel', (chi_t - chi_l) / chi_l)
     assert np.all(rel.max(axis=1) <= 1e-4)
     assert np.all(np.abs(chi_t - chi_l) <= 1e-8 * chi_l)
+
+
+def test_tabulated_partials_match_the_loop(fx):
+    """lmz_jacobian_tabulated (tables of the sums, interpolated) against
+    lmz_jacobian (the order loop at every pixel) on eight 201 x 201 crops:
+    per feature, J^T J and J^T r within 2e-5 of the scale sqrt(A_ii A_jj)
+    and sqrt(A_ii r^T r), r^T r within 1e-5 (both take the residual from
+    their FP32 intensity)."""
+    import torch
+    from pylorenzmie_amd import _lmz
+    from pylorenzmie_amd.theory import Instrument
+    from pylorenzmie_amd.theory.batch import HologramBatch
+    inst = Instrument(noise=float(fx['head_sigma']), **INSTRUMENT)
+    n, side = 8, 201
+    q = np.asarray(fx['head_start'][:n])
+    caps = [_lmz.nmax_cap(p[3], p[4], p[5], inst.n_m, inst.wavelength) for p in q]
+    hb = HologramBatch([(side, side)] * n, caps, None, inst)
+    b = hb.batch
+    lib = _lmz.load_library()
+    pp = torch.zeros((n, _lmz.NPARAM), dtype=torch.float64, device='cuda')
+    pp[:, :7] = torch.from_numpy(q[:, :7])
+    nmax = torch.zeros(n, dtype=torch.int32, device='cuda')
+    coef = torch.zeros(b.coef_floats(), dtype=torch.float32, device='cuda')
+    dcoef = torch.zeros(b.dcoef_floats(), dtype=torch.float32, device='cuda')
+    data = torch.from_numpy(np.asarray(fx['head_data'][:n], np.float32).reshape(-1)).cuda()
+    s = _lmz.stream_handle()
+    _lmz.check(lib.lmz_mie_ab(b.handle, _lmz.ptr(pp), None, _lmz.ptr(nmax),
+                              _lmz.ptr(coef), _lmz.ptr(dcoef), None,
+                              _lmz.ptr(hb.work), s))
+    pl = torch.zeros(36 * b.ntiles, dtype=torch.float64, device='cuda')
+    _lmz.check(lib.lmz_jacobian(b.handle, _lmz.ptr(pp), None, _lmz.ptr(nmax),
+                                _lmz.ptr(coef), _lmz.ptr(dcoef), None, 3,
+                                _lmz.ptr(data), None, None, _lmz.ptr(pl), None, s))
+    nb = _lmz.ctypes.c_size_t()
+    _lmz.check(lib.lmz_jacobian_tabulated_bytes(b.handle, _lmz.ctypes.byref(nb)))
+    work = torch.zeros((nb.value + 255) // 8 + 64, dtype=torch.float64, device='cuda')
+    pt = torch.zeros_like(pl)
+    _lmz.check(lib.lmz_jacobian_tabulated(b.handle, _lmz.ptr(pp), _lmz.ptr(nmax),
+                                          _lmz.ptr(coef), _lmz.ptr(dcoef), 3,
+                                          _lmz.ptr(data), _lmz.ptr(pt),
+                                          _lmz.ptr(work), s))
+    torch.cuda.synchronize()
+    tiles = b.ntiles // n
+    L = pl.cpu().numpy().reshape(n, tiles, 36).sum(axis=1)
+    T = pt.cpu().numpy().reshape(n, tiles, 36).sum(axis=1)
+    iu = np.triu_indices(7)
+    for f in range(n):
+        A = np.zeros((7, 7))
+        A[iu] = L[f, :28]
+        d = np.sqrt(np.diag(A))
+        scale = np.zeros((7, 7))
+        scale[iu] = np.outer(d, d)[iu]
+        err = np.abs(T[f, :28] - L[f, :28]) / scale[iu]
+        errg = np.abs(T[f, 28:35] - L[f, 28:35]) / (d * np.sqrt(L[f, 35]))
+        print(f, err.max(), errg.max())
+        assert err.max() <= 2e-5 and errg.max() <= 2e-5
+        assert T[f, 35] == pytest.approx(L[f, 35], rel=1e-5)

@@ -565,6 +565,114 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
   }
 }
 
+// FP64 radial tables of the forward sums (ftab_params): one thread per
+// sample, 64 per block, blockIdx.y the sample block; thread 0 of the first
+// block writes the feature's grid and sorts the active features into
+// mask_tab (k_forward_ftab) and mask_loop (k_forward).  Coefficient buffer
+// sel[f] ^ flip, as the residual kernel reads.
+__global__ __launch_bounds__(64) void k_fwd_ftab(
+    const lmz_feature *__restrict__ feat, const double *__restrict__ params,
+    const int32_t *__restrict__ sel, int flip, int nbuf,
+    const int32_t *__restrict__ nmax, const float *__restrict__ coef,
+    int flags, const uint8_t *__restrict__ active, FTabPar *__restrict__ fpar,
+    double *__restrict__ ftab, uint8_t *__restrict__ mask_tab,
+    uint8_t *__restrict__ mask_loop) {
+    const int f = blockIdx.x;
+    if (active && !active[f]) {
+        if (blockIdx.y == 0 && threadIdx.x == 0) mask_tab[f] = mask_loop[f] = 0;
+        return;
+    }
+    const lmz_feature &F = feat[f];
+    const int b = sel ? (sel[f] ^ flip) : 0;
+    const int N = nmax[(long long)f * nbuf + b];
+    const FeatUniform fu = feature_uniform(
+        F, params + (long long)LMZ_NPARAM * f, N,
+        F.coef_offset + b * F.nmax_cap);
+    const FTabPar fp = ftab_params(fu);
+    if (blockIdx.y == 0 && threadIdx.x == 0) {
+        fpar[f] = fp;
+        mask_tab[f] = fp.m > 0 ? 1 : 0;
+        mask_loop[f] = fp.m > 0 ? 0 : 1;
+    }
+    const int j = blockIdx.y * 64 + threadIdx.x;
+    if (j >= fp.m) return;
+    ftab_sample(fu, fp, flags & LMZ_BOHREN, j,
+                coef + (long long)LMZ_COEF_STRIDE * fu.slot0,
+                ftab + ((long long)f * LMZ_FTAB_CAP + j) * LMZ_FTAB_STRIDE);
+}
+
+// The FP64 residual (I - d) / noise and per-tile r^T r of the tabulated
+// features' tiles (the LM's trial forward pass on dense crops): each pixel
+// interpolates its feature's table (k_fwd_ftab), kept in LDS while it fits
+// (LMZ_FTAB_LDS rows, restaged when the block's feature changes), and
+// projects and squares as k_forward<2>.  Persistent grid over a tile list;
+// the residual goes to out + (sel[f] ^ flip) * out_stride.
+template <int NT = LMZ_THREADS>
+__global__ __launch_bounds__(NT) void k_forward_ftab(
+    const lmz_feature *__restrict__ feat,
+    const int32_t *__restrict__ tile_feature,
+    const double *__restrict__ params, const int32_t *__restrict__ sel,
+    int flip, int nbuf, const int32_t *__restrict__ nmax,
+    const float *__restrict__ coords, long long npix_total, int flags,
+    const float *__restrict__ data, float *__restrict__ out,
+    long long out_stride, double *__restrict__ chi2_tiles,
+    const FTabPar *__restrict__ fpar, const double *__restrict__ ftab,
+    const int32_t *__restrict__ list, int32_t *count) {
+  __shared__ int next_w[2];
+  __shared__ double stab[LMZ_FTAB_LDS * LMZ_FTAB_STRIDE];
+  TileCursor cursor;
+  int staged = -1;   // feature whose table stab holds
+  const int nwork = count[0];
+  for (int w = next_tile(count, next_w, cursor); w < nwork;
+       w = next_tile(count, next_w, cursor)) {
+    const int t = list[w];
+    const int f = tile_feature[t];
+    const lmz_feature &F = feat[f];
+    const int b = sel ? (sel[f] ^ flip) : 0;
+    const int N = nmax[(long long)f * nbuf + b];
+    const FeatUniform fu = feature_uniform(
+        F, params + (long long)LMZ_NPARAM * f, N,
+        F.coef_offset + b * F.nmax_cap);
+    const bool bohren = flags & LMZ_BOHREN;
+    const FTabPar fp = fpar[f];
+    const double *gtab = ftab + (long long)f * (LMZ_FTAB_CAP * LMZ_FTAB_STRIDE);
+    const bool lds = fp.m <= LMZ_FTAB_LDS;
+    if (lds && staged != f) {
+        // every thread has finished the previous tile (the barriers of the
+        // reduction and of next_tile)
+        const double2 *src = reinterpret_cast<const double2 *>(gtab);
+        double2 *dst = reinterpret_cast<double2 *>(stab);
+        for (int i = threadIdx.x; i < fp.m * (LMZ_FTAB_STRIDE / 2); i += NT)
+            dst[i] = src[i];
+        __syncthreads();
+        staged = f;
+    }
+    float *outf = out ? out + b * out_stride : nullptr;
+    double chi = 0.0;
+#pragma unroll 1
+    for (int q = 0; q < LMZ_TILE / NT; ++q) {
+        const int j = (t - fu.tile0) * LMZ_TILE + q * NT + threadIdx.x;
+        if (j >= fu.npix) break;
+        float ax, ay, az;
+        pixel_coords(fu, j, coords, npix_total, ax, ay, az);
+        const long long gi = fu.pix_offset + j;
+        PixGeomD g;
+        pixel_geometry_d(fu, ax, ay, az, bohren, g);
+        const SumsD S = lds ? forward_sum_ftab<LMZ_FTAB_STRIDE>(g, fp, stab)
+                            : forward_sum_ftab<LMZ_FTAB_STRIDE>(g, fp, gtab);
+        cd E[3];
+        field_from_sums(g, S, true, E);
+        const double I = intensity(E, fu.alphad);
+        const double r = (I - (double)data[gi]) / F.noise;
+        if (outf) outf[gi] = (float)r;
+        chi = fma(r, r, chi);
+    }
+    double v[1] = {chi};
+    block_reduce_store_d<1, NT>(v, chi2_tiles + t);
+    __syncthreads();   // the reduction buffer is reused by the next tile
+  }
+}
+
 // Hologram of one pixel in FP64 (the residual kernel's arithmetic) for the
 // particle described by fu with coefficient table cf; E receives the field
 // without alpha.
@@ -988,6 +1096,31 @@ void launch_jac_rtab(const lmz_batch *b, const double *params,
                        rtab, mask_tab, mask_loop);
 }
 
+void launch_fwd_ftab(const lmz_batch *b, const double *params,
+                     const int32_t *sel, int flip, const int32_t *nmax,
+                     const float *coef, int flags, const uint8_t *active,
+                     FTabPar *fpar, double *ftab, uint8_t *mask_tab,
+                     uint8_t *mask_loop, hipStream_t stream) {
+    hipLaunchKernelGGL(k_fwd_ftab, dim3(b->nfeat, LMZ_FTAB_CAP / 64), dim3(64),
+                       0, stream, b->d_feat, params, sel, flip,
+                       b->coef_buffers, nmax, coef, flags, active, fpar, ftab,
+                       mask_tab, mask_loop);
+}
+
+void launch_forward_ftab(const lmz_batch *b, const double *params,
+                         const int32_t *sel, int flip, const int32_t *nmax,
+                         int flags, const float *data, float *out,
+                         long long out_stride, double *chi2_tiles,
+                         const FTabPar *fpar, const double *ftab,
+                         const TileList *tl, hipStream_t stream) {
+    hipLaunchKernelGGL((k_forward_ftab<LMZ_THREADS>), dim3(tl->grid),
+                       dim3(LMZ_THREADS), 0, stream, b->d_feat,
+                       b->d_tile_feature, params, sel, flip, b->coef_buffers,
+                       nmax, (const float *)nullptr, b->npix_total, flags, data,
+                       out, out_stride, chi2_tiles, fpar, ftab,
+                       (const int32_t *)tl->list, tl->count);
+}
+
 void launch_jacobian_rtab(const lmz_batch *b, const double *params,
                           const int32_t *sel, int flip, const int32_t *nmax,
                           const float *coords, int flags, const float *data,
@@ -1137,8 +1270,9 @@ int persistent_grids(const lmz_batch *b, int *grid_jacobian, int *grid_forward) 
     return LMZ_OK;
 }
 
-// Persistent grid of k_jacobian_rtab: as many workgroups as are resident.
-int rtab_grid(int *grid) {
+// Persistent grids of k_jacobian_rtab and k_forward_ftab: as many
+// workgroups as are resident.
+int rtab_grid(int *grid, int *grid_fwd) {
     int cus = 0;
     if (int rc = device_cus(&cus)) return rc;
     int n = 0;
@@ -1147,6 +1281,14 @@ int rtab_grid(int *grid) {
             LMZ_THREADS, 0) != hipSuccess || n < 1)
         n = 1;
     *grid = cus * n;
+    if (grid_fwd) {
+        n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void *>(&k_forward_ftab<LMZ_THREADS>),
+                LMZ_THREADS, 0) != hipSuccess || n < 1)
+            n = 1;
+        *grid_fwd = cus * n;
+    }
     return LMZ_OK;
 }
 
@@ -1571,7 +1713,7 @@ int lmz_jacobian_tabulated(const lmz_batch *b, const double *params,
                 "work must be 256-byte aligned");
     int grid_j = 0, grid_f = 0, grid_t = 0;
     if (int rc = persistent_grids(b, &grid_j, &grid_f)) return rc;
-    if (int rc = rtab_grid(&grid_t)) return rc;
+    if (int rc = rtab_grid(&grid_t, nullptr)) return rc;
     const TabLayout L = tab_layout(b);
     char *ws = static_cast<char *>(work);
     RTabPar *rpar = reinterpret_cast<RTabPar *>(ws + L.rpar);

@@ -72,6 +72,20 @@ void launch_jac_rtab(const lmz_batch *b, const double *params,
                      const float *coef, const float *dcoef, int flags,
                      const uint8_t *active, RTabPar *rpar, float *rtab,
                      uint8_t *mask_tab, uint8_t *mask_loop, hipStream_t stream);
+// FP64 radial tables of the forward sums (fpar [nfeat], ftab
+// [nfeat][LMZ_FTAB_CAP][LMZ_FTAB_STRIDE]) and the residual pass over the
+// tabulated features' tiles in tl, as launch_residual's.
+void launch_fwd_ftab(const lmz_batch *b, const double *params,
+                     const int32_t *sel, int flip, const int32_t *nmax,
+                     const float *coef, int flags, const uint8_t *active,
+                     FTabPar *fpar, double *ftab, uint8_t *mask_tab,
+                     uint8_t *mask_loop, hipStream_t stream);
+void launch_forward_ftab(const lmz_batch *b, const double *params,
+                         const int32_t *sel, int flip, const int32_t *nmax,
+                         int flags, const float *data, float *out,
+                         long long out_stride, double *chi2_tiles,
+                         const FTabPar *fpar, const double *ftab,
+                         const TileList *tl, hipStream_t stream);
 // Normal-equation partials of the tabulated features' tiles in tl.
 void launch_jacobian_rtab(const lmz_batch *b, const double *params,
                           const int32_t *sel, int flip, const int32_t *nmax,
@@ -106,7 +120,7 @@ void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
                       const TileList &tl, hipStream_t stream);
 // Persistent grids of the LM's pixel launches under the batch's schedule.
 int persistent_grids(const lmz_batch *b, int *grid_jacobian, int *grid_forward);
-int rtab_grid(int *grid);
+int rtab_grid(int *grid, int *grid_fwd);
 int hip_fail(hipError_t e, const char *what);
 int check_batch(const lmz_batch *b);
 

@@ -26,7 +26,8 @@ struct LMLayout {
     size_t coef, dcoef, nmax, sel, work, partials, chi2_tiles, state, ptrial,
         mask_j, mask_f, count, resid, list_j, list_f, tile_start, fd_h,
         fd_params, fd_nmax,
-        fd_coef, rpar, rtab, mask_jt, mask_jl, list_jl, total;
+        fd_coef, rpar, rtab, mask_jt, mask_jl, list_jl, fpar, ftab, mask_ft,
+        mask_fl, list_fl, total;
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -48,7 +49,7 @@ static LMLayout lm_layout(const lmz_batch *b) {
     L.ptrial = take(sizeof(double) * LMZ_NPARAM * (size_t)b->nfeat);
     L.mask_j = take((size_t)b->nfeat);
     L.mask_f = take((size_t)b->nfeat);
-    L.count = take(sizeof(int32_t) * 64);   // active count, three TileLists
+    L.count = take(sizeof(int32_t) * 80);   // active count, four TileLists
     L.resid = take(sizeof(float) * 2 * (size_t)b->npix_total);
     L.list_j = take(sizeof(int32_t) * (size_t)b->ntiles);
     L.list_f = take(sizeof(int32_t) * (size_t)b->ntiles);
@@ -64,6 +65,12 @@ static LMLayout lm_layout(const lmz_batch *b) {
     L.mask_jt = take((size_t)b->nfeat);    // tabulated / order-loop features
     L.mask_jl = take((size_t)b->nfeat);
     L.list_jl = take(sizeof(int32_t) * (size_t)b->ntiles);
+    // FP64 radial tables of the trial forward pass (dense crops)
+    L.fpar = take(sizeof(FTabPar) * (size_t)b->nfeat);
+    L.ftab = take(sizeof(double) * LMZ_FTAB_CAP * LMZ_FTAB_STRIDE * (size_t)b->nfeat);
+    L.mask_ft = take((size_t)b->nfeat);
+    L.mask_fl = take((size_t)b->nfeat);
+    L.list_fl = take(sizeof(int32_t) * (size_t)b->ntiles);
     L.total = off;
     return L;
 }
@@ -218,7 +225,7 @@ using namespace lmz;
 struct lmz_lm_plan {
     const lmz_batch *batch = nullptr;
     int grid_j = 0, grid_f = 0;       // persistent grids of the pixel kernels
-    int grid_t = 0;                   // and of k_jacobian_rtab
+    int grid_t = 0, grid_ft = 0;      // and of k_jacobian_rtab, k_forward_ftab
     hipStream_t s = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
     int32_t *h_count = nullptr;       // pinned
@@ -276,6 +283,53 @@ struct MieSource {
     const int32_t *nlayers;
 };
 
+// The FP64 residual pass at params (coefficient buffer sel[f] ^ flip) over
+// the features in mask (all when null): residuals into resid's buffer, r^T r
+// per tile into chi2_tiles.  Dense crops from FP64 radial tables of the sums
+// (k_forward_ftab), the rest by the order loop at every pixel (k_forward<2>);
+// all by the order loop with FP32 residuals, and with a forward-difference
+// Jacobian, whose differences take r at the base point from the same
+// kernel as at the displaced ones.
+static void enqueue_forward(const lmz_lm_plan *plan, const LMLayout &L,
+                            char *ws, const float *coords, int32_t flags,
+                            const float *data, const lmz_lm_settings &set,
+                            const double *params, int flip,
+                            const uint8_t *mask, hipStream_t s) {
+    const lmz_batch *b = plan->batch;
+    const int32_t *nmax = reinterpret_cast<const int32_t *>(ws + L.nmax);
+    const int32_t *sel = reinterpret_cast<const int32_t *>(ws + L.sel);
+    const float *coef = reinterpret_cast<const float *>(ws + L.coef);
+    double *chi2_tiles = reinterpret_cast<double *>(ws + L.chi2_tiles);
+    int32_t *count = reinterpret_cast<int32_t *>(ws + L.count);
+    float *resid = reinterpret_cast<float *>(ws + L.resid);
+    int32_t *tile_start = reinterpret_cast<int32_t *>(ws + L.tile_start);
+    const long long np = b->npix_total;
+    const TileList tl_f{reinterpret_cast<int32_t *>(ws + L.list_f), count + 32,
+                        plan->grid_f, tile_start};
+    if ((flags & LMZ_FP32) || set.fd_step > 0.0) {
+        if (mask) launch_tile_list(b, mask, tl_f, s);
+        launch_residual(b, params, sel, flip, nmax, coef, coords, flags, data,
+                        resid, np, chi2_tiles, nullptr, mask ? &tl_f : nullptr,
+                        s);
+        return;
+    }
+    FTabPar *fpar = reinterpret_cast<FTabPar *>(ws + L.fpar);
+    double *ftab = reinterpret_cast<double *>(ws + L.ftab);
+    uint8_t *mask_ft = reinterpret_cast<uint8_t *>(ws + L.mask_ft);
+    uint8_t *mask_fl = reinterpret_cast<uint8_t *>(ws + L.mask_fl);
+    const TileList tl_ft{tl_f.list, tl_f.count, plan->grid_ft, tile_start};
+    const TileList tl_fl{reinterpret_cast<int32_t *>(ws + L.list_fl),
+                         count + 64, plan->grid_f, tile_start};
+    launch_fwd_ftab(b, params, sel, flip, nmax, coef, flags, mask, fpar, ftab,
+                    mask_ft, mask_fl, s);
+    launch_tile_list(b, mask_ft, tl_ft, s);
+    launch_forward_ftab(b, params, sel, flip, nmax, flags, data, resid, np,
+                        chi2_tiles, fpar, ftab, &tl_ft, s);
+    launch_tile_list(b, mask_fl, tl_fl, s);
+    launch_residual(b, params, sel, flip, nmax, coef, coords, flags, data,
+                    resid, np, chi2_tiles, nullptr, &tl_fl, s);
+}
+
 static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
                               char *ws, const float *coords, int32_t flags,
                               const float *data, const lmz_lm_settings &set,
@@ -299,8 +353,6 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
     int32_t *tile_start = reinterpret_cast<int32_t *>(ws + L.tile_start);
     const TileList tl_j{reinterpret_cast<int32_t *>(ws + L.list_j), count + 16,
                         plan->grid_j, tile_start};
-    const TileList tl_f{reinterpret_cast<int32_t *>(ws + L.list_f), count + 32,
-                        plan->grid_f, tile_start};
     // J^T J and J^T r at the current point, r from its FP64 forward pass,
     // over the tiles of the features that accepted their last step
     if (set.fd_step > 0.0) {
@@ -345,9 +397,8 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
                            coef, dcoef, work, mask_f, s);
     else if (!src.fixed_ab)
         launch_mie(b, ptrial, sel, 1, nmax, coef, dcoef, nullptr, work, mask_f, s);
-    launch_tile_list(b, mask_f, tl_f, s);
-    launch_residual(b, ptrial, sel, 1, nmax, coef, coords, flags, data,
-                    resid, np, chi2_tiles, nullptr, &tl_f, s);
+    enqueue_forward(plan, L, ws, coords, flags, data, set, ptrial, 1, mask_f,
+                    s);
     hipLaunchKernelGGL(k_lm_accept, dim3(b->nfeat), dim3(64), 0, s, b->d_feat,
                        chi2_tiles, st, params, ptrial, sel, mask_j, mask_f, set);
     hipLaunchKernelGGL(k_lm_count, dim3(1), dim3(256), 0, s, b->nfeat, mask_f,
@@ -439,7 +490,7 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
                 "workspace must be 256-byte aligned");
 
     if (int rc = persistent_grids(b, &plan->grid_j, &plan->grid_f)) return rc;
-    if (int rc = rtab_grid(&plan->grid_t)) return rc;
+    if (int rc = rtab_grid(&plan->grid_t, &plan->grid_ft)) return rc;
     const LMLayout L = lm_layout(b);
     char *ws = static_cast<char *>(workspace);
     LMState *st = reinterpret_cast<LMState *>(ws + L.state);
@@ -473,8 +524,8 @@ static int lm_run(lmz_lm_plan *plan, const float *coords, int32_t flags,
                            dcoef, work, nullptr, s);
     else
         launch_mie(b, params_io, sel, 0, nmax, coef, dcoef, nullptr, work, nullptr, s);
-    launch_residual(b, params_io, sel, 0, nmax, coef, coords, flags, data,
-                    resid, np, chi2_tiles, nullptr, nullptr, s);
+    enqueue_forward(plan, L, ws, coords, flags, data, set, params_io, 0,
+                    nullptr, s);
     hipLaunchKernelGGL(k_lm_init_chi2, dim3(b->nfeat), dim3(64), 0, s, b->nfeat,
                        b->d_feat, chi2_tiles, st);
     hipLaunchKernelGGL(k_lm_mask_init, dim3((b->nfeat + 63) / 64), dim3(64), 0, s,

@@ -114,6 +114,37 @@ int chk_jac_rtab(const lmz_feature *feat, const double *params,
     return p.m;
 }
 
+// FP64 hologram of every pixel of a dense crop by the order loop (holo_loop)
+// and by the FP64 radial table (holo_tab, when tabulated); returns the
+// table's sample count (0: not tabulated).
+int chk_fwd_ftab(const lmz_feature *feat, const double *params,
+                 const float *coef, int nmax, int flags, double *holo_loop,
+                 double *holo_tab, double *h_out) {
+    const FeatUniform fu = feature_uniform(*feat, params, nmax, 0);
+    const bool bohren = flags & LMZ_BOHREN;
+    const FTabPar p = ftab_params(fu);
+    std::vector<double> tab((size_t)LMZ_FTAB_STRIDE * (p.m > 0 ? p.m : 1));
+    for (int j = 0; j < p.m; ++j)
+        ftab_sample(fu, p, bohren, j, coef, &tab[(size_t)j * LMZ_FTAB_STRIDE]);
+    for (int j = 0; j < fu.npix; ++j) {
+        float ax, ay, az;
+        pixel_coords(fu, j, nullptr, fu.npix, ax, ay, az);
+        PixGeomD g;
+        pixel_geometry_d(fu, ax, ay, az, bohren, g);
+        cd E[3];
+        field_from_sums(g, forward_sum(g, coef, nmax), true, E);
+        holo_loop[j] = intensity(E, fu.alphad);
+        if (p.m > 0) {
+            field_from_sums(g, forward_sum_ftab(g, p, tab.data()), true, E);
+            holo_tab[j] = intensity(E, fu.alphad);
+        } else {
+            holo_tab[j] = 0.0;
+        }
+    }
+    *h_out = p.h;
+    return p.m;
+}
+
 // FP32 hologram of one feature (the residual kernel's fast mode).
 void chk_holo32(const lmz_feature *feat, const double *params,
                 const float *coef, int nmax, const float *coords, int npix,

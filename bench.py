@@ -5,10 +5,10 @@ as the config-3 distribution (a_p U[0.4, 3.0] um, n_p U[1.38, 1.65], k_p
 U[0, 0.01], z_p U[50, 400] px, x/y at the crop centre +-5 px, alpha
 U[0.8, 1.2]), Gaussian noise sigma = 0.05, starts perturbed by +-2 px,
 x(1 +- 0.05) a_p, +-0.02 n_p.  One step fits all 4096 with the batched
-device-resident LM (forward-mode Jacobian from radial tables of the multipole
-sums, FP64 trial forward pass and solve, HIP graph per iteration, <= 50
-iterations), from the starting parameters already in HBM to the results
-copied back to the host.
+device-resident LM (forward-mode FP32 Jacobian and FP64 trial forward pass,
+both from radial tables of the multipole sums, FP64 solve, HIP graph per
+iteration, <= 50 iterations), from the starting parameters already in HBM to
+the results copied back to the host.
 
 Also measured:
   fd      the plain GPU finite-difference form (hipOptimizer: one feature at
@@ -18,19 +18,22 @@ Also measured:
           FP64 hologram (lmz_lm_settings.fd_step = 1e-5, the reference's
           diff_step) in place of the analytic one -> the adjoint's share of
           the speedup;
-  roofline  the trial-forward kernel k_forward<2> (dominant: FP64 sums
-          over every pixel and order), timed with HIP events on the stream
-          it runs on, against the FP64 vector peak, with an algorithmic
-          FLOP count per pixel of 64 N + 70, N the order count; and against
-          the attainable FP64 FMA rate of this build, measured in the same
-          run by the FMA-chain probe (tests/native/lmz_valu_peak.hip): the
-          kernel's counted FP64 FLOP and VALU instructions per launch (from
-          the PMC summary committed for these sources) over the probe's;
-  jacobian  the LM's Jacobian pass (lmz_jacobian_tabulated: radial tables
-          of the sums and their interpolation, the J^T J partials) and, for
-          comparison, the order loop at every pixel (lmz_jacobian, 296 N +
-          480 FLOP per pixel against the FP32 peak and the probe's
-          two-operand FP32 FMA rate);
+  roofline  the LM's trial forward pass (lmz_residual_tabulated: FP64
+          radial tables of the multipole sums, k_fwd_ftab, and their
+          12-point interpolation with the residual and chi2 per pixel,
+          k_forward_ftab), timed with HIP events on the stream it runs on,
+          against the FP64 vector peak; algorithmic FLOP: 64 N per table
+          sample (N the order count) and 280 per pixel; and against the
+          attainable FP64 FMA rate of this build, measured in the same run
+          by the FMA-chain probe (tests/native/lmz_valu_peak.hip): the
+          counted FP64 FLOP and VALU instructions per launch (from the PMC
+          summary committed for these sources) over the probe's.  For
+          comparison the order loop at every pixel (k_forward<2>, 64 N + 70
+          FLOP per pixel);
+  jacobian  the LM's Jacobian pass (lmz_jacobian_tabulated: FP32 radial
+          tables and their 6-point interpolation, the J^T J partials;
+          296 N per sample, 954 per pixel) against the FP32 peak, and the
+          order loop at every pixel (k_jacobian, 296 N + 480 per pixel);
   cpu_baseline  the FP64 oracle (numpy model + scipy MINPACK LM) on the
           host's cores (one process per core, two features each, at most
           16 cores), started as a child process that never touches the
@@ -274,11 +277,21 @@ def kernel_times(lm, settings, args):
             b.handle, _lmz.ptr(lm.params), _lmz.ptr(nmax), _lmz.ptr(coef),
             _lmz.ptr(dcoef), flags, _lmz.ptr(lm.data), _lmz.ptr(partials),
             _lmz.ptr(tabwork), s))
+    # the LM's trial forward pass on dense crops: FP64 radial tables of the
+    # sums, interpolated per pixel (k_fwd_ftab, k_forward_ftab)
+    _lmz.check(lib.lmz_residual_tabulated_bytes(b.handle, ctypes.byref(nb)))
+    fwork = torch.empty(nb.value // 8 + 64, dtype=torch.float64, device=dev)
+
+    def ftab():
+        _lmz.check(lib.lmz_residual_tabulated(
+            b.handle, _lmz.ptr(lm.params), _lmz.ptr(nmax), _lmz.ptr(coef),
+            flags, _lmz.ptr(lm.data), None, _lmz.ptr(chi2), _lmz.ptr(fwork), s))
     jac()
     fwd()
     jtab()
+    ftab()
     reps = args.kernel_reps
-    et = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    et = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     ev[0].record()
     for _ in range(reps):
         jac()
@@ -290,13 +303,33 @@ def kernel_times(lm, settings, args):
     for _ in range(reps):
         jtab()
     et[1].record()
+    for _ in range(reps):
+        ftab()
+    et[2].record()
     torch.cuda.synchronize()
     tj = ev[0].elapsed_time(ev[1]) / reps * 1e-3
     tf = ev[1].elapsed_time(ev[2]) / reps * 1e-3
     tt = et[0].elapsed_time(et[1]) / reps * 1e-3
-    del tabwork
+    tft = et[1].elapsed_time(et[2]) / reps * 1e-3
+    del tabwork, fwork
     nm = nmax.cpu().numpy()[0::2].astype(float)
     P = np.array(lm.npix, float)
+    # radial table samples per launch, as lmz_pixel.h's rtab_params and
+    # ftab_params place them (dense crops; the particle's axis from the
+    # parameters, the crop from the feature table)
+    pr = lm.params.cpu().numpy()
+    kpx = 2. * np.pi / INSTRUMENT['wavelength'] * INSTRUMENT['n_m'] * \
+        INSTRUMENT['magnification']
+    side = np.sqrt(np.asarray(lm.npix, float))
+    px, py, Zk = pr[:, 0], pr[:, 1], kpx * np.abs(pr[:, 2])
+    nx = np.maximum(np.maximum(-px, px - (side - 1)), 0.)
+    ny = np.maximum(np.maximum(-py, py - (side - 1)), 0.)
+    fx = np.maximum(px, side - 1 - px)
+    fy = np.maximum(py, side - 1 - py)
+    span = kpx * (np.hypot(fx, fy) - np.hypot(nx, ny))
+    m_j = np.floor(span / np.minimum(0.35 * Zk / nm, 1.) + 12.)
+    m_f = np.floor(span / np.minimum(0.25 * Zk / nm, 1.) + 24.)
+    tables = {'jacobian_samples': float(m_j.sum()), 'forward_samples': float(m_f.sum())}
     flop_j = float(np.sum(P * (296. * nm + 480.)))
     flop_f = float(np.sum(P * (64. * nm + 70.)))
     # the standalone J^T J reduction over a Jacobian written to HBM
@@ -321,7 +354,15 @@ def kernel_times(lm, settings, args):
     torch.cuda.synchronize()
     tn = ev[2].elapsed_time(ev[3]) / max(reps, 5) * 1e-3
     del J, r
-    return tj, tf, tn, tt, flop_j, flop_f, float(np.sum(P)), float(np.mean(nm))
+    # algorithmic FLOP of the tabulated passes: the order loop per table
+    # sample (296 N, 64 N) and per pixel the interpolation (2 x 30 x 6 + 4 x
+    # 6 weights, 2 x 6 x 12 + 4 x 12 weights), the turn by e^{i psi} (15
+    # and 3 complex products) and the pixel's geometry and projection (480,
+    # 70 as the loop's)
+    flop_jt = float(np.sum(296. * nm * m_j) + np.sum(P) * (360. + 24. + 90. + 480.))
+    flop_ft = float(np.sum(64. * nm * m_f) + np.sum(P) * (144. + 48. + 18. + 70.))
+    return (tj, tf, tn, tt, tft, flop_j, flop_f, flop_jt, flop_ft,
+            float(np.sum(P)), float(np.mean(nm)), tables)
 
 
 HEADLINE_FIXTURE = os.path.join(ROOT, 'tests', 'golden', 'headline.npz')
@@ -716,13 +757,15 @@ def run_config(args):
         st = res[:, 38].astype(int)
         ok = ((st & 7) != 0) & ((st & 56) == 0)
         lm.params.copy_(torch.from_numpy(np.hstack([start, np.zeros((n, 1))])).cuda())
-        tj, tf, tn, tt, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
+        (tj, tf, tn, tt, tft, flop_j, flop_f, flop_jt, flop_ft, npx, nmean,
+         _) = kernel_times(lm, settings, args)
         bytes_n = 32. * npx + 36 * 8. * n
         line.update(workload=f'batched 7-parameter LM, {n} crops of side '
                              'log-U[48, 512], sigma in {0.02, 0.05, 0.1}',
                     pixels=float(npx), fits_per_s=n / dt, seconds=dt,
                     converged_frac=float(ok.mean()),
                     iterations_mean=float(res[:, 37].mean()),
+                    forward_tabulated_ms=tft * 1e3,
                     jacobian_tabulated_ms=tt * 1e3,
                     jacobian_loop_ms=tj * 1e3,
                     jacobian_loop_tflops=flop_j / tj / 1e12,
@@ -1246,11 +1289,15 @@ def main():
     log(f"bench: {agree['frac_within_1e-4_rel_and_kp_1e-5_abs']:.4f} of "
         f"{agree['features_compared']} fits within 1e-4 of the converged minimum")
     log('bench: kernel timing')
-    tj, tf, tn, tt, flop_j, flop_f, npx, nmean = kernel_times(lm, settings, args)
+    (tj, tf, tn, tt, tft, flop_j, flop_f, flop_jt, flop_ft, npx, nmean,
+     tables) = kernel_times(lm, settings, args)
     ach_j = flop_j / tj / 1e12
     ach_f = flop_f / tf / 1e12
-    log(f'bench: forward {tf * 1e3:.2f} ms {ach_f:.1f} TFLOP/s; jacobian '
-        f'tabulated {tt * 1e3:.2f} ms, order loop {tj * 1e3:.2f} ms')
+    ach_jt = flop_jt / tt / 1e12
+    ach_ft = flop_ft / tft / 1e12
+    log(f'bench: tabulated forward {tft * 1e3:.2f} ms {ach_ft:.1f} TFLOP/s, '
+        f'jacobian {tt * 1e3:.2f} ms {ach_jt:.1f} TFLOP/s; order loops '
+        f'{tf * 1e3:.2f} ms, {tj * 1e3:.2f} ms')
 
     # J reads the data (4 B / pixel) and writes 36 FP64 partials per tile
     bytes_j = 4. * npx + 36 * 8. * lm.batch.ntiles
@@ -1266,9 +1313,9 @@ def main():
                       'traffic': committed_figure(
                           'k_normal_eq_traffic', 'traffic_bytes_per_launch',
                           args.nfeat, args.side)},
-        'residual_chi2': {'kernel': 'k_forward<2>', 'bound': 'valu (FP64)',
-                          'achieved': bytes_r / tf / 1e9, 'unit': 'GB/s',
-                          'avg_launch_ms': tf * 1e3, 'bytes_per_launch': bytes_r}}
+        'residual_chi2': {'kernel': 'k_forward_ftab', 'bound': 'valu (FP64)',
+                          'achieved': bytes_r / tft / 1e9, 'unit': 'GB/s',
+                          'avg_launch_ms': tft * 1e3, 'bytes_per_launch': bytes_r}}
     log(f'bench: normal_eq {tn * 1e3:.3f} ms {bytes_n / tn / 1e9:.0f} GB/s')
     traffic = committed_figure('k_jacobian_traffic', 'traffic_bytes_per_launch',
                                args.nfeat, args.side)
@@ -1276,14 +1323,16 @@ def main():
                                args.nfeat, args.side)
     lane_inst = committed_figure('valu_counters', 'valu_lane_instructions_per_launch',
                                  args.nfeat, args.side)
-    traffic_f = committed_figure('k_jacobian_traffic', 'forward_hbm_bytes',
-                                 args.nfeat, args.side)
-    counted_f = committed_figure('valu_counters', 'forward_counted_fp64_flop',
-                                 args.nfeat, args.side)
-    lane_f = committed_figure('valu_counters', 'forward_lane_instructions',
-                              args.nfeat, args.side)
-    lane_t = committed_figure('valu_counters', 'tabulated_lane_instructions',
-                              args.nfeat, args.side)
+    traffic_ft = committed_figure('tabulated_passes', 'forward_hbm_bytes',
+                                  args.nfeat, args.side)
+    counted_ft = committed_figure('tabulated_passes', 'forward_counted_fp64_flop',
+                                  args.nfeat, args.side)
+    lane_ft = committed_figure('tabulated_passes', 'forward_lane_instructions',
+                               args.nfeat, args.side)
+    traffic_jt = committed_figure('tabulated_passes', 'jacobian_hbm_bytes',
+                                  args.nfeat, args.side)
+    lane_jt = committed_figure('tabulated_passes', 'jacobian_lane_instructions',
+                               args.nfeat, args.side)
     log('bench: VALU issue-rate probe')
     probe = valu_probe()['rates']
     vop2 = probe['fp32_fma_lane']
@@ -1294,10 +1343,15 @@ def main():
         'fp64_fma_tflops': probe['fp64_fma_lane']['tflops'],
         'fp32_vop2_lane_instructions_per_s': vop2['lane_instructions_per_s']}
     issue = (lane_inst / tj) if lane_inst else None
-    issue_f = (lane_f / tf) if lane_f else None
+    issue_ft = (lane_ft / tft) if lane_ft else None
     fp64_lane = probe['fp64_fma_lane']['lane_instructions_per_s']
     attainable['fp64_lane_instructions_per_s'] = fp64_lane
-    bytes_f = 4. * npx + 8. * lm.batch.ntiles
+    # the tabulated forward reads the data (4 B / pixel), writes one FP64
+    # chi2 partial per tile and its tables (48 B a sample, read back once a
+    # feature's tile run); the tabulated Jacobian reads the data and writes
+    # 36 FP64 partials per tile and its tables (128 B a sample)
+    bytes_ft = 4. * npx + 8. * lm.batch.ntiles + 2 * 48. * tables['forward_samples']
+    bytes_jt = 4. * npx + 36 * 8. * lm.batch.ntiles + 2 * 128. * tables['jacobian_samples']
     fd = None
     fdb = None
     if not args.no_fd:
@@ -1354,32 +1408,47 @@ def main():
                    'features': args.nfeat, 'side': args.side,
                    'mean_nmax': nmean, 'ftol': args.ftol, 'xtol': args.xtol,
                    'gtol': args.gtol, 'schedule': args.schedule},
-        'roofline': {'bound': 'valu', 'kernel': 'k_forward<2>',
+        'roofline': {'bound': 'valu', 'kernels': ['k_fwd_ftab', 'k_forward_ftab'],
+                     'pass': 'trial forward (FP64 residual and chi2) from radial '
+                             'tables of the sums',
                      'dtype': 'f64',
-                     'achieved': ach_f, 'peak': PEAK_FP64, 'unit': 'TFLOP/s',
-                     'frac': ach_f / PEAK_FP64,
-                     'avg_launch_ms': tf * 1e3,
-                     'algorithmic_flop_per_launch': flop_f,
+                     'achieved': ach_ft, 'peak': PEAK_FP64, 'unit': 'TFLOP/s',
+                     'frac': ach_ft / PEAK_FP64,
+                     'avg_launch_ms': tft * 1e3,
+                     'algorithmic_flop_per_launch': flop_ft,
                      'pixels_per_launch': npx,
-                     'traffic': traffic_f,
-                     'algorithmic_bytes_per_launch': bytes_f,
-                     'counted_fp64_flop_per_launch': counted_f,
-                     'counted_achieved': (counted_f / tf / 1e12) if counted_f else None,
+                     'table_samples_per_launch': tables['forward_samples'],
+                     'traffic': traffic_ft,
+                     'algorithmic_bytes_per_launch': bytes_ft,
+                     'counted_fp64_flop_per_launch': counted_ft,
+                     'counted_achieved': (counted_ft / tft / 1e12) if counted_ft else None,
                      'attainable': attainable,
-                     'frac_of_attainable_counted': ((counted_f / tf / 1e12) /
+                     'frac_of_attainable_counted': ((counted_ft / tft / 1e12) /
                                                     attainable['fp64_fma_tflops']
-                                                    if counted_f else None),
-                     'valu_lane_instructions_per_launch': lane_f,
-                     'frac_of_attainable_issue': ((issue_f / fp64_lane)
-                                                  if issue_f else None)},
+                                                    if counted_ft else None),
+                     'valu_lane_instructions_per_launch': lane_ft,
+                     'frac_of_attainable_issue': ((issue_ft / fp64_lane)
+                                                  if issue_ft else None),
+                     'order_loop': {'kernel': 'k_forward<2>',
+                                    'avg_launch_ms': tf * 1e3,
+                                    'achieved': ach_f, 'frac': ach_f / PEAK_FP64,
+                                    'algorithmic_flop_per_launch': flop_f},
+                     'order_loop_over_tabulated': tf / tft},
         'jacobian': {'kernels': ['k_jac_rtab', 'k_jacobian_rtab'],
-                     'dtype': 'f32 (f64 tile sums)',
+                     'pass': 'Jacobian J^T J, J^T r from radial tables of the sums',
+                     'dtype': 'f32 (f64 tile sums)', 'bound': 'valu',
+                     'achieved': ach_jt, 'peak': PEAK_FP32, 'unit': 'TFLOP/s',
+                     'frac': ach_jt / PEAK_FP32,
                      'avg_launch_ms': tt * 1e3,
+                     'algorithmic_flop_per_launch': flop_jt,
                      'pixels_per_launch': npx,
-                     'valu_lane_instructions_per_launch': lane_t,
+                     'table_samples_per_launch': tables['jacobian_samples'],
+                     'traffic': traffic_jt,
+                     'algorithmic_bytes_per_launch': bytes_jt,
+                     'valu_lane_instructions_per_launch': lane_jt,
                      'frac_of_attainable_issue': (
-                         (lane_t / tt / attainable['fp32_vop2_lane_instructions_per_s'])
-                         if lane_t else None),
+                         (lane_jt / tt / attainable['fp32_vop2_lane_instructions_per_s'])
+                         if lane_jt else None),
                      'order_loop': {
                          'kernel': 'k_jacobian', 'bound': 'valu',
                          'avg_launch_ms': tj * 1e3,

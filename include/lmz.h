@@ -270,6 +270,20 @@ int lmz_jacobian_tabulated(const lmz_batch *batch, const double *params,
                            const float *data, double *partials, void *work,
                            hipStream_t stream);
 
+/* The FP64 residual (I - data) / noise and per-tile r^T r as lmz_hologram
+ * with data, the way the batched LM forms them: on dense crops from FP64
+ * radial tables of the multipole sums (one order loop per sample of the
+ * distance from the particle's axis, 12-point interpolation per pixel),
+ * elsewhere by the order loop at every pixel.  Agrees with lmz_hologram to
+ * FP64 rounding.  resid may be NULL.  work: lmz_residual_tabulated_bytes,
+ * 256-byte aligned.  No counterpart in the reference: the trial forward
+ * pass of lmz_lm_run, for callers that evaluate chi2 of dense crops. */
+int lmz_residual_tabulated_bytes(const lmz_batch *batch, size_t *bytes);
+int lmz_residual_tabulated(const lmz_batch *batch, const double *params,
+                           const int32_t *nmax, const float *coef,
+                           int32_t flags, const float *data, float *resid,
+                           double *chi2_tiles, void *work, hipStream_t stream);
+
 /* Vector-Jacobian product of the hologram: grad_params[f][c] = sum over
  * feature f's pixels of grad_out[px] dI/d(param c), for the seven
  * parameters x_p y_p z_p a_p n_p k_p alpha, accumulated in FP64.  J is

@@ -144,6 +144,8 @@ SIGNATURES = {
     'lmz_lm_workspace_bytes': (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     'lmz_jacobian_tabulated_bytes': (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
     'lmz_jacobian_tabulated': (_I, [_P, _P, _P, _P, _P, _I, _P, _P, _P, _P]),
+    'lmz_residual_tabulated_bytes': (_I, [_P, ctypes.POINTER(ctypes.c_size_t)]),
+    'lmz_residual_tabulated': (_I, [_P, _P, _P, _P, _I, _P, _P, _P, _P, _P]),
     'lmz_lm_plan_create': (_I, [_P, ctypes.POINTER(_P)]),
     'lmz_lm_plan_destroy': (_I, [_P]),
     'lmz_lm_run': (_I, [_P, _P, _I, _P, ctypes.POINTER(LMSettings), _P, _P,

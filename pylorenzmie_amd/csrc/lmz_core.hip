@@ -223,6 +223,7 @@ struct TileCursor {
     int k = 0;              // thread 0: chunks found exhausted
 };
 
+template <int CLAIM = LMZ_CLAIM>
 __device__ __forceinline__ int next_tile(int32_t *count, int (&slot)[2],
                                          TileCursor &c) {
     if (c.cur < c.end) return c.cur++;
@@ -238,11 +239,11 @@ __device__ __forceinline__ int next_tile(int32_t *count, int (&slot)[2],
                 ++c.k;
                 continue;
             }
-            const int i = atomicAdd(count + 2 + x, LMZ_CLAIM);
+            const int i = atomicAdd(count + 2 + x, CLAIM);
             if (i < len) {
                 const int base = x * q + (x < r ? x : r);
                 w = base + i;
-                e = base + (i + LMZ_CLAIM < len ? i + LMZ_CLAIM : len);
+                e = base + (i + CLAIM < len ? i + CLAIM : len);
                 break;
             }
             ++c.k;
@@ -502,6 +503,12 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
   __shared__ float4 stab[LMZ_RTAB_LDS * LMZ_RTAB_LDS_STRIDE / 4];
   TileCursor cursor;
   int staged = -1;   // feature whose table stab holds
+  // Each list entry is the first tile of a group of LMZ_RTAB_GROUP
+  // consecutive tiles of one feature (group k: its tiles k G .. k G + G - 1);
+  // the group's partials are summed in registers and reduced once, into the
+  // group's first tile, its other tiles get zeros (the LM sums a feature's
+  // tiles).  The grouping is a function of the tile's place in its feature,
+  // so a feature's partials do not depend on the rest of the batch.
   const int nwork = count[0];
   for (int w = next_tile(count, next_w, cursor); w < nwork;
        w = next_tile(count, next_w, cursor)) {
@@ -528,11 +535,15 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
         staged = f;
     }
     const float *rinf = rin ? rin + b * rin_stride : nullptr;
+    const int ng = min(LMZ_RTAB_GROUP, F.ntiles - (t - fu.tile0));
+    if (threadIdx.x < LMZ_NRED)
+        for (int k = 1; k < ng; ++k)
+            partials[(long long)LMZ_NRED * (t + k) + threadIdx.x] = 0.0;
     float acc[LMZ_NRED];
 #pragma unroll
     for (int i = 0; i < LMZ_NRED; ++i) acc[i] = 0.f;
 #pragma unroll 1
-    for (int q = 0; q < LMZ_TILE / NT; ++q) {
+    for (int q = 0; q < ng * (LMZ_TILE / NT); ++q) {
         const int j = (t - fu.tile0) * LMZ_TILE + q * NT + threadIdx.x;
         if (j >= fu.npix) break;
         float ax, ay, az;
@@ -561,7 +572,7 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
         acc[35] = fmaf(r, r, acc[35]);
     }
     block_reduce_store<LMZ_NRED, NT>(acc, partials + (long long)LMZ_NRED * t);
-    __syncthreads();   // the reduction buffer is reused by the next tile
+    __syncthreads();   // the reduction buffer is reused by the next group
   }
 }
 
@@ -1188,14 +1199,14 @@ void launch_fd_jacobian(const lmz_batch *b, const double *params,
 __global__ __launch_bounds__(1024) void k_tile_starts(
     const lmz_feature *__restrict__ feat, int nfeat,
     const uint8_t *__restrict__ mask, int32_t *__restrict__ start,
-    int32_t *__restrict__ count) {
+    int32_t *__restrict__ count, int group) {
     __shared__ int scan[1024];
     const int tid = threadIdx.x;
     const int per = (nfeat + 1023) / 1024;
     const int f0 = min(nfeat, tid * per), f1 = min(nfeat, f0 + per);
     int mine = 0;
     for (int f = f0; f < f1; ++f)
-        if (mask[f]) mine += feat[f].ntiles;
+        if (mask[f]) mine += (feat[f].ntiles + group - 1) / group;
     scan[tid] = mine;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -1207,7 +1218,7 @@ __global__ __launch_bounds__(1024) void k_tile_starts(
     int pos = scan[tid] - mine;
     for (int f = f0; f < f1; ++f) {
         start[f] = pos;
-        if (mask[f]) pos += feat[f].ntiles;
+        if (mask[f]) pos += (feat[f].ntiles + group - 1) / group;
     }
     if (tid == 1023) count[0] = scan[1023];
     if (tid < LMZ_XCDS) count[2 + tid] = 0;   // fetch counters (next_tile)
@@ -1216,19 +1227,20 @@ __global__ __launch_bounds__(1024) void k_tile_starts(
 __global__ __launch_bounds__(256) void k_tile_fill(
     const lmz_feature *__restrict__ feat, int nfeat,
     const uint8_t *__restrict__ mask, const int32_t *__restrict__ start,
-    int32_t *__restrict__ list) {
+    int32_t *__restrict__ list, int group) {
     const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (f >= nfeat || !mask[f]) return;
-    const int s0 = start[f], t0 = feat[f].tile_offset, nt = feat[f].ntiles;
-    for (int i = threadIdx.x & 63; i < nt; i += 64) list[s0 + i] = t0 + i;
+    const int s0 = start[f], t0 = feat[f].tile_offset;
+    const int ng = (feat[f].ntiles + group - 1) / group;
+    for (int i = threadIdx.x & 63; i < ng; i += 64) list[s0 + i] = t0 + i * group;
 }
 
 void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
-                      const TileList &tl, hipStream_t stream) {
+                      const TileList &tl, hipStream_t stream, int group) {
     hipLaunchKernelGGL(k_tile_starts, dim3(1), dim3(1024), 0, stream, b->d_feat,
-                       b->nfeat, mask, tl.start, tl.count);
+                       b->nfeat, mask, tl.start, tl.count, group);
     hipLaunchKernelGGL(k_tile_fill, dim3((b->nfeat + 3) / 4), dim3(256), 0, stream,
-                       b->d_feat, b->nfeat, mask, tl.start, tl.list);
+                       b->d_feat, b->nfeat, mask, tl.start, tl.list, group);
 }
 
 // Workgroups of one pixel kernel resident per CU at once.
@@ -1728,13 +1740,83 @@ int lmz_jacobian_tabulated(const lmz_batch *b, const double *params,
                         grid_j, start};
     launch_jac_rtab(b, params, nullptr, 0, nmax, coef, dcoef, flags, nullptr,
                     rpar, rtab, mask_t, mask_l, stream);
-    launch_tile_list(b, mask_t, tl_t, stream);
+    launch_tile_list(b, mask_t, tl_t, stream, LMZ_RTAB_GROUP);
     launch_jacobian_rtab(b, params, nullptr, 0, nmax, nullptr, flags, data,
                          nullptr, 0LL, partials, rpar, rtab, &tl_t, stream);
     launch_tile_list(b, mask_l, tl_l, stream);
     launch_jacobian(b, params, nullptr, 0, nmax, coef, dcoef, nullptr, flags,
                     data, nullptr, 0LL, nullptr, nullptr, partials, nullptr,
                     &tl_l, stream);
+    LMZ_HIP(hipGetLastError());
+    return LMZ_OK;
+}
+
+// Workspace of lmz_residual_tabulated: FP64 tables, grids, masks, lists.
+struct FTabLayout {
+    size_t fpar, ftab, mask_t, mask_l, list_t, list_l, count, start, total;
+};
+
+static FTabLayout ftab_layout(const lmz_batch *b) {
+    FTabLayout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 255) & ~size_t(255);
+        return o;
+    };
+    L.fpar = take(sizeof(FTabPar) * (size_t)b->nfeat);
+    L.ftab = take(sizeof(double) * LMZ_FTAB_CAP * LMZ_FTAB_STRIDE * (size_t)b->nfeat);
+    L.mask_t = take((size_t)b->nfeat);
+    L.mask_l = take((size_t)b->nfeat);
+    L.list_t = take(sizeof(int32_t) * (size_t)b->ntiles);
+    L.list_l = take(sizeof(int32_t) * (size_t)b->ntiles);
+    L.count = take(sizeof(int32_t) * 32);
+    L.start = take(sizeof(int32_t) * (size_t)b->nfeat);
+    L.total = off;
+    return L;
+}
+
+int lmz_residual_tabulated_bytes(const lmz_batch *b, size_t *bytes) {
+    LMZ_REQUIRE(bytes != nullptr, "bytes is NULL");
+    if (int rc = check_batch(b)) return rc;
+    *bytes = ftab_layout(b).total;
+    return LMZ_OK;
+}
+
+int lmz_residual_tabulated(const lmz_batch *b, const double *params,
+                           const int32_t *nmax, const float *coef,
+                           int32_t flags, const float *data, float *resid,
+                           double *chi2_tiles, void *work, hipStream_t stream) {
+    if (int rc = forward_args(b, params, nmax, coef, nullptr, nullptr)) return rc;
+    LMZ_REQUIRE(b->all_dense, "lmz_residual_tabulated needs dense crops");
+    LMZ_REQUIRE(flags & LMZ_CARTESIAN, "the residual is of the Cartesian hologram");
+    LMZ_REQUIRE(!(flags & LMZ_FP32), "the tabulated residual is FP64");
+    LMZ_REQUIRE(data && chi2_tiles && work, "data, chi2_tiles and work are required");
+    LMZ_REQUIRE((reinterpret_cast<uintptr_t>(work) & 255) == 0,
+                "work must be 256-byte aligned");
+    int grid_j = 0, grid_f = 0, grid_t = 0, grid_ft = 0;
+    if (int rc = persistent_grids(b, &grid_j, &grid_f)) return rc;
+    if (int rc = rtab_grid(&grid_t, &grid_ft)) return rc;
+    const FTabLayout L = ftab_layout(b);
+    char *ws = static_cast<char *>(work);
+    FTabPar *fpar = reinterpret_cast<FTabPar *>(ws + L.fpar);
+    double *ftab = reinterpret_cast<double *>(ws + L.ftab);
+    uint8_t *mask_t = reinterpret_cast<uint8_t *>(ws + L.mask_t);
+    uint8_t *mask_l = reinterpret_cast<uint8_t *>(ws + L.mask_l);
+    int32_t *count = reinterpret_cast<int32_t *>(ws + L.count);
+    int32_t *start = reinterpret_cast<int32_t *>(ws + L.start);
+    const TileList tl_t{reinterpret_cast<int32_t *>(ws + L.list_t), count,
+                        grid_ft, start};
+    const TileList tl_l{reinterpret_cast<int32_t *>(ws + L.list_l), count + 16,
+                        grid_f, start};
+    launch_fwd_ftab(b, params, nullptr, 0, nmax, coef, flags, nullptr, fpar,
+                    ftab, mask_t, mask_l, stream);
+    launch_tile_list(b, mask_t, tl_t, stream);
+    launch_forward_ftab(b, params, nullptr, 0, nmax, flags, data, resid, 0LL,
+                        chi2_tiles, fpar, ftab, &tl_t, stream);
+    launch_tile_list(b, mask_l, tl_l, stream);
+    launch_residual(b, params, nullptr, 0, nmax, coef, nullptr, flags, data,
+                    resid, 0LL, chi2_tiles, nullptr, &tl_l, stream);
     LMZ_HIP(hipGetLastError());
     return LMZ_OK;
 }

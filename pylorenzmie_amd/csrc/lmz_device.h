@@ -116,8 +116,10 @@ void launch_fd_jacobian(const lmz_batch *b, const double *params,
                         long long rin_stride, double *partials,
                         const uint8_t *active, const TileList *tl,
                         double *work, hipStream_t stream);
+// The list of the masked features' tiles, or of every group-th of them
+// (the first tile of each group of consecutive tiles of a feature).
 void launch_tile_list(const lmz_batch *b, const uint8_t *mask,
-                      const TileList &tl, hipStream_t stream);
+                      const TileList &tl, hipStream_t stream, int group = 1);
 // Persistent grids of the LM's pixel launches under the batch's schedule.
 int persistent_grids(const lmz_batch *b, int *grid_jacobian, int *grid_forward);
 int rtab_grid(int *grid, int *grid_fwd);

@@ -382,7 +382,7 @@ static void enqueue_iteration(const lmz_lm_plan *plan, const LMLayout &L,
                             count + 48, plan->grid_j, tile_start};
         launch_jac_rtab(b, params, sel, 0, nmax, coef, dcoef, flags, mask_j,
                         rpar, rtab, mask_jt, mask_jl, s);
-        launch_tile_list(b, mask_jt, tl_t, s);
+        launch_tile_list(b, mask_jt, tl_t, s, LMZ_RTAB_GROUP);
         launch_jacobian_rtab(b, params, sel, 0, nmax, coords, flags, data,
                              resid, np, partials, rpar, rtab, &tl_t, s);
         launch_tile_list(b, mask_jl, tl_l, s);

@@ -771,6 +771,7 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
 #define LMZ_RTAB_P 6         // interpolation points
 #define LMZ_RTAB_HW 0.35f    // h N / |Z|
 #define LMZ_RTAB_LDS 256      // rows of a table the Jacobian kernel keeps in LDS
+#define LMZ_RTAB_GROUP 4      // tiles of a feature per work item of that kernel
 #define LMZ_RTAB_LDS_STRIDE 36   // floats per LDS row: rows 4 banks apart
 
 struct RTabPar {

@@ -550,17 +550,15 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
         pixel_coords(fu, j, coords, npix_total, ax, ay, az);
         PixGeom g;
         pixel_geometry<true>(fu, ax, ay, az, bohren, g);
-        JacPixel jp;
+        JacPixel jp;   // J scaled by 1 / noise
         if (lds)
             jacobian_pixel_rtab<LMZ_RTAB_LDS_STRIDE>(
-                fu, g, rp, reinterpret_cast<const float *>(stab), jp);
+                fu, g, rp, reinterpret_cast<const float *>(stab), fu.inv_noise, jp);
         else
-            jacobian_pixel_rtab<LMZ_RTAB_STRIDE>(fu, g, rp, gtab, jp);
+            jacobian_pixel_rtab<LMZ_RTAB_STRIDE>(fu, g, rp, gtab, fu.inv_noise, jp);
         const long long gi = fu.pix_offset + j;
         const float r = rinf ? rinf[gi] : (jp.I - data[gi]) * fu.inv_noise;
-        float Jn[LMZ_NJAC];
-#pragma unroll
-        for (int c = 0; c < LMZ_NJAC; ++c) Jn[c] = jp.J[c] * fu.inv_noise;
+        const float *Jn = jp.J;
         int idx = 0;
 #pragma unroll
         for (int a = 0; a < LMZ_NJAC; ++a)

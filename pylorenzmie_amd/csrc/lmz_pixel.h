@@ -662,8 +662,14 @@ LMZ_HD void jacobian_loop(float w, float cl, float invRl, float invR,
 
 // The intensity and its seven derivatives of one pixel from the sums of
 // the order loop (L's sums; its loop state is not read).
+// ROT: L's sums are e^{-i psi} times the pixel's (the radial tables'), with
+// e^{i psi} = ph: the field is turned once, and the adjoint weights, formed
+// from conj(ph) F, meet the unturned sums (Re[conj(a) ph b] =
+// Re[conj(conj(ph) a) b]).  jscale multiplies every derivative.
+template <bool ROT = false>
 LMZ_HD void jacobian_project(const FeatUniform &fu, const PixGeom &g,
-                             const JacLoop &L, JacPixel &out) {
+                             const JacLoop &L, cf ph, float jscale,
+                             JacPixel &out) {
     const float invR2 = g.invR * g.invR;
     // ---- projection by adjoint weights.  The field is E = M S, S = (S_r,
     // S_t, S_p), with the real 3 x 3 geometry matrix of field_from_sums:
@@ -692,10 +698,20 @@ LMZ_HD void jacobian_project(const FeatUniform &fu, const PixGeom &g,
     out.E[0] = u2 * A - v2 * B;
     out.E[1] = uv * ApB;
     out.E[2] = us * C;
+    if constexpr (ROT) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) out.E[i] = cmul(ph, out.E[i]);
+    }
     const float al = fu.alpha;
-    const cf F0{fmaf(al, out.E[0].re, 1.f), al * out.E[0].im};
-    const cf F1 = al * out.E[1], F2 = al * out.E[2];
+    cf F0{fmaf(al, out.E[0].re, 1.f), al * out.E[0].im};
+    cf F1 = al * out.E[1], F2 = al * out.E[2];
     out.I = credot(F0, F0) + credot(F1, F1) + credot(F2, F2);
+    if constexpr (ROT) {
+        const cf pc{ph.re, -ph.im};
+        F0 = cmul(pc, F0);
+        F1 = cmul(pc, F1);
+        F2 = cmul(pc, F2);
+    }
 
     const cf W = u2 * F0 + uv * F1, V = uv * F1 - v2 * F0, U = us * F2;
     const cf gr = a_r * W + c_r * U, gt = a_t * W - invR * U, gp = invR * V;
@@ -725,7 +741,7 @@ LMZ_HD void jacobian_project(const FeatUniform &fu, const PixGeom &g,
     const float DX = s * u * GR - csR * u * Gc + PX;
     const float DY = s * v * GR - csR * v * Gc + PY;
     const float DZ = c * GR + s2 * invR * Gc + PZ;
-    const float two_al = 2.f * al, k2a = fu.k * two_al;
+    const float two_al = 2.f * al * jscale, k2a = fu.k * two_al;
     out.J[0] = -k2a * DX;
     out.J[1] = -k2a * DY;
     out.J[2] = k2a * (out.E[0].im + DZ);
@@ -739,7 +755,7 @@ LMZ_HD void jacobian_project(const FeatUniform &fu, const PixGeom &g,
             iTp{-L.Tp[1].im, L.Tp[1].re};
         out.J[5] = two_al * (credot(gr, iTr) + credot(gt, iTt) + credot(gpm, iTp));
     }
-    out.J[6] = 2.f * (rr + rt + rp);
+    out.J[6] = (2.f * jscale) * (rr + rt + rp);
 }
 
 LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
@@ -748,7 +764,7 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
     JacLoop L;
     jacobian_loop(g.w, g.cl, g.invRl, g.invR, g.xi_m1, g.xi_0, coef, dcoef,
                   nmax, L);
-    jacobian_project(fu, g, L, out);
+    jacobian_project(fu, g, L, cf{1.f, 0.f}, 1.f, out);
 }
 
 // ====================================================================
@@ -841,13 +857,14 @@ LMZ_HD void rtab_sample(const FeatUniform &fu, const RTabPar &p, bool bohren,
 }
 
 // The Jacobian of one pixel of a tabulated feature: its sums interpolated
-// in k rho and turned by e^{i psi} (g.xi_m1), then projected as the loop's.
-// RS: floats from one table row to the next (LMZ_RTAB_STRIDE in global
-// memory, LMZ_RTAB_LDS_STRIDE in LDS).
+// in k rho, then projected as the loop's with the turn by e^{i psi}
+// (g.xi_m1); every derivative times jscale.  RS: floats from one table row
+// to the next (LMZ_RTAB_STRIDE in global memory, LMZ_RTAB_LDS_STRIDE in
+// LDS).
 template <int RS = LMZ_RTAB_STRIDE>
 LMZ_HD void jacobian_pixel_rtab(const FeatUniform &fu, const PixGeom &g,
                                 const RTabPar &p, const float *tab,
-                                JacPixel &out) {
+                                float jscale, JacPixel &out) {
     const float t = (g.rho - p.rho0) * p.inv_h;
     int i0 = (int)floorf(t) - (LMZ_RTAB_P / 2 - 1);
     i0 = i0 < 0 ? 0 : (i0 > p.m - LMZ_RTAB_P ? p.m - LMZ_RTAB_P : i0);
@@ -878,15 +895,13 @@ LMZ_HD void jacobian_pixel_rtab(const FeatUniform &fu, const PixGeom &g,
 #pragma unroll
         for (int c = 0; c < 30; ++c) acc[c] = fmaf(wgt[k], rk[c], acc[c]);
     }
-    const cf ph = g.xi_m1;   // e^{i psi}
     JacLoop L;
     cf *dst[15] = {&L.Sr, &L.St, &L.Sp, &L.mSrR, &L.mStR, &L.mSpR, &L.Src,
                    &L.Stc, &L.Spc, &L.Tr[0], &L.Tr[1], &L.Tt[0], &L.Tt[1],
                    &L.Tp[0], &L.Tp[1]};
 #pragma unroll
-    for (int i = 0; i < 15; ++i)
-        *dst[i] = cmul(ph, cf{acc[2 * i], acc[2 * i + 1]});
-    jacobian_project(fu, g, L, out);
+    for (int i = 0; i < 15; ++i) *dst[i] = cf{acc[2 * i], acc[2 * i + 1]};
+    jacobian_project<true>(fu, g, L, g.xi_m1, jscale, out);
 }
 
 // ====================================================================

@@ -104,7 +104,7 @@ int chk_jac_rtab(const lmz_feature *feat, const double *params,
         pixel_geometry<true>(fu, ax, ay, az, bohren, g);
         JacPixel a, b;
         jacobian_pixel(fu, g, coef, dcoef, nmax, a);
-        if (p.m > 0) jacobian_pixel_rtab(fu, g, p, tab.data(), b);
+        if (p.m > 0) jacobian_pixel_rtab(fu, g, p, tab.data(), 1.f, b);
         for (int c = 0; c < LMZ_NJAC; ++c) {
             jac_loop[c * fu.npix + j] = a.J[c];
             jac_tab[c * fu.npix + j] = p.m > 0 ? b.J[c] : 0.f;

@@ -502,7 +502,7 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
   __shared__ int next_w[2];
   __shared__ float4 stab[LMZ_RTAB_LDS * LMZ_RTAB_LDS_STRIDE / 4];
   TileCursor cursor;
-  int staged = -1;   // feature whose table stab holds
+  int staged = -1, sb = 0, se = 0;   // stab holds rows sb .. se - 1 of staged's table
   // Each list entry is the first tile of a group of LMZ_RTAB_GROUP
   // consecutive tiles of one feature (group k: its tiles k G .. k G + G - 1);
   // the group's partials are summed in registers and reduced once, into the
@@ -523,19 +523,32 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
     const bool bohren = flags & LMZ_BOHREN;
     const RTabPar rp = rpar[f];
     const float *gtab = rtab + (long long)f * (LMZ_RTAB_CAP * LMZ_RTAB_STRIDE);
-    const bool lds = rp.m <= LMZ_RTAB_LDS;
-    if (lds && staged != f) {
-        // every thread has finished the previous tile (the barriers of the
-        // reduction and of next_tile)
-        const float4 *src = reinterpret_cast<const float4 *>(gtab);
+    const int ng = min(LMZ_RTAB_GROUP, F.ntiles - (t - fu.tile0));
+    // the rows the group reads: the whole table, or, for a table larger
+    // than LDS, the window of the group's pixels
+    int lo = 0, hi = rp.m - 1;
+    if (rp.m > LMZ_RTAB_LDS) {
+        const int jp0 = (t - fu.tile0) * LMZ_TILE;
+        table_rows<LMZ_RTAB_P, float>(fu.width, jp0,
+                                      min(jp0 + ng * LMZ_TILE, fu.npix), fu.k,
+                                      fu.ix + fu.fx, fu.iy + fu.fy, rp.rho0,
+                                      rp.inv_h, rp.m, lo, hi);
+    }
+    const bool lds = hi - lo < LMZ_RTAB_LDS;
+    if (lds && !(staged == f && lo >= sb && hi < se)) {
+        // every thread has finished the previous group (the barriers of
+        // the reduction and of next_tile)
+        sb = rp.m > LMZ_RTAB_LDS ? table_window(lo, hi, rp.m, LMZ_RTAB_LDS) : 0;
+        se = min(sb + LMZ_RTAB_LDS, rp.m);
+        const float4 *src = reinterpret_cast<const float4 *>(
+            gtab + (long long)sb * LMZ_RTAB_STRIDE);
         constexpr int G = LMZ_RTAB_STRIDE / 4, S = LMZ_RTAB_LDS_STRIDE / 4;
-        for (int i = threadIdx.x; i < rp.m * G; i += NT)
+        for (int i = threadIdx.x; i < (se - sb) * G; i += NT)
             stab[(i / G) * S + i % G] = src[i];
         __syncthreads();
         staged = f;
     }
     const float *rinf = rin ? rin + b * rin_stride : nullptr;
-    const int ng = min(LMZ_RTAB_GROUP, F.ntiles - (t - fu.tile0));
     if (threadIdx.x < LMZ_NRED)
         for (int k = 1; k < ng; ++k)
             partials[(long long)LMZ_NRED * (t + k) + threadIdx.x] = 0.0;
@@ -553,7 +566,8 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
         JacPixel jp;   // J scaled by 1 / noise
         if (lds)
             jacobian_pixel_rtab<LMZ_RTAB_LDS_STRIDE>(
-                fu, g, rp, reinterpret_cast<const float *>(stab), fu.inv_noise, jp);
+                fu, g, rp, reinterpret_cast<const float *>(stab), fu.inv_noise,
+                jp, sb);
         else
             jacobian_pixel_rtab<LMZ_RTAB_STRIDE>(fu, g, rp, gtab, fu.inv_noise, jp);
         const long long gi = fu.pix_offset + j;
@@ -630,7 +644,7 @@ __global__ __launch_bounds__(NT) void k_forward_ftab(
   __shared__ int next_w[2];
   __shared__ double stab[LMZ_FTAB_LDS * LMZ_FTAB_STRIDE];
   TileCursor cursor;
-  int staged = -1;   // feature whose table stab holds
+  int staged = -1, sb = 0, se = 0;   // stab holds rows sb .. se - 1 of staged's table
   const int nwork = count[0];
   for (int w = next_tile(count, next_w, cursor); w < nwork;
        w = next_tile(count, next_w, cursor)) {
@@ -645,13 +659,27 @@ __global__ __launch_bounds__(NT) void k_forward_ftab(
     const bool bohren = flags & LMZ_BOHREN;
     const FTabPar fp = fpar[f];
     const double *gtab = ftab + (long long)f * (LMZ_FTAB_CAP * LMZ_FTAB_STRIDE);
-    const bool lds = fp.m <= LMZ_FTAB_LDS;
-    if (lds && staged != f) {
+    // the rows the tile reads: the whole table, or, for a table larger
+    // than LDS, the window of the tile's pixels
+    int lo = 0, hi = fp.m - 1;
+    if (fp.m > LMZ_FTAB_LDS) {
+        const int jp0 = (t - fu.tile0) * LMZ_TILE;
+        table_rows<LMZ_FTAB_P, double>(fu.width, jp0,
+                                       min(jp0 + LMZ_TILE, fu.npix), fu.kd,
+                                       (double)fu.ix + fu.fxd,
+                                       (double)fu.iy + fu.fyd, fp.rho0,
+                                       fp.inv_h, fp.m, lo, hi);
+    }
+    const bool lds = hi - lo < LMZ_FTAB_LDS;
+    if (lds && !(staged == f && lo >= sb && hi < se)) {
         // every thread has finished the previous tile (the barriers of the
         // reduction and of next_tile)
-        const double2 *src = reinterpret_cast<const double2 *>(gtab);
+        sb = fp.m > LMZ_FTAB_LDS ? table_window(lo, hi, fp.m, LMZ_FTAB_LDS) : 0;
+        se = min(sb + LMZ_FTAB_LDS, fp.m);
+        const double2 *src = reinterpret_cast<const double2 *>(
+            gtab + (long long)sb * LMZ_FTAB_STRIDE);
         double2 *dst = reinterpret_cast<double2 *>(stab);
-        for (int i = threadIdx.x; i < fp.m * (LMZ_FTAB_STRIDE / 2); i += NT)
+        for (int i = threadIdx.x; i < (se - sb) * (LMZ_FTAB_STRIDE / 2); i += NT)
             dst[i] = src[i];
         __syncthreads();
         staged = f;
@@ -667,7 +695,7 @@ __global__ __launch_bounds__(NT) void k_forward_ftab(
         const long long gi = fu.pix_offset + j;
         PixGeomD g;
         pixel_geometry_d(fu, ax, ay, az, bohren, g);
-        const SumsD S = lds ? forward_sum_ftab<LMZ_FTAB_STRIDE>(g, fp, stab)
+        const SumsD S = lds ? forward_sum_ftab<LMZ_FTAB_STRIDE>(g, fp, stab, sb)
                             : forward_sum_ftab<LMZ_FTAB_STRIDE>(g, fp, gtab);
         cd E[3];
         field_from_sums(g, S, true, E);

@@ -786,7 +786,7 @@ LMZ_HD void jacobian_pixel(const FeatUniform &fu, const PixGeom &g,
 #define LMZ_RTAB_STRIDE 32   // floats per sample: 15 complex sums, 2 spare
 #define LMZ_RTAB_P 6         // interpolation points
 #define LMZ_RTAB_HW 0.35f    // h N / |Z|
-#define LMZ_RTAB_LDS 256      // rows of a table the Jacobian kernel keeps in LDS
+#define LMZ_RTAB_LDS 352      // rows of a table the Jacobian kernel keeps in LDS
 #define LMZ_RTAB_GROUP 4      // tiles of a feature per work item of that kernel
 #define LMZ_RTAB_LDS_STRIDE 36   // floats per LDS row: rows 4 banks apart
 
@@ -864,7 +864,7 @@ LMZ_HD void rtab_sample(const FeatUniform &fu, const RTabPar &p, bool bohren,
 template <int RS = LMZ_RTAB_STRIDE>
 LMZ_HD void jacobian_pixel_rtab(const FeatUniform &fu, const PixGeom &g,
                                 const RTabPar &p, const float *tab,
-                                float jscale, JacPixel &out) {
+                                float jscale, JacPixel &out, int base = 0) {
     const float t = (g.rho - p.rho0) * p.inv_h;
     int i0 = (int)floorf(t) - (LMZ_RTAB_P / 2 - 1);
     i0 = i0 < 0 ? 0 : (i0 > p.m - LMZ_RTAB_P ? p.m - LMZ_RTAB_P : i0);
@@ -886,7 +886,7 @@ LMZ_HD void jacobian_pixel_rtab(const FeatUniform &fu, const PixGeom &g,
 #pragma unroll
     for (int k = 0; k < LMZ_RTAB_P; ++k) wgt[k] = pre[k] * suf[k] * den[k];
     float acc[30];
-    const float *row = tab + i0 * RS;
+    const float *row = tab + (i0 - base) * RS;   // tab holds rows base ..
 #pragma unroll
     for (int c = 0; c < 30; ++c) acc[c] = wgt[0] * row[c];
 #pragma unroll
@@ -924,6 +924,39 @@ struct FTabPar {
     double rho0, h, inv_h;   // grid k rho_j = rho0 + j h
     int m;                   // samples; 0: the feature uses the order loop
 };
+
+// The rows lo .. hi of a radial table (grid k rho_i = rho0 + i / inv_h,
+// m rows, P-point stencils placed as jacobian_pixel_rtab and
+// forward_sum_ftab place them) that pixels j0 .. j1 - 1 of a dense crop
+// read: the stencils of the nearest and the farthest point of the pixels'
+// bounding rectangle from the particle's axis (px, py) (k: wavenumber per
+// pixel), two rows wider on either side for the rounding of each pixel's
+// own k rho.
+template <int P, typename T>
+LMZ_HD void table_rows(int width, int j0, int j1, T k, T px, T py, T rho0,
+                       T inv_h, int m, int &lo, int &hi) {
+    const int r0 = j0 / width, r1 = (j1 - 1) / width;
+    const T c0 = (T)(r0 == r1 ? j0 - r0 * width : 0);
+    const T c1 = (T)(r0 == r1 ? j1 - 1 - r1 * width : width - 1);
+    const T y0 = (T)r0, y1 = (T)r1;
+    const T nx = (px < c0 ? c0 : (px > c1 ? c1 : px)) - px;
+    const T ny = (py < y0 ? y0 : (py > y1 ? y1 : py)) - py;
+    const T fx = (px - c0 > c1 - px) ? px - c0 : c1 - px;
+    const T fy = (py - y0 > y1 - py) ? py - y0 : y1 - py;
+    const T tmin = (k * sqrt(nx * nx + ny * ny) - rho0) * inv_h;
+    const T tmax = (k * sqrt(fx * fx + fy * fy) - rho0) * inv_h;
+    lo = (int)floor(tmin) - (P / 2 - 1) - 2;
+    hi = (int)floor(tmax) - (P / 2 - 1) + (P - 1) + 2;
+    lo = lo < 0 ? 0 : (lo > m - P ? m - P : lo);
+    hi = hi < P - 1 ? P - 1 : (hi > m - 1 ? m - 1 : hi);
+}
+
+// The first row of an LDS window of cap rows (cap < m) that holds rows
+// lo .. hi (hi - lo < cap), centred on them where the table allows.
+LMZ_HD int table_window(int lo, int hi, int m, int cap) {
+    int sb = lo - (cap - (hi - lo + 1)) / 2;
+    return sb < 0 ? 0 : (sb > m - cap ? m - cap : sb);
+}
 
 // The grid of a feature at its current parameters (as rtab_params, in
 // FP64); m = 0 for explicit coordinates, Z = 0, or more than LMZ_FTAB_CAP
@@ -979,7 +1012,7 @@ LMZ_HD void ftab_sample(const FeatUniform &fu, const FTabPar &p, bool bohren,
 // k rho and turned by e^{i psi} (g.xi_m1).  RS: doubles per table row.
 template <int RS = LMZ_FTAB_STRIDE>
 LMZ_HD SumsD forward_sum_ftab(const PixGeomD &g, const FTabPar &p,
-                              const double *tab) {
+                              const double *tab, int base = 0) {
     constexpr int P = LMZ_FTAB_P;
     const double t = (g.rho - p.rho0) * p.inv_h;
     int i0 = (int)floor(t) - (P / 2 - 1);
@@ -999,7 +1032,7 @@ LMZ_HD SumsD forward_sum_ftab(const PixGeomD &g, const FTabPar &p,
 #pragma unroll
     for (int k = P - 2; k >= 0; --k) suf[k] = suf[k + 1] * (d - (double)(k + 1));
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const double *row = tab + i0 * RS;
+    const double *row = tab + (i0 - base) * RS;   // tab holds rows base ..
 #pragma unroll
     for (int k = 0; k < P; ++k) {
         const double wk = pre[k] * suf[k] * inv_den[k];

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
+#include <limits>
 #include <vector>
 
 #include "../../pylorenzmie_amd/csrc/lmz_de.h"
@@ -84,30 +85,70 @@ void chk_pixels(const lmz_feature *feat, const double *params,
     }
 }
 
+}  // extern "C"
+
+// The table rows a group of pixels j0 .. j1 - 1 reads as the persistent
+// kernels give them: a table of more than cap rows (cap > 0) is read
+// through a window of cap rows (table_rows, table_window) where the group's
+// rows fit in one.  Returns the window's first row (the base the
+// interpolation takes) and the table to read: a copy holding the window's
+// rows and NaN in every other, so a read outside the window shows; win
+// counts the groups read through a window.
+template <int P, int RS, typename T, typename Par>
+const T *chk_window(const FeatUniform &fu, const Par &p, T k, T px, T py,
+                    int j0, int j1, int cap, const std::vector<T> &tab,
+                    std::vector<T> &buf, int &base, int &win) {
+    base = 0;
+    if (p.m <= 0 || cap <= 0 || p.m <= cap) return tab.data();
+    int lo, hi;
+    table_rows<P, T>(fu.width, j0, j1, k, px, py, p.rho0, p.inv_h, p.m, lo, hi);
+    if (hi - lo >= cap) return tab.data();
+    const int sb = table_window(lo, hi, p.m, cap);
+    const int se = std::min(sb + cap, p.m);
+    buf.assign(tab.size(), std::numeric_limits<T>::quiet_NaN());
+    std::copy(tab.begin() + (size_t)sb * RS, tab.begin() + (size_t)se * RS,
+              buf.begin() + (size_t)sb * RS);
+    base = sb;
+    ++win;
+    return buf.data() + (size_t)sb * RS;
+}
+
+extern "C" {
+
 // Jacobian of every pixel of a dense crop (feat->width > 0) by the order
 // loop (jac_loop) and by the radial table (jac_tab, when the feature is
-// tabulated); returns the table's sample count (0: not tabulated) and its
-// spacing in k rho.
+// tabulated), the table read as chk_window gives it to groups of `group`
+// pixels; returns the table's sample count (0: not tabulated), its spacing
+// in k rho and the groups read through a window.
 int chk_jac_rtab(const lmz_feature *feat, const double *params,
                  const float *coef, const float *dcoef, int nmax, int flags,
-                 float *jac_loop, float *jac_tab, float *h_out) {
+                 float *jac_loop, float *jac_tab, float *h_out, int cap,
+                 int group, int *nwin) {
     const FeatUniform fu = feature_uniform(*feat, params, nmax, 0);
     const bool bohren = flags & LMZ_BOHREN;
     const RTabPar p = rtab_params(fu);
-    std::vector<float> tab((size_t)LMZ_RTAB_STRIDE * (p.m > 0 ? p.m : 1));
+    std::vector<float> tab((size_t)LMZ_RTAB_STRIDE * (p.m > 0 ? p.m : 1)), buf;
     for (int j = 0; j < p.m; ++j)
         rtab_sample(fu, p, bohren, j, coef, dcoef, &tab[(size_t)j * LMZ_RTAB_STRIDE]);
-    for (int j = 0; j < fu.npix; ++j) {
-        float ax, ay, az;
-        pixel_coords(fu, j, nullptr, fu.npix, ax, ay, az);
-        PixGeom g;
-        pixel_geometry<true>(fu, ax, ay, az, bohren, g);
-        JacPixel a, b;
-        jacobian_pixel(fu, g, coef, dcoef, nmax, a);
-        if (p.m > 0) jacobian_pixel_rtab(fu, g, p, tab.data(), 1.f, b);
-        for (int c = 0; c < LMZ_NJAC; ++c) {
-            jac_loop[c * fu.npix + j] = a.J[c];
-            jac_tab[c * fu.npix + j] = p.m > 0 ? b.J[c] : 0.f;
+    *nwin = 0;
+    for (int j0 = 0; j0 < fu.npix; j0 += group) {
+        const int j1 = std::min(j0 + group, fu.npix);
+        int base;
+        const float *src = chk_window<LMZ_RTAB_P, LMZ_RTAB_STRIDE, float>(
+            fu, p, fu.k, fu.ix + fu.fx, fu.iy + fu.fy, j0, j1, cap, tab, buf,
+            base, *nwin);
+        for (int j = j0; j < j1; ++j) {
+            float ax, ay, az;
+            pixel_coords(fu, j, nullptr, fu.npix, ax, ay, az);
+            PixGeom g;
+            pixel_geometry<true>(fu, ax, ay, az, bohren, g);
+            JacPixel a, b;
+            jacobian_pixel(fu, g, coef, dcoef, nmax, a);
+            if (p.m > 0) jacobian_pixel_rtab(fu, g, p, src, 1.f, b, base);
+            for (int c = 0; c < LMZ_NJAC; ++c) {
+                jac_loop[c * fu.npix + j] = a.J[c];
+                jac_tab[c * fu.npix + j] = p.m > 0 ? b.J[c] : 0.f;
+            }
         }
     }
     *h_out = p.h;
@@ -115,30 +156,41 @@ int chk_jac_rtab(const lmz_feature *feat, const double *params,
 }
 
 // FP64 hologram of every pixel of a dense crop by the order loop (holo_loop)
-// and by the FP64 radial table (holo_tab, when tabulated); returns the
-// table's sample count (0: not tabulated).
+// and by the FP64 radial table (holo_tab, when tabulated), the table read
+// as chk_window gives it to groups of `group` pixels; returns the table's
+// sample count (0: not tabulated), its spacing and the groups read through
+// a window.
 int chk_fwd_ftab(const lmz_feature *feat, const double *params,
                  const float *coef, int nmax, int flags, double *holo_loop,
-                 double *holo_tab, double *h_out) {
+                 double *holo_tab, double *h_out, int cap, int group,
+                 int *nwin) {
     const FeatUniform fu = feature_uniform(*feat, params, nmax, 0);
     const bool bohren = flags & LMZ_BOHREN;
     const FTabPar p = ftab_params(fu);
-    std::vector<double> tab((size_t)LMZ_FTAB_STRIDE * (p.m > 0 ? p.m : 1));
+    std::vector<double> tab((size_t)LMZ_FTAB_STRIDE * (p.m > 0 ? p.m : 1)), buf;
     for (int j = 0; j < p.m; ++j)
         ftab_sample(fu, p, bohren, j, coef, &tab[(size_t)j * LMZ_FTAB_STRIDE]);
-    for (int j = 0; j < fu.npix; ++j) {
-        float ax, ay, az;
-        pixel_coords(fu, j, nullptr, fu.npix, ax, ay, az);
-        PixGeomD g;
-        pixel_geometry_d(fu, ax, ay, az, bohren, g);
-        cd E[3];
-        field_from_sums(g, forward_sum(g, coef, nmax), true, E);
-        holo_loop[j] = intensity(E, fu.alphad);
-        if (p.m > 0) {
-            field_from_sums(g, forward_sum_ftab(g, p, tab.data()), true, E);
-            holo_tab[j] = intensity(E, fu.alphad);
-        } else {
-            holo_tab[j] = 0.0;
+    *nwin = 0;
+    for (int j0 = 0; j0 < fu.npix; j0 += group) {
+        const int j1 = std::min(j0 + group, fu.npix);
+        int base;
+        const double *src = chk_window<LMZ_FTAB_P, LMZ_FTAB_STRIDE, double>(
+            fu, p, fu.kd, (double)fu.ix + fu.fxd, (double)fu.iy + fu.fyd, j0,
+            j1, cap, tab, buf, base, *nwin);
+        for (int j = j0; j < j1; ++j) {
+            float ax, ay, az;
+            pixel_coords(fu, j, nullptr, fu.npix, ax, ay, az);
+            PixGeomD g;
+            pixel_geometry_d(fu, ax, ay, az, bohren, g);
+            cd E[3];
+            field_from_sums(g, forward_sum(g, coef, nmax), true, E);
+            holo_loop[j] = intensity(E, fu.alphad);
+            if (p.m > 0) {
+                field_from_sums(g, forward_sum_ftab(g, p, src, base), true, E);
+                holo_tab[j] = intensity(E, fu.alphad);
+            } else {
+                holo_tab[j] = 0.0;
+            }
         }
     }
     *h_out = p.h;

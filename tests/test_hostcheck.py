@@ -213,11 +213,15 @@ def test_radial_table_jacobian_matches_oracle(chk, q):
     """Dense crop (41 x 41): the Jacobian from the radial tables of the sums
     and from the order loop at every pixel, each within 2e-4 of the column
     scale of the oracle's 6th-order differences (the last case's axis lies
-    outside the crop)."""
+    outside the crop).  Read through windows of half the table by groups of
+    four crop rows and of one (as the kernel reads a table larger than its
+    LDS), the table gives the same bits: every row a pixel reads lies in its
+    group's window."""
     lib, Feature = chk
     P = ctypes.c_void_p
+    I = ctypes.c_int
     lib.chk_jac_rtab.restype = ctypes.c_int
-    lib.chk_jac_rtab.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, P, P, P]
+    lib.chk_jac_rtab.argtypes = [P, P, P, P, I, I, P, P, P, I, I, P]
     side = 41
     N, coef, dcoef, _ = mie(chk, q[3], q[4], q[5])
     F = Feature()
@@ -229,9 +233,19 @@ def test_radial_table_jacobian_matches_oracle(chk, q):
     jl = np.zeros((7, side * side), np.float32)
     jt = np.zeros_like(jl)
     h = ctypes.c_float()
+    nwin = ctypes.c_int()
     m = lib.chk_jac_rtab(ctypes.byref(F), _p(p), _p(coef), _p(dcoef), N, 3,
-                         _p(jl), _p(jt), ctypes.byref(h))
+                         _p(jl), _p(jt), ctypes.byref(h), 0, 1024,
+                         ctypes.byref(nwin))
     assert 12 <= m < 1024 and 0. < h.value <= 1.
+    for rows in (4, 1):
+        jw = np.zeros_like(jl)
+        lib.chk_jac_rtab(ctypes.byref(F), _p(p), _p(coef), _p(dcoef), N, 3,
+                         _p(jl), _p(jw), ctypes.byref(h), m // 2, rows * side,
+                         ctypes.byref(nwin))
+        print('m', m, 'rows', rows, 'windowed groups', nwin.value)
+        assert nwin.value > 0
+        assert np.array_equal(jw, jt)
     Jref = ref.jacobian(grid3((side, side)), dict(zip(ref.PARAMS, q)),
                         INSTRUMENT, order=6)
     for i in range(7):
@@ -250,11 +264,15 @@ def test_forward_radial_table_matches_order_loop(chk, q):
     sums equals the order loop's at every pixel to 2e-13 of the brightest
     pixel, and is no farther from the oracle's (numpy, no FMA) than the
     loop's is (FP64 rounding of the two sums, up to 4e-12 on the bright
-    lobes of 3 um spheres).  The last case's axis lies outside the crop."""
+    lobes of 3 um spheres).  The last case's axis lies outside the crop.
+    Read through windows of half the table by 256-pixel tiles (as the
+    kernel reads a table larger than its LDS), the table gives the same
+    bits."""
     lib, Feature = chk
     P = ctypes.c_void_p
+    I = ctypes.c_int
     lib.chk_fwd_ftab.restype = ctypes.c_int
-    lib.chk_fwd_ftab.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int, P, P, P]
+    lib.chk_fwd_ftab.argtypes = [P, P, P, I, I, P, P, P, I, I, P]
     side = 201
     N, coef, _, _ = mie(chk, q[3], q[4], q[5])
     F = Feature()
@@ -266,9 +284,16 @@ def test_forward_radial_table_matches_order_loop(chk, q):
     hl = np.zeros(side * side)
     ht = np.zeros(side * side)
     h = ctypes.c_double()
+    nwin = ctypes.c_int()
     m = lib.chk_fwd_ftab(ctypes.byref(F), _p(p), _p(coef), N, 3, _p(hl), _p(ht),
-                         ctypes.byref(h))
+                         ctypes.byref(h), 0, 256, ctypes.byref(nwin))
     assert 2 * 12 <= m < 2048 and 0. < h.value <= 1.
+    hw = np.zeros_like(ht)
+    lib.chk_fwd_ftab(ctypes.byref(F), _p(p), _p(coef), N, 3, _p(hl), _p(hw),
+                     ctypes.byref(h), m // 2, 256, ctypes.byref(nwin))
+    print('windowed tiles', nwin.value)
+    assert nwin.value > 0
+    assert np.array_equal(hw, ht)
     print('m', m, 'h', h.value, 'max |tab - loop|', np.abs(ht - hl).max())
     assert np.abs(ht - hl).max() < 2e-13 * hl.max()
     c = grid3((side, side))

@@ -549,9 +549,6 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
         staged = f;
     }
     const float *rinf = rin ? rin + b * rin_stride : nullptr;
-    if (threadIdx.x < LMZ_NRED)
-        for (int k = 1; k < ng; ++k)
-            partials[(long long)LMZ_NRED * (t + k) + threadIdx.x] = 0.0;
     float acc[LMZ_NRED];
 #pragma unroll
     for (int i = 0; i < LMZ_NRED; ++i) acc[i] = 0.f;
@@ -584,6 +581,10 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
         acc[35] = fmaf(r, r, acc[35]);
     }
     block_reduce_store<LMZ_NRED, NT>(acc, partials + (long long)LMZ_NRED * t);
+    // after the group's loads (a load issued after a store waits for it)
+    if (threadIdx.x < LMZ_NRED)
+        for (int k = 1; k < ng; ++k)
+            partials[(long long)LMZ_NRED * (t + k) + threadIdx.x] = 0.0;
     __syncthreads();   // the reduction buffer is reused by the next group
   }
 }
@@ -627,9 +628,11 @@ __global__ __launch_bounds__(64) void k_fwd_ftab(
 // The FP64 residual (I - d) / noise and per-tile r^T r of the tabulated
 // features' tiles (the LM's trial forward pass on dense crops): each pixel
 // interpolates its feature's table (k_fwd_ftab), kept in LDS while it fits
-// (LMZ_FTAB_LDS rows, restaged when the block's feature changes), and
-// projects and squares as k_forward<2>.  Persistent grid over a tile list;
-// the residual goes to out + (sel[f] ^ flip) * out_stride.
+// (LMZ_FTAB_LDS rows, or a window of them), and projects and squares as
+// k_forward<2>.  Persistent grid over a list of groups of LMZ_FTAB_GROUP
+// consecutive tiles of a feature (as k_jacobian_rtab's); each tile's r^T r
+// is reduced on its own.  The residual goes to out + (sel[f] ^ flip) *
+// out_stride.
 template <int NT = LMZ_THREADS>
 __global__ __launch_bounds__(NT) void k_forward_ftab(
     const lmz_feature *__restrict__ feat,
@@ -659,13 +662,23 @@ __global__ __launch_bounds__(NT) void k_forward_ftab(
     const bool bohren = flags & LMZ_BOHREN;
     const FTabPar fp = fpar[f];
     const double *gtab = ftab + (long long)f * (LMZ_FTAB_CAP * LMZ_FTAB_STRIDE);
-    // the rows the tile reads: the whole table, or, for a table larger
-    // than LDS, the window of the tile's pixels
+    constexpr int G = LMZ_FTAB_GROUP * (LMZ_TILE / NT);   // pixels per thread
+    const int ng = min(LMZ_FTAB_GROUP, F.ntiles - (t - fu.tile0));
+    const int jp0 = (t - fu.tile0) * LMZ_TILE;
+    // every load of the group before its first store: a load issued after
+    // a store waits for it (one memory counter)
+    float dv[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        const int j = jp0 + q * NT + threadIdx.x;
+        dv[q] = (q < ng * (LMZ_TILE / NT) && j < fu.npix) ? data[fu.pix_offset + j] : 0.f;
+    }
+    // the rows the group reads: the whole table, or, for a table larger
+    // than LDS, the window of the group's pixels
     int lo = 0, hi = fp.m - 1;
     if (fp.m > LMZ_FTAB_LDS) {
-        const int jp0 = (t - fu.tile0) * LMZ_TILE;
         table_rows<LMZ_FTAB_P, double>(fu.width, jp0,
-                                       min(jp0 + LMZ_TILE, fu.npix), fu.kd,
+                                       min(jp0 + ng * LMZ_TILE, fu.npix), fu.kd,
                                        (double)fu.ix + fu.fxd,
                                        (double)fu.iy + fu.fyd, fp.rho0,
                                        fp.inv_h, fp.m, lo, hi);
@@ -685,11 +698,13 @@ __global__ __launch_bounds__(NT) void k_forward_ftab(
         staged = f;
     }
     float *outf = out ? out + b * out_stride : nullptr;
-    double chi = 0.0;
-#pragma unroll 1
-    for (int q = 0; q < LMZ_TILE / NT; ++q) {
-        const int j = (t - fu.tile0) * LMZ_TILE + q * NT + threadIdx.x;
-        if (j >= fu.npix) break;
+    double chi[LMZ_FTAB_GROUP];
+#pragma unroll
+    for (int i = 0; i < LMZ_FTAB_GROUP; ++i) chi[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+        const int j = jp0 + q * NT + threadIdx.x;
+        if (q >= ng * (LMZ_TILE / NT) || j >= fu.npix) break;
         float ax, ay, az;
         pixel_coords(fu, j, coords, npix_total, ax, ay, az);
         const long long gi = fu.pix_offset + j;
@@ -700,13 +715,14 @@ __global__ __launch_bounds__(NT) void k_forward_ftab(
         cd E[3];
         field_from_sums(g, S, true, E);
         const double I = intensity(E, fu.alphad);
-        const double r = (I - (double)data[gi]) / F.noise;
+        const double r = (I - (double)dv[q]) / F.noise;
         if (outf) outf[gi] = (float)r;
-        chi = fma(r, r, chi);
+        double &c = chi[q / (LMZ_TILE / NT)];
+        c = fma(r, r, c);
     }
-    double v[1] = {chi};
-    block_reduce_store_d<1, NT>(v, chi2_tiles + t);
-    __syncthreads();   // the reduction buffer is reused by the next tile
+    // r^T r of each of the group's tiles, summed as a tile of its own
+    block_reduce_store_d<LMZ_FTAB_GROUP, NT>(chi, chi2_tiles + t, ng);
+    __syncthreads();   // the reduction buffer is reused by the next group
   }
 }
 
@@ -1837,7 +1853,7 @@ int lmz_residual_tabulated(const lmz_batch *b, const double *params,
                         grid_f, start};
     launch_fwd_ftab(b, params, nullptr, 0, nmax, coef, flags, nullptr, fpar,
                     ftab, mask_t, mask_l, stream);
-    launch_tile_list(b, mask_t, tl_t, stream);
+    launch_tile_list(b, mask_t, tl_t, stream, LMZ_FTAB_GROUP);
     launch_forward_ftab(b, params, nullptr, 0, nmax, flags, data, resid, 0LL,
                         chi2_tiles, fpar, ftab, &tl_t, stream);
     launch_tile_list(b, mask_l, tl_l, stream);

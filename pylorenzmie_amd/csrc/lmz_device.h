@@ -183,9 +183,11 @@ __device__ __forceinline__ void block_reduce_store(float (&v)[NV],
     }
 }
 
+// out[i] = block sum of v[i] for i < n (n <= NV); each sum as for NV = 1.
 template <int NV, int NT = LMZ_THREADS>
 __device__ __forceinline__ void block_reduce_store_d(double (&v)[NV],
-                                                     double *__restrict__ out) {
+                                                     double *__restrict__ out,
+                                                     int n = NV) {
     __shared__ double red[NT / 64][NV];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -195,7 +197,7 @@ __device__ __forceinline__ void block_reduce_store_d(double (&v)[NV],
         if (lane == 0) red[wave][i] = s;
     }
     __syncthreads();
-    if (threadIdx.x < NV) {
+    if (threadIdx.x < n) {
         double s = 0.0;
 #pragma unroll
         for (int w = 0; w < NT / 64; ++w) s += red[w][threadIdx.x];

@@ -322,7 +322,7 @@ static void enqueue_forward(const lmz_lm_plan *plan, const LMLayout &L,
                          count + 64, plan->grid_f, tile_start};
     launch_fwd_ftab(b, params, sel, flip, nmax, coef, flags, mask, fpar, ftab,
                     mask_ft, mask_fl, s);
-    launch_tile_list(b, mask_ft, tl_ft, s);
+    launch_tile_list(b, mask_ft, tl_ft, s, LMZ_FTAB_GROUP);
     launch_forward_ftab(b, params, sel, flip, nmax, flags, data, resid, np,
                         chi2_tiles, fpar, ftab, &tl_ft, s);
     launch_tile_list(b, mask_fl, tl_fl, s);

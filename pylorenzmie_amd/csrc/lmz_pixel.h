@@ -919,7 +919,7 @@ LMZ_HD void jacobian_pixel_rtab(const FeatUniform &fu, const PixGeom &g,
 #define LMZ_FTAB_P 12        // interpolation points
 #define LMZ_FTAB_HW 0.25     // h N / |Z|
 #define LMZ_FTAB_LDS 640     // rows of a table the forward kernel keeps in LDS
-#define LMZ_FTAB_GROUP 4     // tiles of a feature per work item of that kernel
+#define LMZ_FTAB_GROUP 8     // tiles of a feature per work item of that kernel
 
 struct FTabPar {
     double rho0, h, inv_h;   // grid k rho_j = rho0 + j h

@@ -442,13 +442,25 @@ __global__ __launch_bounds__(NT) void k_jacobian(
   }
 }
 
+// Orders whose coefficients the table builders stage in LDS (a feature of
+// more orders reads them from global memory): every sample's order loop
+// reads each order's coefficients, one dependent load per order.
+#define LMZ_TAB_NSTAGE 128
+#define LMZ_TAB_THREADS 256
+
+// Copy n floats from global memory to LDS (the whole block).
+__device__ __forceinline__ void stage_floats(float *dst, const float *src, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
 // Radial tables of the Jacobian's sums (lmz_pixel.h, rtab_params): one
-// thread per sample, 64 samples per block, blockIdx.y the sample block;
+// thread per sample, LMZ_TAB_THREADS samples per block, blockIdx.y the
+// sample block, the feature's coefficients staged in LDS;
 // thread 0 of the first block writes the feature's grid (m = 0 when the
 // feature is not tabulated) and sorts the active features into mask_tab
 // (k_jacobian_rtab) and mask_loop (k_jacobian).  The coefficient tables are
 // those k_jacobian reads (buffer sel[f] ^ flip).
-__global__ __launch_bounds__(64) void k_jac_rtab(
+__global__ __launch_bounds__(LMZ_TAB_THREADS) void k_jac_rtab(
     const lmz_feature *__restrict__ feat, const double *__restrict__ params,
     const int32_t *__restrict__ sel, int flip, int nbuf,
     const int32_t *__restrict__ nmax, const float *__restrict__ coef,
@@ -473,12 +485,24 @@ __global__ __launch_bounds__(64) void k_jac_rtab(
         mask_tab[f] = rp.m > 0 ? 1 : 0;
         mask_loop[f] = rp.m > 0 ? 0 : 1;
     }
-    const int j = blockIdx.y * 64 + threadIdx.x;
+    if ((int)blockIdx.y * LMZ_TAB_THREADS >= rp.m) return;   // the whole block
+    __shared__ float sc[LMZ_TAB_NSTAGE * LMZ_COEF_STRIDE];
+    __shared__ float sd[LMZ_TAB_NSTAGE * LMZ_DCOEF_STRIDE];
+    const float *c = coef + (long long)LMZ_COEF_STRIDE * fu.slot0;
+    const float *d = dcoef + (long long)LMZ_DCOEF_STRIDE * fu.slot0;
+    const bool lds = fu.nmax <= LMZ_TAB_NSTAGE;
+    if (lds) {
+        stage_floats(sc, c, fu.nmax * LMZ_COEF_STRIDE);
+        stage_floats(sd, d, fu.nmax * LMZ_DCOEF_STRIDE);
+        __syncthreads();
+    }
+    const int j = blockIdx.y * LMZ_TAB_THREADS + threadIdx.x;
     if (j >= rp.m) return;
-    rtab_sample(fu, rp, flags & LMZ_BOHREN, j,
-                coef + (long long)LMZ_COEF_STRIDE * fu.slot0,
-                dcoef + (long long)LMZ_DCOEF_STRIDE * fu.slot0,
-                rtab + ((long long)f * LMZ_RTAB_CAP + j) * LMZ_RTAB_STRIDE);
+    float *out = rtab + ((long long)f * LMZ_RTAB_CAP + j) * LMZ_RTAB_STRIDE;
+    if (lds)
+        rtab_sample(fu, rp, flags & LMZ_BOHREN, j, sc, sd, out);
+    else
+        rtab_sample(fu, rp, flags & LMZ_BOHREN, j, c, d, out);
 }
 
 // Normal-equation partials of the tabulated features' tiles (the LM's
@@ -594,7 +618,7 @@ __global__ __launch_bounds__(NT) void k_jacobian_rtab(
 // block writes the feature's grid and sorts the active features into
 // mask_tab (k_forward_ftab) and mask_loop (k_forward).  Coefficient buffer
 // sel[f] ^ flip, as the residual kernel reads.
-__global__ __launch_bounds__(64) void k_fwd_ftab(
+__global__ __launch_bounds__(LMZ_TAB_THREADS) void k_fwd_ftab(
     const lmz_feature *__restrict__ feat, const double *__restrict__ params,
     const int32_t *__restrict__ sel, int flip, int nbuf,
     const int32_t *__restrict__ nmax, const float *__restrict__ coef,
@@ -618,11 +642,21 @@ __global__ __launch_bounds__(64) void k_fwd_ftab(
         mask_tab[f] = fp.m > 0 ? 1 : 0;
         mask_loop[f] = fp.m > 0 ? 0 : 1;
     }
-    const int j = blockIdx.y * 64 + threadIdx.x;
+    if ((int)blockIdx.y * LMZ_TAB_THREADS >= fp.m) return;   // the whole block
+    __shared__ float sc[LMZ_TAB_NSTAGE * LMZ_COEF_STRIDE];
+    const float *c = coef + (long long)LMZ_COEF_STRIDE * fu.slot0;
+    const bool lds = fu.nmax <= LMZ_TAB_NSTAGE;
+    if (lds) {
+        stage_floats(sc, c, fu.nmax * LMZ_COEF_STRIDE);
+        __syncthreads();
+    }
+    const int j = blockIdx.y * LMZ_TAB_THREADS + threadIdx.x;
     if (j >= fp.m) return;
-    ftab_sample(fu, fp, flags & LMZ_BOHREN, j,
-                coef + (long long)LMZ_COEF_STRIDE * fu.slot0,
-                ftab + ((long long)f * LMZ_FTAB_CAP + j) * LMZ_FTAB_STRIDE);
+    double *out = ftab + ((long long)f * LMZ_FTAB_CAP + j) * LMZ_FTAB_STRIDE;
+    if (lds)
+        ftab_sample(fu, fp, flags & LMZ_BOHREN, j, sc, out);
+    else
+        ftab_sample(fu, fp, flags & LMZ_BOHREN, j, c, out);
 }
 
 // The FP64 residual (I - d) / noise and per-tile r^T r of the tabulated
@@ -1143,7 +1177,8 @@ void launch_jac_rtab(const lmz_batch *b, const double *params,
                      const uint8_t *active, RTabPar *rpar, float *rtab,
                      uint8_t *mask_tab, uint8_t *mask_loop,
                      hipStream_t stream) {
-    hipLaunchKernelGGL(k_jac_rtab, dim3(b->nfeat, LMZ_RTAB_CAP / 64), dim3(64),
+    hipLaunchKernelGGL(k_jac_rtab, dim3(b->nfeat, LMZ_RTAB_CAP / LMZ_TAB_THREADS),
+                       dim3(LMZ_TAB_THREADS),
                        0, stream, b->d_feat, params, sel, flip,
                        b->coef_buffers, nmax, coef, dcoef, flags, active, rpar,
                        rtab, mask_tab, mask_loop);
@@ -1154,7 +1189,8 @@ void launch_fwd_ftab(const lmz_batch *b, const double *params,
                      const float *coef, int flags, const uint8_t *active,
                      FTabPar *fpar, double *ftab, uint8_t *mask_tab,
                      uint8_t *mask_loop, hipStream_t stream) {
-    hipLaunchKernelGGL(k_fwd_ftab, dim3(b->nfeat, LMZ_FTAB_CAP / 64), dim3(64),
+    hipLaunchKernelGGL(k_fwd_ftab, dim3(b->nfeat, LMZ_FTAB_CAP / LMZ_TAB_THREADS),
+                       dim3(LMZ_TAB_THREADS),
                        0, stream, b->d_feat, params, sel, flip,
                        b->coef_buffers, nmax, coef, flags, active, fpar, ftab,
                        mask_tab, mask_loop);

@@ -281,11 +281,14 @@ def kernel_times(lm, settings, args):
     # sums, interpolated per pixel (k_fwd_ftab, k_forward_ftab)
     _lmz.check(lib.lmz_residual_tabulated_bytes(b.handle, ctypes.byref(nb)))
     fwork = torch.empty(nb.value // 8 + 64, dtype=torch.float64, device=dev)
+    # the residual written as the LM writes it (FP32, one per pixel)
+    fres = torch.empty_like(lm.data)
 
     def ftab():
         _lmz.check(lib.lmz_residual_tabulated(
             b.handle, _lmz.ptr(lm.params), _lmz.ptr(nmax), _lmz.ptr(coef),
-            flags, _lmz.ptr(lm.data), None, _lmz.ptr(chi2), _lmz.ptr(fwork), s))
+            flags, _lmz.ptr(lm.data), _lmz.ptr(fres), _lmz.ptr(chi2),
+            _lmz.ptr(fwork), s))
     jac()
     fwd()
     jtab()
@@ -311,7 +314,7 @@ def kernel_times(lm, settings, args):
     tf = ev[1].elapsed_time(ev[2]) / reps * 1e-3
     tt = et[0].elapsed_time(et[1]) / reps * 1e-3
     tft = et[1].elapsed_time(et[2]) / reps * 1e-3
-    del tabwork, fwork
+    del tabwork, fwork, fres
     nm = nmax.cpu().numpy()[0::2].astype(float)
     P = np.array(lm.npix, float)
     # radial table samples per launch, as lmz_pixel.h's rtab_params and
@@ -1302,9 +1305,10 @@ def main():
     # J reads the data (4 B / pixel) and writes 36 FP64 partials per tile
     bytes_j = 4. * npx + 36 * 8. * lm.batch.ntiles
     # reductions: J^T J from J in HBM (32 B / pixel in, 288 B / feature
-    # out); the residual chi2 (4 B / pixel in, one FP64 per tile out)
+    # out); the residual and chi2 (4 B / pixel in, the FP32 residual and
+    # one FP64 per tile out)
     bytes_n = 32. * npx + 36 * 8. * lm.batch.nfeat
-    bytes_r = 4. * npx + 8. * lm.batch.ntiles
+    bytes_r = 8. * npx + 8. * lm.batch.ntiles
     reductions = {
         'normal_eq': {'kernel': 'k_normal_eq', 'bound': 'hbm',
                       'achieved': bytes_n / tn / 1e9, 'peak': PEAK_HBM,
@@ -1346,11 +1350,12 @@ def main():
     issue_ft = (lane_ft / tft) if lane_ft else None
     fp64_lane = probe['fp64_fma_lane']['lane_instructions_per_s']
     attainable['fp64_lane_instructions_per_s'] = fp64_lane
-    # the tabulated forward reads the data (4 B / pixel), writes one FP64
-    # chi2 partial per tile and its tables (48 B a sample, read back once a
-    # feature's tile run); the tabulated Jacobian reads the data and writes
-    # 36 FP64 partials per tile and its tables (128 B a sample)
-    bytes_ft = 4. * npx + 8. * lm.batch.ntiles + 2 * 48. * tables['forward_samples']
+    # the tabulated forward reads the data and writes the FP32 residual
+    # (4 + 4 B / pixel), one FP64 chi2 partial per tile and its tables (48 B
+    # a sample, read back once a feature's tile run); the tabulated Jacobian
+    # reads the data and writes 36 FP64 partials per tile and its tables
+    # (128 B a sample)
+    bytes_ft = 8. * npx + 8. * lm.batch.ntiles + 2 * 48. * tables['forward_samples']
     bytes_jt = 4. * npx + 36 * 8. * lm.batch.ntiles + 2 * 128. * tables['jacobian_samples']
     fd = None
     fdb = None

@@ -507,8 +507,10 @@ __global__ __launch_bounds__(LMZ_TAB_THREADS) void k_jac_rtab(
 
 // Normal-equation partials of the tabulated features' tiles (the LM's
 // Jacobian pass on dense crops): each pixel interpolates the radial table
-// of its feature (k_jac_rtab), kept in LDS while it fits (LMZ_RTAB_LDS
-// rows, restaged when the block's feature changes), and projects as
+// of its feature (k_jac_rtab) from LDS: the whole table when it fits in
+// LMZ_RTAB_LDS rows, else the window of rows its group of tiles reads
+// (table_rows), restaged only when a group needs rows the block does not
+// hold; global memory when even the window is larger.  Projects as
 // k_jacobian.  Persistent grid over a device-built tile list; residuals
 // from rin (the FP64 forward pass) or data.  The partials stay in registers:
 // without the order loop the kernel has them to spare.
@@ -661,8 +663,9 @@ __global__ __launch_bounds__(LMZ_TAB_THREADS) void k_fwd_ftab(
 
 // The FP64 residual (I - d) / noise and per-tile r^T r of the tabulated
 // features' tiles (the LM's trial forward pass on dense crops): each pixel
-// interpolates its feature's table (k_fwd_ftab), kept in LDS while it fits
-// (LMZ_FTAB_LDS rows, or a window of them), and projects and squares as
+// interpolates its feature's table (k_fwd_ftab) from LDS as
+// k_jacobian_rtab does (whole table, window, or global memory; LMZ_FTAB_LDS
+// rows), and projects and squares as
 // k_forward<2>.  Persistent grid over a list of groups of LMZ_FTAB_GROUP
 // consecutive tiles of a feature (as k_jacobian_rtab's); each tile's r^T r
 // is reduced on its own.  The residual goes to out + (sel[f] ^ flip) *

@@ -699,16 +699,17 @@ __global__ __launch_bounds__(NT) void k_forward_ftab(
     const bool bohren = flags & LMZ_BOHREN;
     const FTabPar fp = fpar[f];
     const double *gtab = ftab + (long long)f * (LMZ_FTAB_CAP * LMZ_FTAB_STRIDE);
-    constexpr int G = LMZ_FTAB_GROUP * (LMZ_TILE / NT);   // pixels per thread
+    constexpr int PT = LMZ_TILE / NT;   // pixels per thread and tile
     const int ng = min(LMZ_FTAB_GROUP, F.ntiles - (t - fu.tile0));
     const int jp0 = (t - fu.tile0) * LMZ_TILE;
-    // every load of the group before its first store: a load issued after
-    // a store waits for it (one memory counter)
-    float dv[G];
+    // the data of a tile are loaded one tile ahead, before the previous
+    // tile's residual stores: a load issued after a store waits for it
+    // (one memory counter)
+    float dcur[PT];
 #pragma unroll
-    for (int q = 0; q < G; ++q) {
-        const int j = jp0 + q * NT + threadIdx.x;
-        dv[q] = (q < ng * (LMZ_TILE / NT) && j < fu.npix) ? data[fu.pix_offset + j] : 0.f;
+    for (int p = 0; p < PT; ++p) {
+        const int j = jp0 + p * NT + threadIdx.x;
+        dcur[p] = j < fu.npix ? data[fu.pix_offset + j] : 0.f;
     }
     // the rows the group reads: the whole table, or, for a table larger
     // than LDS, the window of the group's pixels
@@ -735,31 +736,44 @@ __global__ __launch_bounds__(NT) void k_forward_ftab(
         staged = f;
     }
     float *outf = out ? out + b * out_stride : nullptr;
-    double chi[LMZ_FTAB_GROUP];
+    // one pixel at a time (registers for more waves); each tile's r^T r
+    // reduced at its end, as a tile of its own
+#pragma unroll 1
+    for (int q = 0; q < ng; ++q) {
+        float dnext[PT];
 #pragma unroll
-    for (int i = 0; i < LMZ_FTAB_GROUP; ++i) chi[i] = 0.0;
+        for (int p = 0; p < PT; ++p) {
+            const int j = jp0 + (q + 1) * LMZ_TILE + p * NT + threadIdx.x;
+            dnext[p] = (q + 1 < ng && j < fu.npix) ? data[fu.pix_offset + j] : 0.f;
+        }
+        double chi = 0.0;
+#pragma unroll 1
+        for (int p = 0; p < PT; ++p) {
+            const int j = jp0 + q * LMZ_TILE + p * NT + threadIdx.x;
+            if (j >= fu.npix) break;
+            float dq = 0.f;   // dcur[p] by unrolled selects, not a run-time index
 #pragma unroll
-    for (int q = 0; q < G; ++q) {
-        const int j = jp0 + q * NT + threadIdx.x;
-        if (q >= ng * (LMZ_TILE / NT) || j >= fu.npix) break;
-        float ax, ay, az;
-        pixel_coords(fu, j, coords, npix_total, ax, ay, az);
-        const long long gi = fu.pix_offset + j;
-        PixGeomD g;
-        pixel_geometry_d(fu, ax, ay, az, bohren, g);
-        const SumsD S = lds ? forward_sum_ftab<LMZ_FTAB_STRIDE>(g, fp, stab, sb)
-                            : forward_sum_ftab<LMZ_FTAB_STRIDE>(g, fp, gtab);
-        cd E[3];
-        field_from_sums(g, S, true, E);
-        const double I = intensity(E, fu.alphad);
-        const double r = (I - (double)dv[q]) / F.noise;
-        if (outf) outf[gi] = (float)r;
-        double &c = chi[q / (LMZ_TILE / NT)];
-        c = fma(r, r, c);
+            for (int i = 0; i < PT; ++i) dq = (i == p) ? dcur[i] : dq;
+            float ax, ay, az;
+            pixel_coords(fu, j, coords, npix_total, ax, ay, az);
+            const long long gi = fu.pix_offset + j;
+            PixGeomD g;
+            pixel_geometry_d(fu, ax, ay, az, bohren, g);
+            const SumsD S = lds ? forward_sum_ftab<LMZ_FTAB_STRIDE>(g, fp, stab, sb)
+                                : forward_sum_ftab<LMZ_FTAB_STRIDE>(g, fp, gtab);
+            cd E[3];
+            field_from_sums(g, S, true, E);
+            const double I = intensity(E, fu.alphad);
+            const double r = (I - (double)dq) / F.noise;
+            if (outf) outf[gi] = (float)r;
+            chi = fma(r, r, chi);
+        }
+        double v[1] = {chi};
+        block_reduce_store_d<1, NT>(v, chi2_tiles + t + q);
+        __syncthreads();   // the reduction buffer is reused by the next tile
+#pragma unroll
+        for (int p = 0; p < PT; ++p) dcur[p] = dnext[p];
     }
-    // r^T r of each of the group's tiles, summed as a tile of its own
-    block_reduce_store_d<LMZ_FTAB_GROUP, NT>(chi, chi2_tiles + t, ng);
-    __syncthreads();   // the reduction buffer is reused by the next group
   }
 }
 

@@ -62,19 +62,44 @@ def test_tabulated_and_loop_jacobians_reach_the_same_minimum(fx):
     assert np.all(np.abs(chi_t - chi_l) <= 1e-8 * chi_l)
 
 
-def test_tabulated_partials_match_the_loop(fx):
+# Large spheres close to the focal plane: Jacobian tables of 500-650 rows
+# and forward tables of 700-900 (more than the kernels keep in LDS), read
+# through LDS windows where a tile group's rows fit and from global memory
+# where they do not.
+LARGE_TABLES = np.array([(100.3, 99.6, 50., 3.0, 1.6, 0.0, 1.0),
+                         (99.7, 100.4, 60., 2.8, 1.55, 0.002, 0.9),
+                         (100.0, 100.0, 55., 3.0, 1.45, 0.0, 1.1),
+                         (130.0, 70.0, 52., 2.9, 1.6, 0.0, 1.0)])
+
+
+def _inputs(fx, which, side):
+    """Parameters and data of the entry-point comparisons: the headline
+    fixture's first eight starts and data, or LARGE_TABLES on seeded
+    synthetic data (both paths see the same data)."""
+    if which == 'headline':
+        return (np.asarray(fx['head_start'][:8]),
+                np.asarray(fx['head_data'][:8], np.float32).reshape(-1))
+    rng = np.random.default_rng(5)
+    d = 1. + 0.05 * rng.standard_normal(len(LARGE_TABLES) * side * side)
+    return LARGE_TABLES, d.astype(np.float32)
+
+
+@pytest.mark.parametrize('which', ['headline', 'large_tables'])
+def test_tabulated_partials_match_the_loop(fx, which):
     """lmz_jacobian_tabulated (tables of the sums, interpolated) against
     lmz_jacobian (the order loop at every pixel) on eight 201 x 201 crops:
     per feature, J^T J and J^T r within 2e-5 of the scale sqrt(A_ii A_jj)
-    and sqrt(A_ii r^T r), r^T r within 1e-5 (both take the residual from
-    their FP32 intensity)."""
+    and sqrt(A_ii r^T r) (1e-4 for LARGE_TABLES, read through LDS windows
+    and global memory: FP32 rounding grows with the order count), r^T r
+    within 1e-5 (both take the residual from their FP32 intensity)."""
     import torch
     from pylorenzmie_amd import _lmz
     from pylorenzmie_amd.theory import Instrument
     from pylorenzmie_amd.theory.batch import HologramBatch
     inst = Instrument(noise=float(fx['head_sigma']), **INSTRUMENT)
-    n, side = 8, 201
-    q = np.asarray(fx['head_start'][:n])
+    side = 201
+    q, d = _inputs(fx, which, side)
+    n = len(q)
     caps = [_lmz.nmax_cap(p[3], p[4], p[5], inst.n_m, inst.wavelength) for p in q]
     hb = HologramBatch([(side, side)] * n, caps, None, inst)
     b = hb.batch
@@ -84,7 +109,7 @@ def test_tabulated_partials_match_the_loop(fx):
     nmax = torch.zeros(n, dtype=torch.int32, device='cuda')
     coef = torch.zeros(b.coef_floats(), dtype=torch.float32, device='cuda')
     dcoef = torch.zeros(b.dcoef_floats(), dtype=torch.float32, device='cuda')
-    data = torch.from_numpy(np.asarray(fx['head_data'][:n], np.float32).reshape(-1)).cuda()
+    data = torch.from_numpy(d).cuda()
     s = _lmz.stream_handle()
     _lmz.check(lib.lmz_mie_ab(b.handle, _lmz.ptr(pp), None, _lmz.ptr(nmax),
                               _lmz.ptr(coef), _lmz.ptr(dcoef), None,
@@ -115,11 +140,15 @@ def test_tabulated_partials_match_the_loop(fx):
         err = np.abs(T[f, :28] - L[f, :28]) / scale[iu]
         errg = np.abs(T[f, 28:35] - L[f, 28:35]) / (d * np.sqrt(L[f, 35]))
         print(f, err.max(), errg.max())
-        assert err.max() <= 2e-5 and errg.max() <= 2e-5
+        # FP32 sums over 70-75 orders for the large spheres: up to 2.6e-5
+        # of scale there (measured), under 1e-5 on the headline starts
+        bound = 2e-5 if which == 'headline' else 1e-4
+        assert err.max() <= bound and errg.max() <= bound
         assert T[f, 35] == pytest.approx(L[f, 35], rel=1e-5)
 
 
-def test_tabulated_residual_matches_the_loop(fx):
+@pytest.mark.parametrize('which', ['headline', 'large_tables'])
+def test_tabulated_residual_matches_the_loop(fx, which):
     """lmz_residual_tabulated (FP64 tables of the sums, interpolated) against
     lmz_hologram's residual (the order loop at every pixel) on eight
     201 x 201 crops: the residuals to one FP32 ulp (the store of r = (I - d) /
@@ -130,8 +159,9 @@ def test_tabulated_residual_matches_the_loop(fx):
     from pylorenzmie_amd.theory import Instrument
     from pylorenzmie_amd.theory.batch import HologramBatch
     inst = Instrument(noise=float(fx['head_sigma']), **INSTRUMENT)
-    n, side = 8, 201
-    q = np.asarray(fx['head_start'][:n])
+    side = 201
+    q, d = _inputs(fx, which, side)
+    n = len(q)
     caps = [_lmz.nmax_cap(p[3], p[4], p[5], inst.n_m, inst.wavelength) for p in q]
     hb = HologramBatch([(side, side)] * n, caps, None, inst)
     b = hb.batch
@@ -140,7 +170,7 @@ def test_tabulated_residual_matches_the_loop(fx):
     pp[:, :7] = torch.from_numpy(q[:, :7])
     nmax = torch.zeros(n, dtype=torch.int32, device='cuda')
     coef = torch.zeros(b.coef_floats(), dtype=torch.float32, device='cuda')
-    data = torch.from_numpy(np.asarray(fx['head_data'][:n], np.float32).reshape(-1)).cuda()
+    data = torch.from_numpy(d).cuda()
     s = _lmz.stream_handle()
     _lmz.check(lib.lmz_mie_ab(b.handle, _lmz.ptr(pp), None, _lmz.ptr(nmax),
                               _lmz.ptr(coef), None, None, _lmz.ptr(hb.work), s))
